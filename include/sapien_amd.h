@@ -1,0 +1,142 @@
+/* sapien_amd C ABI — MI355X (gfx950) articulated-body kinematics and dynamics.
+ *
+ * Drop-in boundary for SAPIEN's PinocchioModel path. Every entry point
+ * returns an int status (SAPAMD_OK = 0); on failure the message is available
+ * from sapamd_last_error() (thread-local). No C++ exception crosses the ABI,
+ * no torch type appears in a signature, and the library never frees a buffer
+ * it did not allocate.
+ *
+ * Two families:
+ *   - per-call entry points with the reference's method semantics
+ *     (host arrays, FP64, one configuration per call, results cached in the
+ *     model exactly where the reference caches them in pinocchio::Data);
+ *   - batched entry points over caller-owned device buffers (FP32, row-major,
+ *     contiguous, user dof order), enqueued on a caller-supplied hipStream_t
+ *     passed as void* (nullptr = the null stream).
+ *
+ * Layouts (batched): q/qd/qdd/tau (B, nv); link poses (B, L, 7) as
+ * p xyz | quaternion wxyz; full Jacobian (B, 6, nv) with rows
+ * [linear; angular] in the WORLD frame (linear rows = velocity of the point at
+ * the world origin); mass/Coriolis matrices (B, nv, nv).
+ *
+ * Reference files are cited relative to the SAPIEN repository root.
+ */
+#ifndef SAPIEN_AMD_H
+#define SAPIEN_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SAPAMD_OK = 0,
+  SAPAMD_INVALID_ARGUMENT = 1, /* Python maps to ValueError   (pinocchio_model.cpp:105,122,136) */
+  SAPAMD_RUNTIME_ERROR = 2,    /* Python maps to RuntimeError (pinocchio_model.cpp:33-36,64-66,70,113,151) */
+  SAPAMD_DEVICE_ERROR = 3      /* HIP runtime failure */
+};
+
+typedef struct sapamd_model sapamd_model;
+
+typedef struct {
+  int32_t nv;          /* user dofs (model.nv) */
+  int32_t nq;          /* configuration size with continuous joints as (cos, sin) (model.nq) */
+  int32_t njoints;     /* movable joints, universe excluded */
+  int32_t nlinks;      /* links given to set_link_order */
+  int32_t max_depth;   /* longest root-to-leaf joint chain */
+  int32_t chain;       /* 1 when the joints form one serial chain */
+  int32_t supported;   /* 0 when the URDF holds floating/planar joints */
+  int32_t order_set;   /* 1 after a successful set_joint_order */
+  int32_t device;      /* HIP device holding the model tables */
+} sapamd_info;
+
+/* Thread-local message of the last failed call on this thread. */
+const char *sapamd_last_error(void);
+/* Library version string. */
+const char *sapamd_version(void);
+
+/* ---------------------------------------------------------------- model */
+/* PinocchioModel(urdf, gravity): pinocchio_model.cpp:39-47 (fromURDFXML), :310-312.
+ * Builds the model on the current HIP device. */
+int sapamd_model_create(const char *urdf, const double gravity[3], sapamd_model **out);
+void sapamd_model_destroy(sapamd_model *m);
+int sapamd_model_info(const sapamd_model *m, sapamd_info *out);
+/* set_joint_order: pinocchio_model.cpp:99-128. */
+int sapamd_set_joint_order(sapamd_model *m, const char *const *names, int32_t n);
+/* set_link_order: pinocchio_model.cpp:130-140. */
+int sapamd_set_link_order(sapamd_model *m, const char *const *names, int32_t n);
+/* Joint limits in the user order (lower, upper); continuous joints give (-pi, pi). */
+int sapamd_joint_limits(const sapamd_model *m, double *lower, double *upper);
+/* Per user dof: 1 when the joint is continuous (returned q is atan2-wrapped). */
+int sapamd_continuous_mask(const sapamd_model *m, int32_t *out);
+
+/* ---------------------------------------------------------------- per-call (reference semantics, FP64) */
+/* compute_forward_kinematics: pinocchio_model.cpp:146-148 (caches every link pose). */
+int sapamd_compute_forward_kinematics(sapamd_model *m, const double *qpos, int64_t n);
+/* get_link_pose: pinocchio_model.cpp:150-161 -> p xyz | q wxyz, FP32 like sapien::Pose. */
+int sapamd_get_link_pose(const sapamd_model *m, int32_t link, float pose7[7]);
+/* compute_full_jacobian: pinocchio_model.cpp:163-165. */
+int sapamd_compute_full_jacobian(sapamd_model *m, const double *qpos, int64_t n);
+/* get_link_jacobian: pinocchio_model.cpp:167-186; local != 0 gives the body (link)
+ * frame Jacobian, as the reference code does (its docstring at :355-362 is inverted).
+ * out: 6 x nv row-major. */
+int sapamd_get_link_jacobian(const sapamd_model *m, int32_t link, int32_t local, double *out);
+/* compute_single_link_local_jacobian: pinocchio_model.cpp:188-199. out 6 x nv. */
+int sapamd_compute_single_link_local_jacobian(sapamd_model *m, const double *qpos, int64_t n, int32_t link,
+                                              double *out);
+/* compute_generalized_mass_matrix: pinocchio_model.cpp:201-206 (full symmetric). */
+int sapamd_compute_generalized_mass_matrix(sapamd_model *m, const double *qpos, int64_t n, double *out);
+/* compute_coriolis_matrix: pinocchio_model.cpp:208-212 (Christoffel-consistent C). */
+int sapamd_compute_coriolis_matrix(sapamd_model *m, const double *qpos, const double *qvel, int64_t n,
+                                   double *out);
+/* compute_inverse_dynamics: pinocchio_model.cpp:214-219 (RNEA). */
+int sapamd_compute_inverse_dynamics(sapamd_model *m, const double *qpos, const double *qvel, const double *qacc,
+                                    int64_t n, double *out);
+/* compute_forward_dynamics: pinocchio_model.cpp:221-226 (ABA). */
+int sapamd_compute_forward_dynamics(sapamd_model *m, const double *qpos, const double *qvel, const double *qf,
+                                    int64_t n, double *out);
+/* compute_inverse_kinematics: pinocchio_model.cpp:228-300. pose7 = p xyz | q wxyz;
+ * initial_qpos may be NULL (n0 = 0: neutral); active_qmask may be NULL (nm = 0:
+ * all active). Outputs q (nv), success, best error (6). */
+int sapamd_compute_inverse_kinematics(sapamd_model *m, int32_t link, const float pose7[7],
+                                      const double *initial_qpos, int64_t n0, const int32_t *active_qmask,
+                                      int64_t nm, double eps, int32_t max_iterations, double dt, double damp,
+                                      double *q_out, int32_t *success, double err6[6]);
+
+/* ---------------------------------------------------------------- batched device path (FP32) */
+/* Fused FK (every link pose) + full WORLD Jacobian + RNEA over B environments;
+ * the throughput-critical entry point. Any of poses/J/tau may be NULL to skip it. */
+int sapamd_fk_jac_rnea_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd,
+                               const float *qdd, float *poses, float *J, float *tau, void *stream);
+/* Batched compute_forward_kinematics + get_link_pose for all links: poses (B, L, 7). */
+int sapamd_fk_batched(const sapamd_model *m, int64_t B, const float *q, float *poses, void *stream);
+/* Batched compute_full_jacobian: (B, 6, nv). */
+int sapamd_jacobian_batched(const sapamd_model *m, int64_t B, const float *q, float *J, void *stream);
+/* Batched get_link_jacobian for every link (world and body frame): (B, L, 6, nv) each; either may be NULL. */
+int sapamd_link_jacobians_batched(const sapamd_model *m, int64_t B, const float *q, float *J_world, float *J_local,
+                                  void *stream);
+/* Batched compute_inverse_dynamics: tau (B, nv). */
+int sapamd_rnea_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd, const float *qdd,
+                        float *tau, void *stream);
+/* Batched compute_generalized_mass_matrix: (B, nv, nv). */
+int sapamd_crba_batched(const sapamd_model *m, int64_t B, const float *q, float *M, void *stream);
+/* Batched compute_coriolis_matrix: (B, nv, nv). */
+int sapamd_coriolis_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd, float *C,
+                            void *stream);
+/* Batched compute_forward_dynamics (ABA): qdd (B, nv). */
+int sapamd_aba_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd, const float *tau,
+                       float *qdd, void *stream);
+/* Batched compute_inverse_kinematics for B problems on one link: target (B, 7),
+ * q0 (B, nv) or NULL (neutral), mask (nv) or NULL; outputs q (B, nv),
+ * success (B), err (B, 6), iterations (B, may be NULL). Problems exit
+ * independently; a persistent launch refills finished lanes from a work queue. */
+int sapamd_clik_batched(const sapamd_model *m, int64_t B, int32_t link, const float *target, const float *q0,
+                        const int32_t *mask, double eps, int32_t max_iterations, double dt, double damp,
+                        float *q_out, int32_t *success, float *err, int32_t *iterations, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SAPIEN_AMD_H */

@@ -1,0 +1,655 @@
+// C ABI implementation (declared in include/sapien_amd.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/sapien_amd.h"
+#include "launch.h"
+#include "model.h"
+
+using namespace sapamd;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIP_OK(expr)                                                                            \
+  do {                                                                                          \
+    hipError_t _e = (expr);                                                                     \
+    if (_e != hipSuccess) throw DeviceError(std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+template <class F> int guarded(F &&f) {
+  try {
+    f();
+    return SAPAMD_OK;
+  } catch (const std::invalid_argument &e) {
+    g_last_error = e.what();
+    return SAPAMD_INVALID_ARGUMENT;
+  } catch (const DeviceError &e) {
+    g_last_error = e.what();
+    return SAPAMD_DEVICE_ERROR;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return SAPAMD_RUNTIME_ERROR;
+  }
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    HIP_OK(hipGetDevice(&prev));
+    if (prev != dev) HIP_OK(hipSetDevice(dev));
+  }
+  ~DeviceGuard() {
+    int cur;
+    if (hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+void skew2(const double a[3], double K[9]) {
+  K[0] = 0; K[1] = -a[2]; K[2] = a[1];
+  K[3] = a[2]; K[4] = 0; K[5] = -a[0];
+  K[6] = -a[1]; K[7] = a[0]; K[8] = 0;
+}
+void mm3(const double *A, const double *B, double *C) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
+}
+
+}  // namespace
+
+struct sapamd_model {
+  std::unique_ptr<ArticulatedModel> host;
+  int device = 0;
+  void *tab32 = nullptr, *tab64 = nullptr;
+  int32_t *d_chain = nullptr;   // every link's support chain, concatenated
+  std::vector<int> chain_off, chain_len;
+  int32_t *d_counter = nullptr;
+  hipStream_t stream = nullptr;
+  double *h_buf = nullptr, *d_buf = nullptr;
+  size_t buf_doubles = 0;
+  bool tables_ready = false;
+  std::vector<double> pose_cache, jw_cache, jl_cache;
+
+  ~sapamd_model() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (tab32) (void)hipFree(tab32);
+    if (tab64) (void)hipFree(tab64);
+    if (d_chain) (void)hipFree(d_chain);
+    if (d_counter) (void)hipFree(d_counter);
+    if (d_buf) (void)hipFree(d_buf);
+    if (h_buf) (void)hipHostFree(h_buf);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  int nv() const { return host->nv(); }
+  int L() const { return host->nlinks(); }
+
+  template <class T> std::vector<char> build_table() const {
+    const auto &J = host->joints();
+    const auto &F = host->frames();
+    const auto &lf = host->link_frames();
+    int nj = (int)J.size(), nl = (int)lf.size();
+    std::vector<char> buf(table_bytes<T>(nj, nl), 0);
+    auto *hdr = reinterpret_cast<ModelTable<T> *>(buf.data());
+    hdr->nj = nj;
+    hdr->nlinks = nl;
+    hdr->nv = host->nv();
+    hdr->chain = host->is_chain() ? 1 : 0;
+    hdr->max_depth = host->max_depth();
+    for (int i = 0; i < 3; ++i) hdr->gravity[i] = T(host->gravity()[i]);
+    auto *jr = const_cast<JointRec<T> *>(joints_of(hdr));
+    std::vector<int> nchild(nj, 0);
+    for (int j = 0; j < nj; ++j)
+      if (J[j].parent >= 0) nchild[J[j].parent]++;
+    for (int j = 0; j < nj; ++j) {
+      const Joint &jt = J[j];
+      JointRec<T> &r = jr[j];
+      double K[9], K2[9], B[9], C[9];
+      skew2(jt.axis, K);
+      mm3(K, K, K2);
+      mm3(jt.place.R, K, B);
+      mm3(jt.place.R, K2, C);
+      for (int k = 0; k < 9; ++k) {
+        r.A[k] = T(jt.place.R[k]);
+        r.Bq[k] = jt.kind == JK_PRISMATIC ? T(0) : T(B[k]);
+        r.Cq[k] = jt.kind == JK_PRISMATIC ? T(0) : T(C[k]);
+      }
+      for (int i = 0; i < 3; ++i) {
+        r.p[i] = T(jt.place.p[i]);
+        r.ap[i] = T(jt.place.R[3 * i] * jt.axis[0] + jt.place.R[3 * i + 1] * jt.axis[1] +
+                    jt.place.R[3 * i + 2] * jt.axis[2]);
+        r.axis[i] = T(jt.axis[i]);
+      }
+      double m, c[3], Ic[9];
+      jt.inertia.com_form(m, c, Ic);
+      r.m = T(m);
+      for (int i = 0; i < 3; ++i) r.c[i] = T(c[i]);
+      r.Ic[0] = T(Ic[0]); r.Ic[1] = T(Ic[1]); r.Ic[2] = T(Ic[2]);
+      r.Ic[3] = T(Ic[4]); r.Ic[4] = T(Ic[5]); r.Ic[5] = T(Ic[8]);
+      r.parent = jt.parent;
+      r.kind = jt.kind == JK_PRISMATIC ? KIND_PRISMATIC : jt.kind == JK_CONTINUOUS ? KIND_CONTINUOUS : KIND_REVOLUTE;
+      r.vuser = host->vuser()[j];
+      r.depth = jt.depth;
+      r.nchild = nchild[j];
+    }
+    auto *lr = const_cast<LinkRec<T> *>(links_of(hdr));
+    for (int l = 0; l < nl; ++l) {
+      const Frame &f = F[lf[l]];
+      bool ident = true;
+      for (int k = 0; k < 9; ++k) {
+        lr[l].R[k] = T(f.place.R[k]);
+        if (f.place.R[k] != (k % 4 == 0 ? 1.0 : 0.0)) ident = false;
+      }
+      for (int i = 0; i < 3; ++i) {
+        lr[l].p[i] = T(f.place.p[i]);
+        if (f.place.p[i] != 0.0) ident = false;
+      }
+      lr[l].joint = f.joint;
+      lr[l].ident = ident ? 1 : 0;
+    }
+    return buf;
+  }
+
+  // Rebuild device tables and per-call buffers after the orders change.
+  void refresh() {
+    tables_ready = false;
+    if (device < 0 || !host->supported() || !host->joint_order_set()) return;
+    if (host->njoints() > kMaxGenericJoints)
+      throw std::runtime_error("models with more than 64 movable joints are not supported");
+    DeviceGuard g(device);
+    HIP_OK(hipStreamSynchronize(stream));
+    auto t32 = build_table<float>();
+    auto t64 = build_table<double>();
+    if (tab32) HIP_OK(hipFree(tab32));
+    if (tab64) HIP_OK(hipFree(tab64));
+    tab32 = tab64 = nullptr;
+    HIP_OK(hipMalloc(&tab32, t32.size()));
+    HIP_OK(hipMalloc(&tab64, t64.size()));
+    HIP_OK(hipMemcpy(tab32, t32.data(), t32.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(tab64, t64.data(), t64.size(), hipMemcpyHostToDevice));
+    std::vector<int32_t> chains;
+    chain_off.assign(L(), 0);
+    chain_len.assign(L(), 0);
+    for (int l = 0; l < L(); ++l) {
+      int jl = host->frames()[host->link_frames()[l]].joint;
+      std::vector<int> c = jl >= 0 ? host->support(jl) : std::vector<int>{};
+      chain_off[l] = (int)chains.size();
+      chain_len[l] = (int)c.size();
+      chains.insert(chains.end(), c.begin(), c.end());
+    }
+    if (d_chain) HIP_OK(hipFree(d_chain));
+    d_chain = nullptr;
+    HIP_OK(hipMalloc(&d_chain, sizeof(int32_t) * (chains.size() + 1)));
+    if (!chains.empty())
+      HIP_OK(hipMemcpy(d_chain, chains.data(), chains.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    size_t n = nv(), l = L();
+    size_t need = 8 * n + 7 * l + 12 * n * l + 2 * n * n + 64;
+    if (need > buf_doubles) {
+      if (d_buf) HIP_OK(hipFree(d_buf));
+      if (h_buf) HIP_OK(hipHostFree(h_buf));
+      d_buf = nullptr;
+      h_buf = nullptr;
+      HIP_OK(hipMalloc(&d_buf, need * sizeof(double)));
+      HIP_OK(hipHostMalloc(&h_buf, need * sizeof(double), hipHostMallocDefault));
+      buf_doubles = need;
+    }
+    tables_ready = true;
+  }
+
+  void reset_caches() {
+    // pinocchio::Data starts with oMi = identity and J = 0: a link pose read
+    // before any compute is the link's placement in its joint frame.
+    size_t n = nv(), l = L();
+    pose_cache.assign(7 * l, 0.0);
+    for (size_t k = 0; k < l; ++k) {
+      const Frame &f = host->frames()[host->link_frames()[k]];
+      const double *R = f.place.R;
+      for (int i = 0; i < 3; ++i) pose_cache[7 * k + i] = f.place.p[i];
+      double t = R[0] + R[4] + R[8], *qq = &pose_cache[7 * k + 3];
+      if (t > 0) {
+        double r = std::sqrt(t + 1), h = 0.5 / r;
+        qq[0] = 0.5 * r; qq[1] = (R[7] - R[5]) * h; qq[2] = (R[2] - R[6]) * h; qq[3] = (R[3] - R[1]) * h;
+      } else {
+        int i = 0;
+        if (R[4] > R[0]) i = 1;
+        if (R[8] > R[3 * i + i]) i = 2;
+        int j = (i + 1) % 3, k2 = (j + 1) % 3;
+        double r = std::sqrt(R[3 * i + i] - R[3 * j + j] - R[3 * k2 + k2] + 1), h = 0.5 / r;
+        double xyz[3];
+        xyz[i] = 0.5 * r;
+        xyz[j] = (R[3 * j + i] + R[3 * i + j]) * h;
+        xyz[k2] = (R[3 * k2 + i] + R[3 * i + k2]) * h;
+        qq[0] = (R[3 * k2 + j] - R[3 * j + k2]) * h;
+        qq[1] = xyz[0]; qq[2] = xyz[1]; qq[3] = xyz[2];
+      }
+    }
+    jw_cache.assign(6 * n * l, 0.0);
+    jl_cache.assign(6 * n * l, 0.0);
+  }
+
+  void require_tables() const {
+    if (device < 0) throw DeviceError("no HIP device is available");
+    if (!host->supported())
+      throw std::runtime_error("Unsupported joint in computation. Currently support: fixed, revolute, prismatic");
+    if (!tables_ready) throw std::runtime_error("posS2P failed");
+  }
+  void check_link(int32_t link) const {
+    if (link < 0 || link >= L()) throw std::runtime_error("link index out of bound");
+  }
+  void check_vec(int64_t len, const char *what) const {
+    if (len != nv()) throw std::runtime_error(std::string(what) + " must have length nv");
+  }
+
+  // stage host doubles to the device buffer at `off` (in doubles)
+  double *stage(const double *src, size_t n, size_t off) {
+    std::memcpy(h_buf + off, src, n * sizeof(double));
+    HIP_OK(hipMemcpyAsync(d_buf + off, h_buf + off, n * sizeof(double), hipMemcpyHostToDevice, stream));
+    return d_buf + off;
+  }
+  void fetch(double *dst, size_t n, size_t off) {
+    HIP_OK(hipMemcpyAsync(h_buf + off, d_buf + off, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipStreamSynchronize(stream));
+    std::memcpy(dst, h_buf + off, n * sizeof(double));
+  }
+  const ModelTable<double> *t64() const { return static_cast<const ModelTable<double> *>(tab64); }
+  const ModelTable<float> *t32() const { return static_cast<const ModelTable<float> *>(tab32); }
+};
+
+extern "C" {
+
+const char *sapamd_last_error(void) { return g_last_error.c_str(); }
+const char *sapamd_version(void) { return "sapien_amd 0.1.0 (gfx950)"; }
+
+int sapamd_model_create(const char *urdf, const double gravity[3], sapamd_model **out) {
+  return guarded([&] {
+    if (!urdf || !gravity || !out) throw std::invalid_argument("null argument");
+    auto m = std::make_unique<sapamd_model>();
+    m->host = std::make_unique<ArticulatedModel>(std::string(urdf), gravity);
+    // Without a HIP device the model still parses and orders (host logic);
+    // every compute call then fails with a device error.
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+      (void)hipGetLastError();
+      m->device = -1;
+    } else {
+      HIP_OK(hipGetDevice(&m->device));
+      HIP_OK(hipStreamCreate(&m->stream));
+      HIP_OK(hipMalloc(&m->d_counter, sizeof(int32_t) * 4));
+    }
+    m->reset_caches();
+    *out = m.release();
+  });
+}
+
+void sapamd_model_destroy(sapamd_model *m) { delete m; }
+
+int sapamd_model_info(const sapamd_model *m, sapamd_info *out) {
+  return guarded([&] {
+    if (!m || !out) throw std::invalid_argument("null argument");
+    out->nv = m->host->nv();
+    out->nq = m->host->nq();
+    out->njoints = m->host->njoints();
+    out->nlinks = m->host->nlinks();
+    out->max_depth = m->host->max_depth();
+    out->chain = m->host->is_chain();
+    out->supported = m->host->supported();
+    out->order_set = m->host->joint_order_set();
+    out->device = m->device;
+  });
+}
+
+int sapamd_set_joint_order(sapamd_model *m, const char *const *names, int32_t n) {
+  return guarded([&] {
+    std::vector<std::string> v(names, names + n);
+    m->host->set_joint_order(v);
+    m->refresh();
+    m->reset_caches();
+  });
+}
+
+int sapamd_set_link_order(sapamd_model *m, const char *const *names, int32_t n) {
+  return guarded([&] {
+    std::vector<std::string> v(names, names + n);
+    m->host->set_link_order(v);
+    m->refresh();
+    m->reset_caches();
+  });
+}
+
+int sapamd_joint_limits(const sapamd_model *m, double *lower, double *upper) {
+  return guarded([&] {
+    if (!m->host->joint_order_set()) throw std::runtime_error("posS2P failed");
+    const auto &J = m->host->joints();
+    for (size_t j = 0; j < J.size(); ++j) {
+      int u = m->host->vuser()[j];
+      if (u < 0) continue;
+      bool cont = J[j].kind == JK_CONTINUOUS;
+      lower[u] = cont ? -M_PI : J[j].lower;
+      upper[u] = cont ? M_PI : J[j].upper;
+    }
+  });
+}
+
+int sapamd_continuous_mask(const sapamd_model *m, int32_t *out) {
+  return guarded([&] {
+    if (!m->host->joint_order_set()) throw std::runtime_error("posS2P failed");
+    const auto &J = m->host->joints();
+    for (size_t j = 0; j < J.size(); ++j) {
+      int u = m->host->vuser()[j];
+      if (u >= 0) out[u] = J[j].kind == JK_CONTINUOUS;
+    }
+  });
+}
+
+// ---------------------------------------------------------------- per-call
+int sapamd_compute_forward_kinematics(sapamd_model *m, const double *qpos, int64_t n) {
+  return guarded([&] {
+    m->host->check_compute((size_t)n);
+    m->require_tables();
+    DeviceGuard g(m->device);
+    size_t nv = m->nv(), L = m->L();
+    double *dq = m->stage(qpos, nv, 0);
+    KinOut<double> o;
+    o.poses = m->d_buf + nv;
+    HIP_OK(launch_kinematics<double>(m->t64(), m->host->njoints(), 1, KF_POSES, dq, nullptr, nullptr, o, m->stream));
+    m->fetch(m->pose_cache.data(), 7 * L, nv);
+  });
+}
+
+int sapamd_get_link_pose(const sapamd_model *m, int32_t link, float pose7[7]) {
+  return guarded([&] {
+    m->check_link(link);
+    for (int i = 0; i < 7; ++i) pose7[i] = (float)m->pose_cache[7 * link + i];
+  });
+}
+
+int sapamd_compute_full_jacobian(sapamd_model *m, const double *qpos, int64_t n) {
+  return guarded([&] {
+    m->host->check_compute((size_t)n);
+    m->require_tables();
+    DeviceGuard g(m->device);
+    size_t nv = m->nv(), L = m->L();
+    double *dq = m->stage(qpos, nv, 0);
+    KinOut<double> o;
+    o.poses = m->d_buf + nv;
+    o.jw_links = o.poses + 7 * L;
+    o.jl_links = o.jw_links + 6 * nv * L;
+    HIP_OK(launch_kinematics<double>(m->t64(), m->host->njoints(), 1, KF_POSES | KF_JLINKS, dq, nullptr, nullptr, o,
+                                     m->stream));
+    m->fetch(m->pose_cache.data(), 7 * L, nv);
+    m->fetch(m->jw_cache.data(), 6 * nv * L, nv + 7 * L);
+    m->fetch(m->jl_cache.data(), 6 * nv * L, nv + 7 * L + 6 * nv * L);
+  });
+}
+
+int sapamd_get_link_jacobian(const sapamd_model *m, int32_t link, int32_t local, double *out) {
+  return guarded([&] {
+    m->check_link(link);
+    size_t sz = 6 * (size_t)m->nv();
+    const auto &c = local ? m->jl_cache : m->jw_cache;
+    std::memcpy(out, c.data() + link * sz, sz * sizeof(double));
+  });
+}
+
+int sapamd_compute_single_link_local_jacobian(sapamd_model *m, const double *qpos, int64_t n, int32_t link,
+                                              double *out) {
+  return guarded([&] {
+    m->check_link(link);
+    m->host->check_compute((size_t)n);
+    m->require_tables();
+    DeviceGuard g(m->device);
+    size_t nv = m->nv();
+    double *dq = m->stage(qpos, nv, 0);
+    HIP_OK(launch_link_local_jacobian<double>(m->t64(), m->host->njoints(), 1, link, dq, m->d_buf + nv, m->stream));
+    m->fetch(out, 6 * nv, nv);
+  });
+}
+
+int sapamd_compute_generalized_mass_matrix(sapamd_model *m, const double *qpos, int64_t n, double *out) {
+  return guarded([&] {
+    m->host->check_compute((size_t)n);
+    m->require_tables();
+    DeviceGuard g(m->device);
+    size_t nv = m->nv();
+    double *dq = m->stage(qpos, nv, 0);
+    HIP_OK(launch_crba<double>(m->t64(), m->host->njoints(), 1, dq, m->d_buf + nv, m->stream));
+    m->fetch(out, nv * nv, nv);
+  });
+}
+
+int sapamd_compute_coriolis_matrix(sapamd_model *m, const double *qpos, const double *qvel, int64_t n,
+                                   double *out) {
+  return guarded([&] {
+    m->host->check_compute((size_t)n);
+    m->require_tables();
+    DeviceGuard g(m->device);
+    size_t nv = m->nv();
+    double *dq = m->stage(qpos, nv, 0);
+    double *dv = m->stage(qvel, nv, nv);
+    HIP_OK(launch_coriolis<double>(m->t64(), m->host->njoints(), 1, dq, dv, m->d_buf + 2 * nv, m->stream));
+    m->fetch(out, nv * nv, 2 * nv);
+  });
+}
+
+int sapamd_compute_inverse_dynamics(sapamd_model *m, const double *qpos, const double *qvel, const double *qacc,
+                                    int64_t n, double *out) {
+  return guarded([&] {
+    m->host->check_compute((size_t)n);
+    m->require_tables();
+    DeviceGuard g(m->device);
+    size_t nv = m->nv();
+    double *dq = m->stage(qpos, nv, 0);
+    double *dv = m->stage(qvel, nv, nv);
+    double *da = m->stage(qacc, nv, 2 * nv);
+    KinOut<double> o;
+    o.tau = m->d_buf + 3 * nv;
+    HIP_OK(launch_kinematics<double>(m->t64(), m->host->njoints(), 1, KF_RNEA, dq, dv, da, o, m->stream));
+    m->fetch(out, nv, 3 * nv);
+  });
+}
+
+int sapamd_compute_forward_dynamics(sapamd_model *m, const double *qpos, const double *qvel, const double *qf,
+                                    int64_t n, double *out) {
+  return guarded([&] {
+    m->host->check_compute((size_t)n);
+    m->require_tables();
+    DeviceGuard g(m->device);
+    size_t nv = m->nv();
+    double *dq = m->stage(qpos, nv, 0);
+    double *dv = m->stage(qvel, nv, nv);
+    double *df = m->stage(qf, nv, 2 * nv);
+    HIP_OK(launch_aba<double>(m->t64(), m->host->njoints(), 1, dq, dv, df, m->d_buf + 3 * nv, m->stream));
+    m->fetch(out, nv, 3 * nv);
+  });
+}
+
+int sapamd_compute_inverse_kinematics(sapamd_model *m, int32_t link, const float pose7[7],
+                                      const double *initial_qpos, int64_t n0, const int32_t *active_qmask,
+                                      int64_t nm, double eps, int32_t max_iterations, double dt, double damp,
+                                      double *q_out, int32_t *success, double err6[6]) {
+  return guarded([&] {
+    m->check_link(link);
+    if (n0 > 0) m->host->check_compute((size_t)n0);
+    if (!m->host->supported())
+      throw std::runtime_error("Unsupported joint in computation. Currently support: fixed, revolute, prismatic");
+    if (!m->host->joint_order_set()) throw std::runtime_error("posP2S failed");
+    if (nm > 0 && nm != m->nv()) throw std::runtime_error("active_qmask must have length nv");
+    m->require_tables();
+    DeviceGuard g(m->device);
+    size_t nv = m->nv();
+    // device buffer: [target 7 | q0 nv | q_out nv | err 6 | mask nv (int) | success, iters (int)]
+    double tgt[7];
+    for (int i = 0; i < 7; ++i) tgt[i] = pose7[i];
+    double *dt_ = m->stage(tgt, 7, 0);
+    double *dq0 = n0 > 0 ? m->stage(initial_qpos, nv, 8) : nullptr;
+    double *dqo = m->d_buf + 8 + nv;
+    double *derr = dqo + nv;
+    int32_t *dmask = reinterpret_cast<int32_t *>(derr + 8);
+    int32_t *dints = dmask + nv + 2;
+    if (nm > 0)
+      HIP_OK(hipMemcpyAsync(dmask, active_qmask, nv * sizeof(int32_t), hipMemcpyHostToDevice, m->stream));
+    ClikParams P;
+    P.link = link;
+    P.chain = m->d_chain + m->chain_off[link];
+    P.nchain = m->chain_len[link];
+    P.eps = eps;
+    P.dt = dt;
+    P.damp = damp;
+    P.max_iter = max_iterations;
+    P.mask = nm > 0 ? dmask : nullptr;
+    P.mask_ld = 0;
+    P.counter = m->d_counter;
+    P.persistent_waves = 0;
+    HIP_OK(launch_clik<double>(m->t64(), m->host->njoints(), 1, P, dt_, dq0, dqo, dints, derr, dints + 1,
+                               m->stream));
+    m->fetch(q_out, nv, 8 + nv);
+    m->fetch(err6, 6, 8 + 2 * nv);
+    int32_t ints[2];
+    HIP_OK(hipMemcpyAsync(ints, dints, sizeof(ints), hipMemcpyDeviceToHost, m->stream));
+    HIP_OK(hipStreamSynchronize(m->stream));
+    *success = ints[0];
+  });
+}
+
+// ---------------------------------------------------------------- batched
+namespace {
+void check_batched(const sapamd_model *m, int64_t B) {
+  if (!m) throw std::invalid_argument("null model");
+  if (B < 0) throw std::invalid_argument("negative batch size");
+  m->require_tables();
+}
+void need(const void *p, const char *what) {
+  if (!p) throw std::invalid_argument(std::string(what) + " must not be null");
+}
+hipStream_t S(void *s) { return static_cast<hipStream_t>(s); }
+}  // namespace
+
+int sapamd_fk_jac_rnea_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd,
+                               const float *qdd, float *poses, float *J, float *tau, void *stream) {
+  return guarded([&] {
+    check_batched(m, B);
+    if (B == 0) return;
+    need(q, "q");
+    int flags = 0;
+    KinOut<float> o;
+    if (poses) { flags |= KF_POSES; o.poses = poses; if (m->L() == 0) throw std::runtime_error("link index out of bound"); }
+    if (J) { flags |= KF_JFULL; o.jfull = J; }
+    if (tau) { need(qd, "qd"); need(qdd, "qdd"); flags |= KF_RNEA; o.tau = tau; }
+    HIP_OK(launch_kinematics<float>(m->t32(), m->host->njoints(), B, flags, q, qd, qdd, o, S(stream)));
+  });
+}
+
+int sapamd_fk_batched(const sapamd_model *m, int64_t B, const float *q, float *poses, void *stream) {
+  need(poses, "poses");
+  return sapamd_fk_jac_rnea_batched(m, B, q, nullptr, nullptr, poses, nullptr, nullptr, stream);
+}
+
+int sapamd_jacobian_batched(const sapamd_model *m, int64_t B, const float *q, float *J, void *stream) {
+  if (!J) { g_last_error = "J must not be null"; return SAPAMD_INVALID_ARGUMENT; }
+  return sapamd_fk_jac_rnea_batched(m, B, q, nullptr, nullptr, nullptr, J, nullptr, stream);
+}
+
+int sapamd_link_jacobians_batched(const sapamd_model *m, int64_t B, const float *q, float *J_world, float *J_local,
+                                  void *stream) {
+  return guarded([&] {
+    check_batched(m, B);
+    if (B == 0) return;
+    need(q, "q");
+    if (!J_world || !J_local) throw std::invalid_argument("J_world and J_local must both be given");
+    KinOut<float> o;
+    o.jw_links = J_world;
+    o.jl_links = J_local;
+    HIP_OK(launch_kinematics<float>(m->t32(), m->host->njoints(), B, KF_JLINKS, q, nullptr, nullptr, o, S(stream)));
+  });
+}
+
+int sapamd_rnea_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd, const float *qdd,
+                        float *tau, void *stream) {
+  if (!tau) { g_last_error = "tau must not be null"; return SAPAMD_INVALID_ARGUMENT; }
+  return sapamd_fk_jac_rnea_batched(m, B, q, qd, qdd, nullptr, nullptr, tau, stream);
+}
+
+int sapamd_crba_batched(const sapamd_model *m, int64_t B, const float *q, float *M, void *stream) {
+  return guarded([&] {
+    check_batched(m, B);
+    if (B == 0) return;
+    need(q, "q");
+    need(M, "M");
+    HIP_OK(launch_crba<float>(m->t32(), m->host->njoints(), B, q, M, S(stream)));
+  });
+}
+
+int sapamd_coriolis_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd, float *C,
+                            void *stream) {
+  return guarded([&] {
+    check_batched(m, B);
+    if (B == 0) return;
+    need(q, "q");
+    need(qd, "qd");
+    need(C, "C");
+    HIP_OK(launch_coriolis<float>(m->t32(), m->host->njoints(), B, q, qd, C, S(stream)));
+  });
+}
+
+int sapamd_aba_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd, const float *tau,
+                       float *qdd, void *stream) {
+  return guarded([&] {
+    check_batched(m, B);
+    if (B == 0) return;
+    need(q, "q");
+    need(qd, "qd");
+    need(tau, "tau");
+    need(qdd, "qdd");
+    HIP_OK(launch_aba<float>(m->t32(), m->host->njoints(), B, q, qd, tau, qdd, S(stream)));
+  });
+}
+
+int sapamd_clik_batched(const sapamd_model *m, int64_t B, int32_t link, const float *target, const float *q0,
+                        const int32_t *mask, double eps, int32_t max_iterations, double dt, double damp,
+                        float *q_out, int32_t *success, float *err, int32_t *iterations, void *stream) {
+  return guarded([&] {
+    check_batched(m, B);
+    m->check_link(link);
+    if (B == 0) return;
+    need(target, "target");
+    need(q_out, "q_out");
+    need(success, "success");
+    need(err, "err");
+    ClikParams P;
+    P.link = link;
+    P.chain = m->d_chain + m->chain_off[link];
+    P.nchain = m->chain_len[link];
+    P.eps = eps;
+    P.dt = dt;
+    P.damp = damp;
+    P.max_iter = max_iterations;
+    P.mask = mask;
+    P.mask_ld = 0;
+    P.counter = m->d_counter;
+    int dev_cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, m->device) == hipSuccess) dev_cus = prop.multiProcessorCount;
+    int64_t waves = (B + 63) / 64;
+    // Persistent launch with a work queue once there are more problems than
+    // resident lanes: finished lanes pull the next problem. The queue counter
+    // belongs to the model, so overlapping batched CLIK calls on one model race.
+    P.persistent_waves = waves > 4 * dev_cus ? 4 * dev_cus : 0;
+    if (P.persistent_waves) HIP_OK(launch_fill_i32(m->d_counter, 0, 1, S(stream)));
+    else P.counter = nullptr;
+    HIP_OK(launch_clik<float>(m->t32(), m->host->njoints(), B, P, target, q0, q_out, success, err, iterations,
+                              S(stream)));
+  });
+}
+
+}  // extern "C"

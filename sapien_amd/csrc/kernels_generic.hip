@@ -1,0 +1,671 @@
+// Generic thread-per-environment kernels for any tree up to 64 joints, in FP32
+// (batched path, heterogeneous fallback) and FP64 (the drop-in per-call path).
+//
+// Each lane owns one environment and sweeps the joints in topological order with
+// the world-frame recursions of spatial.h:
+//   FK        oMi[j] = oMi[parent] * place_j * X_J(q_j)            (pinocchio_model.cpp:146-148)
+//   Jacobian  col_j = [p_j x a_j; a_j] or [a_j; 0]                 (:163-186)
+//   RNEA      v_j = v_par + col_j qd_j, a_j = a_par + col_j qdd_j + (v_j x col_j) qd_j,
+//             f_j = I_j a_j + v_j x* I_j v_j, F_par += F_j, tau_j = col_j . F_j   (:214-219)
+//   CRBA      Yc_par += Yc_j, M_ij = col_i . (Yc_j col_j), i ancestor-or-self of j   (:201-206)
+//   ABA       Featherstone's three passes in the world frame (no frame changes)       (:221-226)
+//   Coriolis  C = sum_i J_i^T (I_i Jdot_i + B(I_i, v_i) J_i) by subtree sums (Christoffel-consistent) (:208-212)
+//   CLIK      damped least squares on log6 with Jlog6, per-problem early exit        (:228-300)
+// Per-joint state lives in private arrays sized by the next power of two >= nj.
+#include <hip/hip_runtime.h>
+
+#include "launch.h"
+#include "spatial.h"
+
+namespace sapamd {
+
+namespace {
+
+template <class T> __device__ __forceinline__ S6<T> gravity_acc(const ModelTable<T> *M) {
+  return {{-M->gravity[0], -M->gravity[1], -M->gravity[2]}, {T(0), T(0), T(0)}};
+}
+
+// Body (local) frame image of a world motion vector under link pose (R, p):
+// Ad(M)^-1 m = [R^T (l - p x a); R^T a].
+template <class T> __device__ __forceinline__ S6<T> to_local(const T *R, V3<T> p, S6<T> m) {
+  return {mtv(R, m.l - cross(p, m.a)), mtv(R, m.a)};
+}
+
+template <class T, int MJ>
+__global__ __launch_bounds__(64) void k_kinematics(const ModelTable<T> *__restrict__ M, int64_t B, int flags,
+                                                   const T *__restrict__ q, const T *__restrict__ qd,
+                                                   const T *__restrict__ qdd, KinOut<T> out) {
+  int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const int nj = M->nj, nv = M->nv, L = M->nlinks;
+  const JointRec<T> *JR = joints_of(M);
+  const LinkRec<T> *LR = links_of(M);
+  T oR[MJ][9];
+  V3<T> op[MJ];
+  S6<T> col[MJ], vel[MJ], acc[MJ], F[MJ];
+  const bool dyn = flags & KF_RNEA;
+  const S6<T> a0 = gravity_acc(M);
+  const T *qb = q + b * nv;
+  for (int j = 0; j < nj; ++j) {
+    const JointRec<T> &J = JR[j];
+    const int u = J.vuser, p = J.parent;
+    T qj = qb[u], s, c;
+    sincos_t(qj, &s, &c);
+    joint_world(J, qj, s, c, p >= 0 ? oR[p] : nullptr, p >= 0 ? op[p] : V3<T>{0, 0, 0}, p < 0, oR[j], op[j]);
+    col[j] = jcol(J, oR[j], op[j]);
+    if (dyn) {
+      T qdj = qd[b * nv + u], qddj = qdd[b * nv + u];
+      S6<T> vp = p >= 0 ? vel[p] : zero6<T>();
+      S6<T> ap = p >= 0 ? acc[p] : a0;
+      vel[j] = vp + qdj * col[j];
+      acc[j] = ap + qddj * col[j] + qdj * crossm(vel[j], col[j]);
+      WInertia<T> Y = world_inertia(J, oR[j], op[j]);
+      F[j] = inertia_apply(Y, acc[j]) + crossf(vel[j], inertia_apply(Y, vel[j]));
+    }
+  }
+  if (dyn) {
+    for (int j = nj - 1; j >= 0; --j) {
+      out.tau[b * nv + JR[j].vuser] = dot6(col[j], F[j]);
+      int p = JR[j].parent;
+      if (p >= 0) F[p] = F[p] + F[j];
+    }
+  }
+  if (flags & KF_JFULL) {
+    T *Jb = out.jfull + b * 6 * nv;
+    for (int j = 0; j < nj; ++j) {
+      int u = JR[j].vuser;
+      Jb[0 * nv + u] = col[j].l.x; Jb[1 * nv + u] = col[j].l.y; Jb[2 * nv + u] = col[j].l.z;
+      Jb[3 * nv + u] = col[j].a.x; Jb[4 * nv + u] = col[j].a.y; Jb[5 * nv + u] = col[j].a.z;
+    }
+  }
+  if (flags & (KF_POSES | KF_JLINKS)) {
+    for (int l = 0; l < L; ++l) {
+      const LinkRec<T> &Lk = LR[l];
+      const int jl = Lk.joint;
+      T R[9];
+      V3<T> pw;
+      if (jl < 0) {
+        for (int k = 0; k < 9; ++k) R[k] = Lk.R[k];
+        pw = load3(Lk.p);
+      } else {
+        mm(oR[jl], Lk.R, R);
+        pw = mv(oR[jl], load3(Lk.p)) + op[jl];
+      }
+      if (flags & KF_POSES) {
+        T *P = out.poses + (b * L + l) * 7;
+        P[0] = pw.x; P[1] = pw.y; P[2] = pw.z;
+        quat_of(R, P + 3);
+      }
+      if (flags & KF_JLINKS) {
+        T *Jw = out.jw_links + (b * L + l) * 6 * nv;
+        T *Jl = out.jl_links + (b * L + l) * 6 * nv;
+        for (int k = 0; k < 6 * nv; ++k) { Jw[k] = T(0); Jl[k] = T(0); }
+        for (int k = jl; k >= 0; k = JR[k].parent) {
+          int u = JR[k].vuser;
+          S6<T> cw = col[k], cl = to_local(R, pw, cw);
+          Jw[0 * nv + u] = cw.l.x; Jw[1 * nv + u] = cw.l.y; Jw[2 * nv + u] = cw.l.z;
+          Jw[3 * nv + u] = cw.a.x; Jw[4 * nv + u] = cw.a.y; Jw[5 * nv + u] = cw.a.z;
+          Jl[0 * nv + u] = cl.l.x; Jl[1 * nv + u] = cl.l.y; Jl[2 * nv + u] = cl.l.z;
+          Jl[3 * nv + u] = cl.a.x; Jl[4 * nv + u] = cl.a.y; Jl[5 * nv + u] = cl.a.z;
+        }
+      }
+    }
+  }
+}
+
+template <class T, int MJ>
+__global__ __launch_bounds__(64) void k_link_local_jacobian(const ModelTable<T> *__restrict__ M, int64_t B, int link,
+                                                            const T *__restrict__ q, T *__restrict__ Jout) {
+  int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const int nj = M->nj, nv = M->nv;
+  const JointRec<T> *JR = joints_of(M);
+  const LinkRec<T> &Lk = links_of(M)[link];
+  T oR[MJ][9];
+  V3<T> op[MJ];
+  const T *qb = q + b * nv;
+  for (int j = 0; j < nj; ++j) {
+    const JointRec<T> &J = JR[j];
+    int p = J.parent;
+    T qj = qb[J.vuser], s, c;
+    sincos_t(qj, &s, &c);
+    joint_world(J, qj, s, c, p >= 0 ? oR[p] : nullptr, p >= 0 ? op[p] : V3<T>{0, 0, 0}, p < 0, oR[j], op[j]);
+  }
+  T *Jb = Jout + b * 6 * nv;
+  for (int k = 0; k < 6 * nv; ++k) Jb[k] = T(0);
+  const int jl = Lk.joint;
+  if (jl < 0) return;
+  T R[9];
+  mm(oR[jl], Lk.R, R);
+  V3<T> pw = mv(oR[jl], load3(Lk.p)) + op[jl];
+  for (int k = jl; k >= 0; k = JR[k].parent) {
+    int u = JR[k].vuser;
+    S6<T> cl = to_local(R, pw, jcol(JR[k], oR[k], op[k]));
+    Jb[0 * nv + u] = cl.l.x; Jb[1 * nv + u] = cl.l.y; Jb[2 * nv + u] = cl.l.z;
+    Jb[3 * nv + u] = cl.a.x; Jb[4 * nv + u] = cl.a.y; Jb[5 * nv + u] = cl.a.z;
+  }
+}
+
+template <class T, int MJ>
+__global__ __launch_bounds__(64) void k_crba(const ModelTable<T> *__restrict__ M, int64_t B, const T *__restrict__ q,
+                                             T *__restrict__ Mout) {
+  int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const int nj = M->nj, nv = M->nv;
+  const JointRec<T> *JR = joints_of(M);
+  T oR[MJ][9];
+  V3<T> op[MJ];
+  S6<T> col[MJ];
+  AInertia<T> Yc[MJ];
+  const T *qb = q + b * nv;
+  for (int j = 0; j < nj; ++j) {
+    const JointRec<T> &J = JR[j];
+    int p = J.parent;
+    T qj = qb[J.vuser], s, c;
+    sincos_t(qj, &s, &c);
+    joint_world(J, qj, s, c, p >= 0 ? oR[p] : nullptr, p >= 0 ? op[p] : V3<T>{0, 0, 0}, p < 0, oR[j], op[j]);
+    col[j] = jcol(J, oR[j], op[j]);
+    Yc[j] = additive(world_inertia(J, oR[j], op[j]));
+  }
+  for (int j = nj - 1; j > 0; --j) {
+    int p = JR[j].parent;
+    if (p >= 0) add_to(Yc[p], Yc[j]);
+  }
+  T *Mb = Mout + b * nv * nv;
+  for (int k = 0; k < nv * nv; ++k) Mb[k] = T(0);
+  for (int j = 0; j < nj; ++j) {
+    S6<T> Fj = apply(Yc[j], col[j]);
+    int uj = JR[j].vuser;
+    for (int i = j; i >= 0; i = JR[i].parent) {
+      int ui = JR[i].vuser;
+      T v = dot6(col[i], Fj);
+      Mb[ui * nv + uj] = v;
+      Mb[uj * nv + ui] = v;
+    }
+  }
+}
+
+// ---- dense 6x6 helpers (row-major) for ABA and the Coriolis matrix
+template <class T> struct Mat6 {
+  T a[36];
+};
+template <class T> __device__ __forceinline__ void skew_into(T *m, int r0, int c0, V3<T> v, T sgn) {
+  m[(r0 + 0) * 6 + c0 + 0] = T(0);     m[(r0 + 0) * 6 + c0 + 1] = -sgn * v.z; m[(r0 + 0) * 6 + c0 + 2] = sgn * v.y;
+  m[(r0 + 1) * 6 + c0 + 0] = sgn * v.z; m[(r0 + 1) * 6 + c0 + 1] = T(0);     m[(r0 + 1) * 6 + c0 + 2] = -sgn * v.x;
+  m[(r0 + 2) * 6 + c0 + 0] = -sgn * v.y; m[(r0 + 2) * 6 + c0 + 1] = sgn * v.x; m[(r0 + 2) * 6 + c0 + 2] = T(0);
+}
+// Y as a 6x6 matrix: [[m I, -[h]x], [[h]x, I_O]].
+template <class T> __device__ __forceinline__ Mat6<T> mat_of(const AInertia<T> &Y) {
+  Mat6<T> m;
+  for (int k = 0; k < 36; ++k) m.a[k] = T(0);
+  m.a[0] = m.a[7] = m.a[14] = Y.m;
+  skew_into(m.a, 0, 3, Y.h, T(-1));
+  skew_into(m.a, 3, 0, Y.h, T(1));
+  m.a[21] = Y.I[0]; m.a[22] = Y.I[1]; m.a[23] = Y.I[2];
+  m.a[27] = Y.I[1]; m.a[28] = Y.I[3]; m.a[29] = Y.I[4];
+  m.a[33] = Y.I[2]; m.a[34] = Y.I[4]; m.a[35] = Y.I[5];
+  return m;
+}
+template <class T> __device__ __forceinline__ S6<T> mul6(const Mat6<T> &A, S6<T> x) {
+  T v[6] = {x.l.x, x.l.y, x.l.z, x.a.x, x.a.y, x.a.z}, r[6];
+  for (int i = 0; i < 6; ++i) {
+    T acc = T(0);
+    for (int k = 0; k < 6; ++k) acc += A.a[i * 6 + k] * v[k];
+    r[i] = acc;
+  }
+  return {{r[0], r[1], r[2]}, {r[3], r[4], r[5]}};
+}
+template <class T> __device__ __forceinline__ S6<T> mul6t(const Mat6<T> &A, S6<T> x) {
+  T v[6] = {x.l.x, x.l.y, x.l.z, x.a.x, x.a.y, x.a.z}, r[6];
+  for (int i = 0; i < 6; ++i) {
+    T acc = T(0);
+    for (int k = 0; k < 6; ++k) acc += A.a[k * 6 + i] * v[k];
+    r[i] = acc;
+  }
+  return {{r[0], r[1], r[2]}, {r[3], r[4], r[5]}};
+}
+template <class T> __device__ __forceinline__ void vec_of(S6<T> x, T *v) {
+  v[0] = x.l.x; v[1] = x.l.y; v[2] = x.l.z; v[3] = x.a.x; v[4] = x.a.y; v[5] = x.a.z;
+}
+
+template <class T, int MJ>
+__global__ __launch_bounds__(64) void k_aba(const ModelTable<T> *__restrict__ M, int64_t B, const T *__restrict__ q,
+                                            const T *__restrict__ qd, const T *__restrict__ tau,
+                                            T *__restrict__ qdd) {
+  int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const int nj = M->nj, nv = M->nv;
+  const JointRec<T> *JR = joints_of(M);
+  T oR[MJ][9];
+  V3<T> op[MJ];
+  S6<T> col[MJ], vel[MJ], cb[MJ], pA[MJ], U[MJ];
+  Mat6<T> IA[MJ];
+  T D[MJ], uu[MJ];
+  const T *qb = q + b * nv;
+  for (int j = 0; j < nj; ++j) {
+    const JointRec<T> &J = JR[j];
+    int p = J.parent, u = J.vuser;
+    T qj = qb[u], s, c;
+    sincos_t(qj, &s, &c);
+    joint_world(J, qj, s, c, p >= 0 ? oR[p] : nullptr, p >= 0 ? op[p] : V3<T>{0, 0, 0}, p < 0, oR[j], op[j]);
+    col[j] = jcol(J, oR[j], op[j]);
+    T qdj = qd[b * nv + u];
+    vel[j] = (p >= 0 ? vel[p] : zero6<T>()) + qdj * col[j];
+    cb[j] = qdj * crossm(vel[j], col[j]);
+    WInertia<T> Y = world_inertia(J, oR[j], op[j]);
+    IA[j] = mat_of(additive(Y));
+    pA[j] = crossf(vel[j], inertia_apply(Y, vel[j]));
+  }
+  for (int j = nj - 1; j >= 0; --j) {
+    U[j] = mul6(IA[j], col[j]);
+    D[j] = dot6(col[j], U[j]);
+    uu[j] = tau[b * nv + JR[j].vuser] - dot6(col[j], pA[j]);
+    int p = JR[j].parent;
+    if (p >= 0) {
+      T Uv[6];
+      vec_of(U[j], Uv);
+      T invD = T(1) / D[j];
+      Mat6<T> Ia;
+      for (int r = 0; r < 6; ++r)
+        for (int k = 0; k < 6; ++k) Ia.a[r * 6 + k] = IA[j].a[r * 6 + k] - Uv[r] * Uv[k] * invD;
+      S6<T> pa = pA[j] + mul6(Ia, cb[j]) + (uu[j] * invD) * U[j];
+      for (int k = 0; k < 36; ++k) IA[p].a[k] += Ia.a[k];
+      pA[p] = pA[p] + pa;
+    }
+  }
+  const S6<T> a0 = gravity_acc(M);
+  S6<T> acc[MJ];
+  for (int j = 0; j < nj; ++j) {
+    int p = JR[j].parent;
+    acc[j] = (p >= 0 ? acc[p] : a0) + cb[j];
+    T x = (uu[j] - dot6(U[j], acc[j])) / D[j];
+    qdd[b * nv + JR[j].vuser] = x;
+    acc[j] = acc[j] + x * col[j];
+  }
+}
+
+// B(I, v) = 1/2 [ (v x*) I + (I v) xbar* - I (v x) ]  (Echeandia & Wensing, 2021)
+template <class T> __device__ Mat6<T> coriolis_body(const Mat6<T> &I, S6<T> v) {
+  Mat6<T> crf, crm, bf, out;
+  for (int k = 0; k < 36; ++k) { crf.a[k] = T(0); crm.a[k] = T(0); bf.a[k] = T(0); }
+  // crm(v) = [[w]x, [v]x; 0, [w]x]; crf(v) = [[w]x, 0; [v]x, [w]x]
+  skew_into(crm.a, 0, 0, v.a, T(1)); skew_into(crm.a, 0, 3, v.l, T(1)); skew_into(crm.a, 3, 3, v.a, T(1));
+  skew_into(crf.a, 0, 0, v.a, T(1)); skew_into(crf.a, 3, 0, v.l, T(1)); skew_into(crf.a, 3, 3, v.a, T(1));
+  // (f xbar*) m = m x* f, f = I v: [[0, -[fl]x], [-[fl]x, -[fa]x]]
+  S6<T> f = mul6(I, v);
+  skew_into(bf.a, 0, 3, f.l, T(-1)); skew_into(bf.a, 3, 0, f.l, T(-1)); skew_into(bf.a, 3, 3, f.a, T(-1));
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 6; ++c) {
+      T s1 = T(0), s2 = T(0);
+      for (int k = 0; k < 6; ++k) {
+        s1 += crf.a[r * 6 + k] * I.a[k * 6 + c];
+        s2 += I.a[r * 6 + k] * crm.a[k * 6 + c];
+      }
+      out.a[r * 6 + c] = T(0.5) * (s1 + bf.a[r * 6 + c] - s2);
+    }
+  return out;
+}
+
+template <class T, int MJ>
+__global__ __launch_bounds__(64) void k_coriolis(const ModelTable<T> *__restrict__ M, int64_t B,
+                                                 const T *__restrict__ q, const T *__restrict__ qd,
+                                                 T *__restrict__ Cout) {
+  int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const int nj = M->nj, nv = M->nv;
+  const JointRec<T> *JR = joints_of(M);
+  T oR[MJ][9];
+  V3<T> op[MJ];
+  S6<T> col[MJ], dcol[MJ], vel[MJ];
+  Mat6<T> Ic[MJ], Bc[MJ];
+  const T *qb = q + b * nv;
+  for (int j = 0; j < nj; ++j) {
+    const JointRec<T> &J = JR[j];
+    int p = J.parent, u = J.vuser;
+    T qj = qb[u], s, c;
+    sincos_t(qj, &s, &c);
+    joint_world(J, qj, s, c, p >= 0 ? oR[p] : nullptr, p >= 0 ? op[p] : V3<T>{0, 0, 0}, p < 0, oR[j], op[j]);
+    col[j] = jcol(J, oR[j], op[j]);
+    vel[j] = (p >= 0 ? vel[p] : zero6<T>()) + qd[b * nv + u] * col[j];
+    dcol[j] = crossm(vel[j], col[j]);
+    Ic[j] = mat_of(additive(world_inertia(J, oR[j], op[j])));
+    Bc[j] = coriolis_body(Ic[j], vel[j]);
+  }
+  for (int j = nj - 1; j > 0; --j) {
+    int p = JR[j].parent;
+    if (p >= 0)
+      for (int k = 0; k < 36; ++k) { Ic[p].a[k] += Ic[j].a[k]; Bc[p].a[k] += Bc[j].a[k]; }
+  }
+  T *Cb = Cout + b * nv * nv;
+  for (int k = 0; k < nv * nv; ++k) Cb[k] = T(0);
+  for (int j = 0; j < nj; ++j) {
+    int uj = JR[j].vuser;
+    // k ancestor-or-self of j: C_kj = col_k . (Ic_j dcol_j + Bc_j col_j)
+    S6<T> F1 = mul6(Ic[j], dcol[j]) + mul6(Bc[j], col[j]);
+    for (int k = j; k >= 0; k = JR[k].parent) Cb[JR[k].vuser * nv + uj] = dot6(col[k], F1);
+    // i strict ancestor of j: C_ji = (Ic_j col_j) . dcol_i + (Bc_j^T col_j) . col_i
+    S6<T> G = mul6(Ic[j], col[j]), H = mul6t(Bc[j], col[j]);
+    for (int i = JR[j].parent; i >= 0; i = JR[i].parent) Cb[uj * nv + JR[i].vuser] = dot6(G, dcol[i]) + dot6(H, col[i]);
+  }
+}
+
+// ---------------------------------------------------------------- SE(3) log for CLIK
+// Rotation log with theta = atan2(|axial|, (tr - 1) / 2), well conditioned at
+// every angle; near theta = pi the axis comes from the largest diagonal entry.
+template <class T> __device__ __forceinline__ V3<T> log3_dev(const T *R, T &theta) {
+  const T PI = T(3.14159265358979323846);
+  T c = (R[0] + R[4] + R[8] - T(1)) * T(0.5);
+  V3<T> ax = {T(0.5) * (R[7] - R[5]), T(0.5) * (R[2] - R[6]), T(0.5) * (R[3] - R[1])};
+  T s = sqrt(dot(ax, ax));
+  theta = atan2(s, c);
+  if (theta < PI - T(1e-2)) {
+    T k = s > T(1e-7) ? theta / s : T(1) + theta * theta / T(6);
+    return k * ax;
+  }
+  T omc = T(1) - c;
+  T d0 = R[0] - c, d1 = R[4] - c, d2 = R[8] - c;
+  V3<T> k;
+  if (d0 >= d1 && d0 >= d2) {
+    k.x = sqrt(fmax(d0 / omc, T(0)));
+    k.y = (R[1] + R[3]) / (T(2) * omc * k.x);
+    k.z = (R[2] + R[6]) / (T(2) * omc * k.x);
+  } else if (d1 >= d2) {
+    k.y = sqrt(fmax(d1 / omc, T(0)));
+    k.x = (R[1] + R[3]) / (T(2) * omc * k.y);
+    k.z = (R[5] + R[7]) / (T(2) * omc * k.y);
+  } else {
+    k.z = sqrt(fmax(d2 / omc, T(0)));
+    k.x = (R[2] + R[6]) / (T(2) * omc * k.z);
+    k.y = (R[5] + R[7]) / (T(2) * omc * k.z);
+  }
+  if (dot(k, ax) < T(0)) k = T(-1) * k;
+  return theta * k;
+}
+
+// alpha = (t/2) cot(t/2), beta = (1 - alpha) / t^2, dbeta = beta'(t) / t; series below 0.5.
+template <class T> __device__ __forceinline__ void log_coeffs_dev(T t, T &alpha, T &beta, T &dbeta) {
+  T t2 = t * t;
+  if (t < T(0.5)) {
+    alpha = T(1) - t2 * (T(1) / T(12) + t2 * (T(1) / T(720) + t2 * (T(1) / T(30240) + t2 / T(1209600))));
+    beta = T(1) / T(12) + t2 * (T(1) / T(720) + t2 * (T(1) / T(30240) + t2 * (T(1) / T(1209600) + t2 / T(47900160))));
+    dbeta = T(1) / T(360) + t2 * (T(1) / T(7560) + t2 * (T(1) / T(201600) + t2 / T(5987520)));
+  } else {
+    T st, ct;
+    sincos_t(t, &st, &ct);
+    T omc = T(1) - ct;
+    alpha = t * st / (T(2) * omc);
+    beta = T(1) / t2 - st / (T(2) * t * omc);
+    dbeta = T(-2) / (t2 * t2) + (T(1) + st / t) / (T(2) * t2 * omc);
+  }
+}
+
+template <class T>
+__device__ __forceinline__ S6<T> log6_dev(const T *R, V3<T> p) {
+  T t;
+  V3<T> w = log3_dev(R, t);
+  T al, be, db;
+  log_coeffs_dev(t, al, be, db);
+  return {al * p - T(0.5) * cross(w, p) + (be * dot(w, p)) * w, w};
+}
+
+// Jlog6 (6x6 row-major): [[A, C A], [0, A]], A = alpha I + beta w w^T + [w]x / 2,
+// C = dbeta (w.p) w w^T - (t^2 dbeta + 2 beta) p w^T + beta w p^T + beta (w.p) I + [p]x / 2.
+template <class T> __device__ __forceinline__ void jlog6_dev(const T *R, V3<T> p, T *Jl) {
+  T t;
+  V3<T> w = log3_dev(R, t);
+  T al, be, db;
+  log_coeffs_dev(t, al, be, db);
+  T wv[3] = {w.x, w.y, w.z}, pv[3] = {p.x, p.y, p.z};
+  T A[9], C[9];
+  T wp = dot(w, p);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      A[3 * i + j] = be * wv[i] * wv[j] + (i == j ? al : T(0));
+      C[3 * i + j] = db * wp * wv[i] * wv[j] - (t * t * db + T(2) * be) * pv[i] * wv[j] + be * wv[i] * pv[j] +
+                     (i == j ? be * wp : T(0));
+    }
+  A[1] -= T(0.5) * w.z; A[2] += T(0.5) * w.y; A[5] -= T(0.5) * w.x;
+  A[3] += T(0.5) * w.z; A[6] -= T(0.5) * w.y; A[7] += T(0.5) * w.x;
+  C[1] -= T(0.5) * p.z; C[2] += T(0.5) * p.y; C[5] -= T(0.5) * p.x;
+  C[3] += T(0.5) * p.z; C[6] -= T(0.5) * p.y; C[7] += T(0.5) * p.x;
+  T CA[9];
+  mm(C, A, CA);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      Jl[i * 6 + j] = A[3 * i + j];
+      Jl[i * 6 + 3 + j] = CA[3 * i + j];
+      Jl[(3 + i) * 6 + j] = T(0);
+      Jl[(3 + i) * 6 + 3 + j] = A[3 * i + j];
+    }
+}
+
+// y = (J J^T + damp I)^-1 e by LDL^T of the 6x6 SPD matrix.
+template <class T> __device__ __forceinline__ void ldlt6_solve(T *A, T *y) {
+  for (int j = 0; j < 6; ++j) {
+    T d = A[j * 6 + j];
+    for (int k = 0; k < j; ++k) d -= A[j * 6 + k] * A[j * 6 + k] * A[k * 6 + k];
+    A[j * 6 + j] = d;
+    for (int i = j + 1; i < 6; ++i) {
+      T s = A[i * 6 + j];
+      for (int k = 0; k < j; ++k) s -= A[i * 6 + k] * A[j * 6 + k] * A[k * 6 + k];
+      A[i * 6 + j] = s / d;
+    }
+  }
+  for (int i = 0; i < 6; ++i)
+    for (int k = 0; k < i; ++k) y[i] -= A[i * 6 + k] * y[k];
+  for (int i = 0; i < 6; ++i) y[i] /= A[i * 6 + i];
+  for (int i = 5; i >= 0; --i)
+    for (int k = i + 1; k < 6; ++k) y[i] -= A[k * 6 + i] * y[k];
+}
+
+template <class T> __device__ __forceinline__ void pose_to_mat(const T *P, T *R, V3<T> &p) {
+  T w = P[3], x = P[4], y = P[5], z = P[6];
+  T n = T(1) / sqrt(w * w + x * x + y * y + z * z);
+  w *= n; x *= n; y *= n; z *= n;
+  R[0] = T(1) - T(2) * (y * y + z * z); R[1] = T(2) * (x * y - w * z); R[2] = T(2) * (x * z + w * y);
+  R[3] = T(2) * (x * y + w * z); R[4] = T(1) - T(2) * (x * x + z * z); R[5] = T(2) * (y * z - w * x);
+  R[6] = T(2) * (x * z - w * y); R[7] = T(2) * (y * z + w * x); R[8] = T(1) - T(2) * (x * x + y * y);
+  p = {P[0], P[1], P[2]};
+}
+
+template <class T, int MJ>
+__global__ __launch_bounds__(64) void k_clik(const ModelTable<T> *__restrict__ M, int64_t B, ClikParams P,
+                                             const T *__restrict__ target, const T *__restrict__ q0,
+                                             T *__restrict__ q_out, int32_t *__restrict__ success,
+                                             T *__restrict__ err_out, int32_t *__restrict__ iters) {
+  const int nv = M->nv, nc = P.nchain;
+  const JointRec<T> *JR = joints_of(M);
+  const LinkRec<T> &Lk = links_of(M)[P.link];
+  int64_t prob = P.persistent_waves ? (int64_t)atomicAdd(P.counter, 1) : blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  while (prob < B) {
+    // desired pose of the link's joint frame: oMdes = target * link2joint^-1
+    T Rt[9], Rd[9];
+    V3<T> pt;
+    pose_to_mat(target + prob * 7, Rt, pt);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        Rd[3 * i + j] = Rt[3 * i] * Lk.R[3 * j] + Rt[3 * i + 1] * Lk.R[3 * j + 1] + Rt[3 * i + 2] * Lk.R[3 * j + 2];
+    V3<T> pd = pt - mv(Rd, load3(Lk.p));
+    T qc[MJ], qbest[MJ], msk[MJ];
+    for (int k = 0; k < nc; ++k) {
+      int u = JR[P.chain[k]].vuser;
+      qc[k] = q0 ? q0[prob * nv + u] : T(0);
+      qbest[k] = qc[k];
+      msk[k] = P.mask ? T(P.mask[prob * P.mask_ld + u]) : T(1);
+    }
+    T minErr = T(1e10), bestErr[6] = {0, 0, 0, 0, 0, 0};
+    bool ok = false;
+    int it = 0;
+    for (;; ++it) {
+      T oR[MJ][9];
+      V3<T> op[MJ];
+      T Rj[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+      V3<T> pj = {0, 0, 0};
+      for (int k = 0; k < nc; ++k) {
+        const JointRec<T> &J = JR[P.chain[k]];
+        T s, c;
+        sincos_t(qc[k], &s, &c);
+        joint_world(J, qc[k], s, c, k > 0 ? oR[k - 1] : nullptr, k > 0 ? op[k - 1] : V3<T>{0, 0, 0}, k == 0, oR[k],
+                    op[k]);
+      }
+      if (nc > 0) {
+        for (int k = 0; k < 9; ++k) Rj[k] = oR[nc - 1][k];
+        pj = op[nc - 1];
+      }
+      // iMd = oMj^-1 oMdes
+      T Ri[9];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Ri[3 * i + j] = Rj[i] * Rd[j] + Rj[3 + i] * Rd[3 + j] + Rj[6 + i] * Rd[6 + j];
+      V3<T> pi = mtv(Rj, pd - pj);
+      S6<T> e = log6_dev(Ri, pi);
+      T ev[6];
+      vec_of(e, ev);
+      T en = sqrt(dot6(e, e));
+      if (en < minErr) {
+        minErr = en;
+        for (int k = 0; k < nc; ++k) qbest[k] = qc[k];
+        for (int r = 0; r < 6; ++r) bestErr[r] = ev[r];
+      }
+      if (en < T(P.eps)) { ok = true; break; }
+      if (it >= P.max_iter) break;
+      // Jlog6 of iMd^-1 = (Ri^T, -Ri^T pi)
+      T Rinv[9];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Rinv[3 * i + j] = Ri[3 * j + i];
+      T Jl[36];
+      jlog6_dev(Rinv, T(-1) * mtv(Ri, pi), Jl);
+      T Jk[MJ][6];
+      T JJt[36];
+      for (int k = 0; k < 36; ++k) JJt[k] = T(0);
+      for (int k = 0; k < nc; ++k) {
+        S6<T> cw = jcol(JR[P.chain[k]], oR[k], op[k]);
+        S6<T> cl = to_local(Rj, pj, cw);
+        T cv[6];
+        vec_of(cl, cv);
+        for (int r = 0; r < 6; ++r) {
+          T acc = T(0);
+          for (int m = 0; m < 6; ++m) acc += Jl[r * 6 + m] * cv[m];
+          Jk[k][r] = -acc * msk[k];
+        }
+        for (int r = 0; r < 6; ++r)
+          for (int m = 0; m <= r; ++m) JJt[r * 6 + m] += Jk[k][r] * Jk[k][m];
+      }
+      for (int r = 0; r < 6; ++r) {
+        JJt[r * 6 + r] += T(P.damp);
+        for (int m = 0; m < r; ++m) JJt[m * 6 + r] = JJt[r * 6 + m];
+      }
+      T y[6];
+      for (int r = 0; r < 6; ++r) y[r] = ev[r];
+      ldlt6_solve(JJt, y);
+      for (int k = 0; k < nc; ++k) {
+        T v = T(0);
+        for (int r = 0; r < 6; ++r) v -= Jk[k][r] * y[r];
+        qc[k] += v * T(P.dt);
+      }
+    }
+    // result: chain dofs from the best iterate, the rest from q0; continuous wrapped
+    T *qo = q_out + prob * nv;
+    for (int u = 0; u < nv; ++u) qo[u] = q0 ? q0[prob * nv + u] : T(0);
+    for (int k = 0; k < nc; ++k) {
+      const JointRec<T> &J = JR[P.chain[k]];
+      T v = qbest[k];
+      if (J.kind == KIND_CONTINUOUS) {
+        T s, c;
+        sincos_t(v, &s, &c);
+        v = atan2(s, c);
+      }
+      qo[J.vuser] = v;
+    }
+    success[prob] = ok ? 1 : 0;
+    for (int r = 0; r < 6; ++r) err_out[prob * 6 + r] = bestErr[r];
+    if (iters) iters[prob] = it;
+    if (!P.persistent_waves) break;
+    prob = atomicAdd(P.counter, 1);
+  }
+}
+
+__global__ void k_fill_i32(int32_t *dst, int32_t value, int64_t n) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = value;
+}
+
+inline int pick_mj(int nj) { return nj <= 8 ? 8 : nj <= 16 ? 16 : nj <= 32 ? 32 : 64; }
+
+#define SAPAMD_DISPATCH_MJ(nj, KERNEL, ...)                                                      \
+  switch (pick_mj(nj)) {                                                                         \
+    case 8: hipLaunchKernelGGL((KERNEL<T, 8>), __VA_ARGS__); break;                             \
+    case 16: hipLaunchKernelGGL((KERNEL<T, 16>), __VA_ARGS__); break;                           \
+    case 32: hipLaunchKernelGGL((KERNEL<T, 32>), __VA_ARGS__); break;                           \
+    default: hipLaunchKernelGGL((KERNEL<T, 64>), __VA_ARGS__); break;                           \
+  }
+
+inline dim3 grid_for(int64_t B) { return dim3((unsigned)((B + 63) / 64)); }
+
+}  // namespace
+
+hipError_t launch_fill_i32(int32_t *dst, int32_t value, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, value, n);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_kinematics(const ModelTable<T> *tab, int nj, int64_t B, int flags, const T *q, const T *qd,
+                             const T *qdd, KinOut<T> out, hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  SAPAMD_DISPATCH_MJ(nj, k_kinematics, grid_for(B), dim3(64), 0, s, tab, B, flags, q, qd, qdd, out);
+  return hipGetLastError();
+}
+template <class T>
+hipError_t launch_link_local_jacobian(const ModelTable<T> *tab, int nj, int64_t B, int link, const T *q, T *J,
+                                      hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  SAPAMD_DISPATCH_MJ(nj, k_link_local_jacobian, grid_for(B), dim3(64), 0, s, tab, B, link, q, J);
+  return hipGetLastError();
+}
+template <class T>
+hipError_t launch_crba(const ModelTable<T> *tab, int nj, int64_t B, const T *q, T *M, hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  SAPAMD_DISPATCH_MJ(nj, k_crba, grid_for(B), dim3(64), 0, s, tab, B, q, M);
+  return hipGetLastError();
+}
+template <class T>
+hipError_t launch_coriolis(const ModelTable<T> *tab, int nj, int64_t B, const T *q, const T *qd, T *C,
+                           hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  SAPAMD_DISPATCH_MJ(nj, k_coriolis, grid_for(B), dim3(64), 0, s, tab, B, q, qd, C);
+  return hipGetLastError();
+}
+template <class T>
+hipError_t launch_aba(const ModelTable<T> *tab, int nj, int64_t B, const T *q, const T *qd, const T *tau, T *qdd,
+                      hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  SAPAMD_DISPATCH_MJ(nj, k_aba, grid_for(B), dim3(64), 0, s, tab, B, q, qd, tau, qdd);
+  return hipGetLastError();
+}
+template <class T>
+hipError_t launch_clik(const ModelTable<T> *tab, int nj, int64_t B, const ClikParams &P, const T *target,
+                       const T *q0, T *q_out, int32_t *success, T *err, int32_t *iters, hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  dim3 grid = P.persistent_waves ? dim3((unsigned)P.persistent_waves) : grid_for(B);
+  SAPAMD_DISPATCH_MJ(P.nchain, k_clik, grid, dim3(64), 0, s, tab, B, P, target, q0, q_out, success, err, iters);
+  return hipGetLastError();
+}
+
+#define SAPAMD_INST(T)                                                                                           \
+  template hipError_t launch_kinematics<T>(const ModelTable<T> *, int, int64_t, int, const T *, const T *,     \
+                                           const T *, KinOut<T>, hipStream_t);                                 \
+  template hipError_t launch_link_local_jacobian<T>(const ModelTable<T> *, int, int64_t, int, const T *, T *,  \
+                                                    hipStream_t);                                              \
+  template hipError_t launch_crba<T>(const ModelTable<T> *, int, int64_t, const T *, T *, hipStream_t);         \
+  template hipError_t launch_coriolis<T>(const ModelTable<T> *, int, int64_t, const T *, const T *, T *,       \
+                                         hipStream_t);                                                         \
+  template hipError_t launch_aba<T>(const ModelTable<T> *, int, int64_t, const T *, const T *, const T *, T *,  \
+                                    hipStream_t);                                                              \
+  template hipError_t launch_clik<T>(const ModelTable<T> *, int, int64_t, const ClikParams &, const T *,        \
+                                     const T *, T *, int32_t *, T *, int32_t *, hipStream_t);
+
+SAPAMD_INST(float)
+SAPAMD_INST(double)
+
+}  // namespace sapamd

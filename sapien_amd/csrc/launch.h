@@ -1,0 +1,59 @@
+// Kernel launchers (host-callable), one per algorithm, templated on the
+// arithmetic type. All pointers are device pointers in the user dof order:
+// q/qd/qdd/tau (B, nv); poses (B, L, 7) = p xyz | q wxyz; J (B, 6, nv);
+// per-link Jacobians (B, L, 6, nv); M/C (B, nv, nv).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "tables.h"
+
+namespace sapamd {
+
+enum KinFlags : int { KF_POSES = 1, KF_JFULL = 2, KF_JLINKS = 4, KF_RNEA = 8 };
+
+template <class T> struct KinOut {
+  T *poses = nullptr, *jfull = nullptr, *jw_links = nullptr, *jl_links = nullptr, *tau = nullptr;
+};
+
+constexpr int kMaxGenericJoints = 64;
+
+template <class T>
+hipError_t launch_kinematics(const ModelTable<T> *tab, int nj, int64_t B, int flags, const T *q, const T *qd,
+                             const T *qdd, KinOut<T> out, hipStream_t s);
+template <class T>
+hipError_t launch_crba(const ModelTable<T> *tab, int nj, int64_t B, const T *q, T *M, hipStream_t s);
+template <class T>
+hipError_t launch_coriolis(const ModelTable<T> *tab, int nj, int64_t B, const T *q, const T *qd, T *C,
+                           hipStream_t s);
+template <class T>
+hipError_t launch_aba(const ModelTable<T> *tab, int nj, int64_t B, const T *q, const T *qd, const T *tau, T *qdd,
+                      hipStream_t s);
+// Local (body-frame) Jacobian of one link: Ad(link2world)^-1 J (B, 6, nv).
+template <class T>
+hipError_t launch_link_local_jacobian(const ModelTable<T> *tab, int nj, int64_t B, int link, const T *q, T *J,
+                                      hipStream_t s);
+
+struct ClikParams {
+  int link;            // user link index
+  const int32_t *chain;  // internal joint ids root -> link joint (device), length nchain
+  int nchain;
+  double eps, dt, damp;
+  int max_iter;
+  const int32_t *mask;  // (nv) or (B, nv) int, nullptr = all active
+  int64_t mask_ld;      // 0 = shared mask, nv = per problem
+  int32_t *counter;     // work-queue counter, zeroed before the launch
+  int persistent_waves; // 0 = one lane per problem
+};
+// CLIK (pinocchio_model.cpp:228-300) for B problems sharing one target link.
+// target (B, 7) p | wxyz; q0 (B, nv) or nullptr (neutral); outputs q (B, nv),
+// success (B), err (B, 6), iters (B).
+template <class T>
+hipError_t launch_clik(const ModelTable<T> *tab, int nj, int64_t B, const ClikParams &P, const T *target,
+                       const T *q0, T *q_out, int32_t *success, T *err, int32_t *iters, hipStream_t s);
+
+// dst[0:n] = value (a kernel, not hipMemsetAsync, so it is safe under graph capture)
+hipError_t launch_fill_i32(int32_t *dst, int32_t value, int64_t n, hipStream_t s);
+
+}  // namespace sapamd

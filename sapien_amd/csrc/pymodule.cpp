@@ -1,0 +1,553 @@
+// pybind11 host module `sapien_amd._pinocchio`: the drop-in PinocchioModel
+// (same class, method and argument names, defaults, return types and error
+// types as the reference binding, pinocchio/pinocchio_model.cpp:306-367) and a
+// FP32 sapien.Pose stand-in (include/sapien/math/pose.h:24-49), plus batched
+// methods over (B, dof) tensors consumed through DLPack. Every computation goes
+// through the C ABI of include/sapien_amd.h, which runs HIP kernels on gfx950.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <array>
+#include <cmath>
+#include <memory>
+#include <random>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "dlpack_abi.h"
+#include "sapien_amd.h"
+
+namespace py = pybind11;
+
+namespace {
+
+void check(int rc) {
+  if (rc == SAPAMD_OK) return;
+  std::string msg = sapamd_last_error();
+  if (rc == SAPAMD_INVALID_ARGUMENT) throw py::value_error(msg);
+  throw std::runtime_error(msg);
+}
+
+using DArr = py::array_t<double, py::array::c_style | py::array::forcecast>;
+using IArr = py::array_t<int32_t, py::array::c_style | py::array::forcecast>;
+
+// ------------------------------------------------------------------ Pose (FP32, q = wxyz)
+struct Pose {
+  std::array<float, 3> p{0, 0, 0};
+  std::array<float, 4> q{1, 0, 0, 0};
+
+  std::array<float, 3> rotate(const std::array<float, 3> &v) const {
+    // 2 (u.v) u + (w^2 - u.u) v + 2 w (u x v)  (include/sapien/math/quat.h:76-79)
+    float w = q[0], x = q[1], y = q[2], z = q[3];
+    float uv = x * v[0] + y * v[1] + z * v[2], uu = x * x + y * y + z * z;
+    std::array<float, 3> c = {y * v[2] - z * v[1], z * v[0] - x * v[2], x * v[1] - y * v[0]};
+    return {2 * uv * x + (w * w - uu) * v[0] + 2 * w * c[0], 2 * uv * y + (w * w - uu) * v[1] + 2 * w * c[1],
+            2 * uv * z + (w * w - uu) * v[2] + 2 * w * c[2]};
+  }
+  Pose mul(const Pose &o) const {
+    Pose r;
+    auto rp = rotate(o.p);
+    for (int i = 0; i < 3; ++i) r.p[i] = rp[i] + p[i];
+    const auto &a = q, &b = o.q;
+    r.q = {a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3], a[0] * b[1] + b[0] * a[1] + a[2] * b[3] - b[2] * a[3],
+           a[0] * b[2] + b[0] * a[2] + a[3] * b[1] - b[3] * a[1], a[0] * b[3] + b[0] * a[3] + a[1] * b[2] - b[1] * a[2]};
+    return r;
+  }
+  Pose inv() const {
+    Pose r;
+    r.q = {q[0], -q[1], -q[2], -q[3]};
+    auto np = r.rotate({-p[0], -p[1], -p[2]});
+    r.p = np;
+    return r;
+  }
+  std::array<double, 9> matrix() const {
+    double n = std::sqrt((double)q[0] * q[0] + (double)q[1] * q[1] + (double)q[2] * q[2] + (double)q[3] * q[3]);
+    double w = q[0] / n, x = q[1] / n, y = q[2] / n, z = q[3] / n;
+    return {1 - 2 * (y * y + z * z), 2 * (x * y - w * z),     2 * (x * z + w * y),
+            2 * (x * y + w * z),     1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+            2 * (x * z - w * y),     2 * (y * z + w * x),     1 - 2 * (x * x + y * y)};
+  }
+  static std::array<float, 4> quat_of(const double *R) {
+    // Shoemake's method (largest of w, x, y, z first), normalized
+    double t = R[0] + R[4] + R[8], o[4];
+    if (t > 0) {
+      double r = std::sqrt(t + 1), h = 0.5 / r;
+      o[0] = 0.5 * r; o[1] = (R[7] - R[5]) * h; o[2] = (R[2] - R[6]) * h; o[3] = (R[3] - R[1]) * h;
+    } else {
+      int i = 0;
+      if (R[4] > R[0]) i = 1;
+      if (R[8] > R[3 * i + i]) i = 2;
+      int j = (i + 1) % 3, k = (j + 1) % 3;
+      double r = std::sqrt(R[3 * i + i] - R[3 * j + j] - R[3 * k + k] + 1), h = 0.5 / r;
+      o[1 + i] = 0.5 * r;
+      o[1 + j] = (R[3 * j + i] + R[3 * i + j]) * h;
+      o[1 + k] = (R[3 * k + i] + R[3 * i + k]) * h;
+      o[0] = (R[3 * k + j] - R[3 * j + k]) * h;
+    }
+    double n = std::sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2] + o[3] * o[3]);
+    return {float(o[0] / n), float(o[1] / n), float(o[2] / n), float(o[3] / n)};
+  }
+  std::array<float, 3> rpy() const {
+    auto R = matrix();  // R = Rz(y) Ry(p) Rx(r)
+    double pitch = std::asin(std::max(-1.0, std::min(1.0, -R[6])));
+    double roll, yaw;
+    if (std::abs(R[6]) < 1 - 1e-9) {
+      roll = std::atan2(R[7], R[8]);
+      yaw = std::atan2(R[3], R[0]);
+    } else {
+      roll = 0;
+      yaw = std::atan2(-R[1], R[4]);
+    }
+    return {float(roll), float(pitch), float(yaw)};
+  }
+  void set_rpy(const std::array<float, 3> &v) {
+    double cr = std::cos(v[0] / 2.0), sr = std::sin(v[0] / 2.0), cp = std::cos(v[1] / 2.0), sp = std::sin(v[1] / 2.0),
+           cy = std::cos(v[2] / 2.0), sy = std::sin(v[2] / 2.0);
+    // q = qz(y) * qy(p) * qx(r)  (include/sapien/math/conversion.h:59-61)
+    q = {float(cy * cp * cr + sy * sp * sr), float(cy * cp * sr - sy * sp * cr), float(cy * sp * cr + sy * cp * sr),
+         float(sy * cp * cr - cy * sp * sr)};
+  }
+};
+
+Pose pose_from_object(const py::object &o) {
+  if (py::isinstance<Pose>(o)) return o.cast<Pose>();
+  Pose r;
+  auto p = o.attr("p").cast<std::vector<float>>();
+  auto q = o.attr("q").cast<std::vector<float>>();
+  if (p.size() != 3 || q.size() != 4) throw py::value_error("pose needs p (3) and q (4)");
+  for (int i = 0; i < 3; ++i) r.p[i] = p[i];
+  for (int i = 0; i < 4; ++i) r.q[i] = q[i];
+  return r;
+}
+
+// ------------------------------------------------------------------ DLPack intake
+struct DLIn {
+  dlabi::ManagedTensor *mt = nullptr;
+  void *ptr = nullptr;
+  ~DLIn() {
+    if (mt && mt->deleter) mt->deleter(mt);
+  }
+};
+
+std::unique_ptr<DLIn> take(const py::object &t, uint8_t code, const std::vector<int64_t> &shape, int device) {
+  py::object cap = t.attr("__dlpack__")();
+  PyObject *c = cap.ptr();
+  if (!PyCapsule_IsValid(c, "dltensor")) throw py::type_error("__dlpack__ did not return a dltensor capsule");
+  auto *mt = static_cast<dlabi::ManagedTensor *>(PyCapsule_GetPointer(c, "dltensor"));
+  PyCapsule_SetName(c, "used_dltensor");
+  auto in = std::make_unique<DLIn>();
+  in->mt = mt;
+  const dlabi::Tensor &d = mt->dl_tensor;
+  if (d.device.device_type != dlabi::kROCM)
+    throw py::value_error("batched methods need a GPU tensor (DLPack device kDLROCM)");
+  if (d.device.device_id != device) throw py::value_error("tensor is on a different GPU than the model");
+  if (d.dtype.code != code || d.dtype.bits != 32 || d.dtype.lanes != 1) throw py::value_error("unexpected dtype");
+  if (d.ndim != (int32_t)shape.size()) throw py::value_error("unexpected tensor rank");
+  int64_t numel = 1;
+  for (int i = 0; i < d.ndim; ++i) numel *= d.shape[i];
+  int64_t stride = 1;
+  for (int i = d.ndim - 1; i >= 0; --i) {
+    if (shape[i] >= 0 && d.shape[i] != shape[i]) throw py::value_error("unexpected tensor shape");
+    if (numel > 0 && d.strides && d.shape[i] > 1 && d.strides[i] != stride)
+      throw py::value_error("tensor must be contiguous");
+    stride *= d.shape[i];
+  }
+  in->ptr = static_cast<char *>(d.data) + d.byte_offset;
+  return in;
+}
+
+// ------------------------------------------------------------------ model
+class PyModel {
+ public:
+  PyModel(const std::string &urdf, const std::vector<double> &gravity) {
+    if (gravity.size() != 3) throw py::value_error("gravity must have 3 entries");
+    check(sapamd_model_create(urdf.c_str(), gravity.data(), &m_));
+  }
+  ~PyModel() { sapamd_model_destroy(m_); }
+  PyModel(const PyModel &) = delete;
+
+  sapamd_info info() const {
+    sapamd_info i;
+    check(sapamd_model_info(m_, &i));
+    return i;
+  }
+  int nv() const { return info().nv; }
+
+  void set_link_order(const std::vector<std::string> &names) {
+    std::vector<const char *> p;
+    for (auto &s : names) p.push_back(s.c_str());
+    check(sapamd_set_link_order(m_, p.data(), (int32_t)p.size()));
+  }
+  void set_joint_order(const std::vector<std::string> &names) {
+    std::vector<const char *> p;
+    for (auto &s : names) p.push_back(s.c_str());
+    check(sapamd_set_joint_order(m_, p.data(), (int32_t)p.size()));
+  }
+  void compute_forward_kinematics(const DArr &q) { check(sapamd_compute_forward_kinematics(m_, q.data(), q.size())); }
+  Pose get_link_pose(int64_t link) {
+    float p7[7];
+    check(sapamd_get_link_pose(m_, (int32_t)clamp_index(link), p7));
+    Pose r;
+    for (int i = 0; i < 3; ++i) r.p[i] = p7[i];
+    for (int i = 0; i < 4; ++i) r.q[i] = p7[3 + i];
+    return r;
+  }
+  void compute_full_jacobian(const DArr &q) { check(sapamd_compute_full_jacobian(m_, q.data(), q.size())); }
+  py::array_t<double> get_link_jacobian(int64_t link, bool local) {
+    int n = nv();
+    py::array_t<double> out({6, n});
+    check(sapamd_get_link_jacobian(m_, (int32_t)clamp_index(link), local, out.mutable_data()));
+    return out;
+  }
+  py::array_t<double> compute_single_link_local_jacobian(const DArr &q, int64_t link) {
+    int n = nv();
+    py::array_t<double> out({6, n});
+    check(sapamd_compute_single_link_local_jacobian(m_, q.data(), q.size(), (int32_t)clamp_index(link),
+                                                    out.mutable_data()));
+    return out;
+  }
+  py::array_t<double> compute_generalized_mass_matrix(const DArr &q) {
+    int n = nv();
+    py::array_t<double> out({n, n});
+    check(sapamd_compute_generalized_mass_matrix(m_, q.data(), q.size(), out.mutable_data()));
+    return out;
+  }
+  py::array_t<double> compute_coriolis_matrix(const DArr &q, const DArr &qd) {
+    int n = nv();
+    same_len(q, qd);
+    py::array_t<double> out({n, n});
+    check(sapamd_compute_coriolis_matrix(m_, q.data(), qd.data(), q.size(), out.mutable_data()));
+    return out;
+  }
+  py::array_t<double> compute_inverse_dynamics(const DArr &q, const DArr &qd, const DArr &qdd) {
+    same_len(q, qd);
+    same_len(q, qdd);
+    py::array_t<double> out(nv());
+    check(sapamd_compute_inverse_dynamics(m_, q.data(), qd.data(), qdd.data(), q.size(), out.mutable_data()));
+    return out;
+  }
+  py::array_t<double> compute_forward_dynamics(const DArr &q, const DArr &qd, const DArr &qf) {
+    same_len(q, qd);
+    same_len(q, qf);
+    py::array_t<double> out(nv());
+    check(sapamd_compute_forward_dynamics(m_, q.data(), qd.data(), qf.data(), q.size(), out.mutable_data()));
+    return out;
+  }
+  py::tuple compute_inverse_kinematics(int64_t link, const py::object &pose, const DArr &q0, const IArr &mask,
+                                       double eps, int max_iterations, double dt, double damp) {
+    Pose P = pose_from_object(pose);
+    float p7[7] = {P.p[0], P.p[1], P.p[2], P.q[0], P.q[1], P.q[2], P.q[3]};
+    int n = nv();
+    py::array_t<double> q(n), err(6);
+    int32_t ok = 0;
+    check(sapamd_compute_inverse_kinematics(m_, (int32_t)clamp_index(link), p7, q0.size() ? q0.data() : nullptr,
+                                            q0.size(), mask.size() ? mask.data() : nullptr, mask.size(), eps,
+                                            max_iterations, dt, damp, q.mutable_data(), &ok, err.mutable_data()));
+    return py::make_tuple(q, bool(ok), err);
+  }
+  py::array_t<double> get_random_qpos() {
+    int n = nv();
+    std::vector<double> lo(n), hi(n);
+    check(sapamd_joint_limits(m_, lo.data(), hi.data()));
+    py::array_t<double> out(n);
+    for (int i = 0; i < n; ++i) out.mutable_data()[i] = std::uniform_real_distribution<double>(lo[i], hi[i])(rng_);
+    return out;
+  }
+  py::tuple joint_limits() {
+    int n = nv();
+    py::array_t<double> lo(n), hi(n);
+    check(sapamd_joint_limits(m_, lo.mutable_data(), hi.mutable_data()));
+    return py::make_tuple(lo, hi);
+  }
+
+  // ---------------------------------------------------------------- batched (DLPack, FP32)
+  py::object fk_jac_rnea_batched(py::object q, py::object qd, py::object qdd, bool poses, bool jac, bool rnea,
+                                 py::object stream) {
+    Batch b(*this, q);
+    py::object qdT, qddT, P = py::none(), J = py::none(), T = py::none();
+    std::unique_ptr<DLIn> vqd, vqdd, vP, vJ, vT;
+    if (rnea) {
+      qdT = b.input(qd);
+      qddT = b.input(qdd);
+      vqd = b.view(qdT);
+      vqdd = b.view(qddT);
+      T = b.empty({b.B, b.n});
+      vT = b.view(T);
+    }
+    if (poses) {
+      P = b.empty({b.B, info().nlinks, 7});
+      vP = b.view(P);
+    }
+    if (jac) {
+      J = b.empty({b.B, 6, b.n});
+      vJ = b.view(J);
+    }
+    auto vq = b.view(b.q);
+    check(sapamd_fk_jac_rnea_batched(m_, b.B, f(vq), rnea ? f(vqd) : nullptr, rnea ? f(vqdd) : nullptr,
+                                     poses ? f(vP) : nullptr, jac ? f(vJ) : nullptr, rnea ? f(vT) : nullptr,
+                                     b.stream(stream)));
+    return py::make_tuple(P, J, T);
+  }
+  py::object fk_batched(py::object q, py::object stream) {
+    return fk_jac_rnea_batched(q, py::none(), py::none(), true, false, false, stream).cast<py::tuple>()[0];
+  }
+  py::object jac_batched(py::object q, py::object stream) {
+    return fk_jac_rnea_batched(q, py::none(), py::none(), false, true, false, stream).cast<py::tuple>()[1];
+  }
+  py::object rnea_batched(py::object q, py::object qd, py::object qdd, py::object stream) {
+    return fk_jac_rnea_batched(q, qd, qdd, false, false, true, stream).cast<py::tuple>()[2];
+  }
+  py::object link_jacobians_batched(py::object q, py::object stream) {
+    Batch b(*this, q);
+    int L = info().nlinks;
+    py::object Jw = b.empty({b.B, L, 6, b.n}), Jl = b.empty({b.B, L, 6, b.n});
+    auto vq = b.view(b.q);
+    auto vw = b.view(Jw), vl = b.view(Jl);
+    check(sapamd_link_jacobians_batched(m_, b.B, f(vq), f(vw), f(vl), b.stream(stream)));
+    return py::make_tuple(Jw, Jl);
+  }
+  py::object crba_batched(py::object q, py::object stream) {
+    Batch b(*this, q);
+    py::object M = b.empty({b.B, b.n, b.n});
+    auto vq = b.view(b.q);
+    auto vM = b.view(M);
+    check(sapamd_crba_batched(m_, b.B, f(vq), f(vM), b.stream(stream)));
+    return M;
+  }
+  py::object coriolis_batched(py::object q, py::object qd, py::object stream) {
+    Batch b(*this, q);
+    py::object qdT = b.input(qd), C = b.empty({b.B, b.n, b.n});
+    auto vq = b.view(b.q), vqd = b.view(qdT), vC = b.view(C);
+    check(sapamd_coriolis_batched(m_, b.B, f(vq), f(vqd), f(vC), b.stream(stream)));
+    return C;
+  }
+  py::object aba_batched(py::object q, py::object qd, py::object tau, py::object stream) {
+    Batch b(*this, q);
+    py::object qdT = b.input(qd), tT = b.input(tau), A = b.empty({b.B, b.n});
+    auto vq = b.view(b.q), vqd = b.view(qdT), vt = b.view(tT), vA = b.view(A);
+    check(sapamd_aba_batched(m_, b.B, f(vq), f(vqd), f(vt), f(vA), b.stream(stream)));
+    return A;
+  }
+  py::object clik_batched(int64_t link, py::object target, py::object q0, py::object mask, double eps,
+                          int max_iterations, double dt, double damp, py::object stream) {
+    Batch b(*this, target, 7);
+    py::object q0T = q0.is_none() ? py::none() : b.input(q0);
+    py::object Q = b.empty({b.B, b.n}), E = b.empty({b.B, 6});
+    py::object S = b.empty_i32({b.B}), I = b.empty_i32({b.B});
+    py::object maskT = py::none();
+    std::unique_ptr<DLIn> vmask, vq0;
+    if (!mask.is_none()) {
+      maskT = b.input_i32(mask, b.n);
+      vmask = take(maskT, dlabi::kInt, {b.n}, b.device);
+    }
+    if (!q0T.is_none()) vq0 = b.view(q0T);
+    auto vt = b.view(b.q), vQ = b.view(Q), vE = b.view(E);
+    auto vS = take(S, dlabi::kInt, {b.B}, b.device), vI = take(I, dlabi::kInt, {b.B}, b.device);
+    check(sapamd_clik_batched(m_, b.B, (int32_t)clamp_index(link), f(vt), vq0 ? f(vq0) : nullptr,
+                              vmask ? static_cast<int32_t *>(vmask->ptr) : nullptr, eps, max_iterations, dt, damp,
+                              f(vQ), static_cast<int32_t *>(vS->ptr), f(vE), static_cast<int32_t *>(vI->ptr),
+                              b.stream(stream)));
+    return py::make_tuple(Q, S.attr("bool")(), E, I);
+  }
+
+  sapamd_model *raw() const { return m_; }
+
+ private:
+  sapamd_model *m_ = nullptr;
+  std::mt19937_64 rng_{std::random_device{}()};
+
+  static int64_t clamp_index(int64_t i) { return i < 0 || i > INT32_MAX ? INT32_MAX : i; }
+  static void same_len(const DArr &a, const DArr &b) {
+    if (a.size() != b.size()) throw std::runtime_error("qpos, qvel, qacc/qf must have the same length");
+  }
+  static float *f(const std::unique_ptr<DLIn> &v) { return static_cast<float *>(v->ptr); }
+
+  // Normalizes batched inputs: any tensor / array-like becomes a contiguous
+  // float32 tensor on the model's GPU, shape (B, ncols).
+  struct Batch {
+    PyModel &self;
+    py::module_ torch;
+    py::object dev, q;
+    int device;
+    int64_t B = 0, n = 0;
+    Batch(PyModel &s, py::object q_in, int64_t ncols = -1) : self(s), torch(py::module_::import("torch")) {
+      sapamd_info i = s.info();
+      device = i.device;
+      n = i.nv;
+      dev = torch.attr("device")("cuda", device);
+      q = input(q_in, ncols < 0 ? n : ncols);
+      B = q.attr("shape").cast<py::tuple>()[0].cast<int64_t>();
+    }
+    py::object input(py::object x, int64_t ncols = -1) {
+      if (ncols < 0) ncols = n;
+      if (x.is_none()) throw py::value_error("missing batched input");
+      if (!py::isinstance(x, torch.attr("Tensor"))) x = torch.attr("as_tensor")(x);
+      x = x.attr("detach")().attr("to")(py::arg("device") = dev, py::arg("dtype") = torch.attr("float32"));
+      if (x.attr("dim")().cast<int>() == 1) x = x.attr("unsqueeze")(0);
+      x = x.attr("contiguous")();
+      auto shp = x.attr("shape").cast<py::tuple>();
+      if (shp.size() != 2 || shp[1].cast<int64_t>() != ncols)
+        throw std::runtime_error("batched input must have shape (B, " + std::to_string(ncols) + ")");
+      if (B && shp[0].cast<int64_t>() != B) throw std::runtime_error("batched inputs disagree on B");
+      return x;
+    }
+    py::object input_i32(py::object x, int64_t len) {
+      if (!py::isinstance(x, torch.attr("Tensor"))) x = torch.attr("as_tensor")(x);
+      x = x.attr("detach")().attr("to")(py::arg("device") = dev, py::arg("dtype") = torch.attr("int32")).attr("contiguous")();
+      if (x.attr("numel")().cast<int64_t>() != len) throw std::runtime_error("active_qmask must have length nv");
+      return x.attr("reshape")(len);
+    }
+    py::object empty(std::vector<int64_t> shape) {
+      return torch.attr("empty")(py::cast(shape), py::arg("device") = dev, py::arg("dtype") = torch.attr("float32"));
+    }
+    py::object empty_i32(std::vector<int64_t> shape) {
+      return torch.attr("empty")(py::cast(shape), py::arg("device") = dev, py::arg("dtype") = torch.attr("int32"));
+    }
+    std::unique_ptr<DLIn> view(const py::object &t) {
+      auto shp = t.attr("shape").cast<std::vector<int64_t>>();
+      return take(t, dlabi::kFloat, shp, device);
+    }
+    void *stream(const py::object &s) {
+      if (!s.is_none()) return reinterpret_cast<void *>(s.cast<intptr_t>());
+      py::object cs = torch.attr("cuda").attr("current_stream")(dev);
+      return reinterpret_cast<void *>(cs.attr("cuda_stream").cast<intptr_t>());
+    }
+  };
+};
+
+}  // namespace
+
+PYBIND11_MODULE(_pinocchio, m) {
+  m.doc() = "MI355X-native PinocchioModel (HIP kernels behind a C ABI)";
+  m.def("version", [] { return std::string(sapamd_version()); });
+  m.def("_abi_handle", [](PyModel &pm) { return reinterpret_cast<intptr_t>(pm.raw()); },
+        "raw sapamd_model* of a model, for ctypes callers of the C ABI");
+
+  py::class_<Pose>(m, "Pose")
+      .def(py::init([](std::vector<float> p, std::vector<float> q) {
+             if (p.size() != 3 || q.size() != 4) throw py::value_error("Pose(p: 3, q: 4)");
+             Pose r;
+             for (int i = 0; i < 3; ++i) r.p[i] = p[i];
+             for (int i = 0; i < 4; ++i) r.q[i] = q[i];
+             return r;
+           }),
+           py::arg("p") = std::vector<float>{0, 0, 0}, py::arg("q") = std::vector<float>{1, 0, 0, 0})
+      .def(py::init([](const DArr &T) {
+             if (T.ndim() != 2 || T.shape(0) != 4 || T.shape(1) != 4) throw py::value_error("expected a 4x4 matrix");
+             const double *t = T.data();
+             double R[9] = {t[0], t[1], t[2], t[4], t[5], t[6], t[8], t[9], t[10]};
+             Pose r;
+             r.p = {float(t[3]), float(t[7]), float(t[11])};
+             r.q = Pose::quat_of(R);
+             return r;
+           }),
+           py::arg("matrix"))
+      .def_property(
+          "p", [](const Pose &s) { return py::array_t<float>(3, s.p.data()); },
+          [](Pose &s, std::vector<float> v) { for (int i = 0; i < 3; ++i) s.p[i] = v.at(i); })
+      .def_property(
+          "q", [](const Pose &s) { return py::array_t<float>(4, s.q.data()); },
+          [](Pose &s, std::vector<float> v) { for (int i = 0; i < 4; ++i) s.q[i] = v.at(i); })
+      .def("get_p", [](const Pose &s) { return py::array_t<float>(3, s.p.data()); })
+      .def("get_q", [](const Pose &s) { return py::array_t<float>(4, s.q.data()); })
+      .def("set_p", [](Pose &s, std::vector<float> v) { for (int i = 0; i < 3; ++i) s.p[i] = v.at(i); })
+      .def("set_q", [](Pose &s, std::vector<float> v) { for (int i = 0; i < 4; ++i) s.q[i] = v.at(i); })
+      .def_property(
+          "rpy", [](const Pose &s) { auto r = s.rpy(); return py::array_t<float>(3, r.data()); },
+          [](Pose &s, std::vector<float> v) { s.set_rpy({v.at(0), v.at(1), v.at(2)}); })
+      .def("get_rpy", [](const Pose &s) { auto r = s.rpy(); return py::array_t<float>(3, r.data()); })
+      .def("set_rpy", [](Pose &s, std::vector<float> v) { s.set_rpy({v.at(0), v.at(1), v.at(2)}); })
+      .def("inv", &Pose::inv)
+      .def("__mul__", &Pose::mul)
+      .def("to_transformation_matrix",
+           [](const Pose &s) {
+             auto R = s.matrix();
+             py::array_t<float> T({4, 4});
+             float *t = T.mutable_data();
+             for (int i = 0; i < 16; ++i) t[i] = 0;
+             for (int i = 0; i < 3; ++i) {
+               for (int j = 0; j < 3; ++j) t[4 * i + j] = float(R[3 * i + j]);
+               t[4 * i + 3] = s.p[i];
+             }
+             t[15] = 1;
+             return T;
+           })
+      .def(py::pickle([](const Pose &s) { return py::make_tuple(s.p, s.q); },
+                      [](py::tuple t) {
+                        Pose r;
+                        r.p = t[0].cast<std::array<float, 3>>();
+                        r.q = t[1].cast<std::array<float, 4>>();
+                        return r;
+                      }))
+      .def("__repr__", [](const Pose &s) {
+        std::ostringstream os;
+        os << "Pose([" << s.p[0] << ", " << s.p[1] << ", " << s.p[2] << "], [" << s.q[0] << ", " << s.q[1] << ", "
+           << s.q[2] << ", " << s.q[3] << "])";
+        return os.str();
+      });
+
+  py::class_<PyModel>(m, "PinocchioModel")
+      .def(py::init<const std::string &, const std::vector<double> &>(), py::arg("urdf"), py::arg("gravity"))
+      .def("set_link_order", &PyModel::set_link_order)
+      .def("set_joint_order", &PyModel::set_joint_order)
+      .def("compute_forward_kinematics", &PyModel::compute_forward_kinematics,
+           "Compute and cache forward kinematics. After computation, use get_link_pose to retrieve the computed "
+           "pose for a specific link.",
+           py::arg("qpos"))
+      .def("get_link_pose", &PyModel::get_link_pose,
+           "Given link index, get link pose (in articulation base frame) from forward kinematics. Must be called "
+           "after compute_forward_kinematics.",
+           py::arg("link_index"))
+      .def("compute_inverse_kinematics", &PyModel::compute_inverse_kinematics,
+           "Compute inverse kinematics with the CLIK algorithm (damped least squares on the SE(3) log error).\n"
+           "Returns (qpos, success, error6).",
+           py::arg("link_index"), py::arg("pose"), py::arg("initial_qpos") = DArr(), py::arg("active_qmask") = IArr(),
+           py::arg("eps") = 1e-4, py::arg("max_iterations") = 1000, py::arg("dt") = 0.1, py::arg("damp") = 1e-6)
+      .def("compute_forward_dynamics", &PyModel::compute_forward_dynamics, py::arg("qpos"), py::arg("qvel"),
+           py::arg("qf"))
+      .def("compute_inverse_dynamics", &PyModel::compute_inverse_dynamics, py::arg("qpos"), py::arg("qvel"),
+           py::arg("qacc"))
+      .def("compute_generalized_mass_matrix", &PyModel::compute_generalized_mass_matrix, py::arg("qpos"))
+      .def("compute_coriolis_matrix", &PyModel::compute_coriolis_matrix, py::arg("qpos"), py::arg("qvel"))
+      .def("compute_full_jacobian", &PyModel::compute_full_jacobian, "Compute and cache Jacobian for all links",
+           py::arg("qpos"))
+      .def("get_link_jacobian", &PyModel::get_link_jacobian,
+           "Given link index, get the Jacobian. Must be called after compute_full_jacobian.\n"
+           "local=False: world (spatial) frame; local=True: link (body) frame.",
+           py::arg("link_index"), py::arg("local") = false)
+      .def("compute_single_link_local_jacobian", &PyModel::compute_single_link_local_jacobian,
+           "Compute the link(body) Jacobian for a single link.", py::arg("qpos"), py::arg("link_index"))
+      .def("get_random_qpos", &PyModel::get_random_qpos)
+      .def("joint_limits", &PyModel::joint_limits)
+      .def_property_readonly("nv", [](const PyModel &s) { return s.info().nv; })
+      .def_property_readonly("nq", [](const PyModel &s) { return s.info().nq; })
+      .def_property_readonly("nlinks", [](const PyModel &s) { return s.info().nlinks; })
+      // batched FP32 methods over (B, dof) tensors (DLPack, kDLROCM)
+      .def("compute_forward_kinematics_batched", &PyModel::fk_batched, py::arg("qpos"), py::arg("stream") = py::none(),
+           "Link poses (B, L, 7) = p xyz | q wxyz for a (B, nv) batch.")
+      .def("compute_full_jacobian_batched", &PyModel::jac_batched, py::arg("qpos"), py::arg("stream") = py::none(),
+           "WORLD-frame Jacobian (B, 6, nv).")
+      .def("compute_link_jacobians_batched", &PyModel::link_jacobians_batched, py::arg("qpos"),
+           py::arg("stream") = py::none(), "Per-link Jacobians (world, local), each (B, L, 6, nv).")
+      .def("compute_inverse_dynamics_batched", &PyModel::rnea_batched, py::arg("qpos"), py::arg("qvel"),
+           py::arg("qacc"), py::arg("stream") = py::none())
+      .def("compute_fk_jacobian_rnea_batched",
+           [](PyModel &s, py::object q, py::object qd, py::object qdd, py::object stream) {
+             return s.fk_jac_rnea_batched(q, qd, qdd, true, true, true, stream);
+           },
+           py::arg("qpos"), py::arg("qvel"), py::arg("qacc"), py::arg("stream") = py::none(),
+           "Fused link poses (B, L, 7), Jacobian (B, 6, nv) and RNEA torques (B, nv).")
+      .def("compute_generalized_mass_matrix_batched", &PyModel::crba_batched, py::arg("qpos"),
+           py::arg("stream") = py::none())
+      .def("compute_coriolis_matrix_batched", &PyModel::coriolis_batched, py::arg("qpos"), py::arg("qvel"),
+           py::arg("stream") = py::none())
+      .def("compute_forward_dynamics_batched", &PyModel::aba_batched, py::arg("qpos"), py::arg("qvel"), py::arg("qf"),
+           py::arg("stream") = py::none())
+      .def("compute_inverse_kinematics_batched", &PyModel::clik_batched, py::arg("link_index"), py::arg("poses"),
+           py::arg("initial_qpos") = py::none(), py::arg("active_qmask") = py::none(), py::arg("eps") = 1e-4,
+           py::arg("max_iterations") = 1000, py::arg("dt") = 0.1, py::arg("damp") = 1e-6,
+           py::arg("stream") = py::none(),
+           "Batched CLIK: poses (B, 7) -> (qpos (B, nv), success (B,), error (B, 6), iterations (B,)).");
+}

@@ -1,0 +1,63 @@
+// Device-resident model tables (one contiguous block per model and precision).
+//
+// Layout in HBM: a 128-byte header followed by nj JointRec and nlinks LinkRec,
+// all in the internal depth-first joint order (parents before children). Every
+// kernel reads these with wave-uniform addresses, so they are served by the
+// scalar cache / broadcast from LDS, never per-lane gathers.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+#if !defined(__HIPCC__) && !defined(__host__)
+#define __host__
+#define __device__
+#endif
+
+namespace sapamd {
+
+enum : int32_t { KIND_REVOLUTE = 0, KIND_PRISMATIC = 1, KIND_CONTINUOUS = 2 };
+
+template <class T> struct JointRec {
+  // Joint placement composed with the joint motion, per joint kind:
+  //   revolute/continuous: R_local(q) = A + sin(q) Bq + (1 - cos(q)) Cq, p_local = p
+  //   prismatic:           R_local    = A,                             p_local = p + ap q
+  // with A = R_place, Bq = R_place [a]x, Cq = R_place [a]x^2, ap = R_place a.
+  T A[9], Bq[9], Cq[9];
+  T p[3], ap[3];
+  T axis[3];      // a, the unit joint axis in the joint frame
+  T m, c[3];      // mass and centre of mass (joint frame), fixed children merged
+  T Ic[6];        // rotational inertia about the COM: xx xy xz yy yz zz (joint frame)
+  int32_t parent; // parent joint, -1 = universe
+  int32_t kind;
+  int32_t vuser;  // user (set_joint_order) dof index
+  int32_t depth;  // 1 for children of the universe
+  int32_t nchild;
+  int32_t pad[3];
+};
+
+template <class T> struct LinkRec {
+  T R[9], p[3];   // link frame relative to its joint frame
+  int32_t joint;  // -1 = universe (the root link)
+  int32_t ident;  // 1 when R = I and p = 0
+  int32_t pad[2];
+};
+
+template <class T> struct ModelTable {
+  int32_t nj, nlinks, nv, chain;  // chain = 1 when joint j's parent is j - 1 for all j
+  int32_t max_depth, pad0, pad1, pad2;
+  T gravity[4];
+  // JointRec<T> joints[nj]; LinkRec<T> links[nlinks]; follow
+};
+
+template <class T> __host__ __device__ inline const JointRec<T> *joints_of(const ModelTable<T> *m) {
+  return reinterpret_cast<const JointRec<T> *>(reinterpret_cast<const char *>(m) + 128);
+}
+template <class T> __host__ __device__ inline const LinkRec<T> *links_of(const ModelTable<T> *m) {
+  return reinterpret_cast<const LinkRec<T> *>(reinterpret_cast<const char *>(joints_of(m)) +
+                                              sizeof(JointRec<T>) * m->nj);
+}
+template <class T> inline size_t table_bytes(int nj, int nlinks) {
+  return 128 + sizeof(JointRec<T>) * nj + sizeof(LinkRec<T>) * nlinks;
+}
+
+}  // namespace sapamd

@@ -1,0 +1,71 @@
+"""ctypes view of the C ABI (include/sapien_amd.h).
+
+This is the binding a non-Python host (or a ctypes caller) uses: the same
+``libsapien_amd.so`` the pybind module links against, called with raw device
+pointers. ``launcher_fk_jac_rnea`` returns a minimal-overhead callable used by
+the benchmark and the hipGraph capture tests.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsapien_amd.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "sapien_amd.h")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python -m sapien_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        P, I64, I32, D = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double
+        L.sapamd_last_error.restype = ctypes.c_char_p
+        L.sapamd_version.restype = ctypes.c_char_p
+        L.sapamd_fk_jac_rnea_batched.argtypes = [P, I64, P, P, P, P, P, P, P]
+        L.sapamd_fk_jac_rnea_batched.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    """Every function declared in include/sapien_amd.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(sapamd_[a-z0-9_]+)\s*\(", text)))
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().sapamd_last_error().decode()
+        if rc == 1:
+            raise ValueError(msg)
+        raise RuntimeError(msg)
+
+
+def handle(model):
+    from . import _pinocchio
+
+    return ctypes.c_void_p(_pinocchio._abi_handle(model))
+
+
+def launcher_fk_jac_rnea(model):
+    """f(B, q, qd, qdd, P, J, T, stream) over torch tensors (or raw ints)."""
+    L = lib()
+    h = handle(model)
+    fn = L.sapamd_fk_jac_rnea_batched
+
+    def ptr(x):
+        if x is None:
+            return None
+        return x if isinstance(x, int) else x.data_ptr()
+
+    def launch(B, q, qd, qdd, P, J, T, stream):
+        rc = fn(h, B, ptr(q), ptr(qd), ptr(qdd), ptr(P), ptr(J), ptr(T), stream)
+        if rc:
+            check(rc)
+
+    return launch
