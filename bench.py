@@ -61,6 +61,9 @@ def main():
     if args.kernel != "auto":
         sa.set_kernel_variant(m, args.kernel)
     n, L = rob.nv, rob.nlinks
+    from sapien_amd import tune
+
+    tuned = tune.autotune(m, batch_sizes=(B,), use_cache=False) if sa.kernel_variant(m) == "lanes_per_joint" else {}
 
     # 64 rotating buffer sets, seeded like the workload (seed 1001 + set index)
     sets = []
@@ -116,7 +119,9 @@ def main():
         "dtype": "fp32",
         "data": "synthetic: seeded 7-dof chain (seed 0), q~U(limits), qd~N(0,1), qdd~N(0,var 4), 64 rotating sets",
         "config": {"workload": "chain7 FK (8 link poses) + WORLD Jacobian (6x7) + RNEA, B=16384, FP32",
-                   "envs": B, "dof": n, "links": L, "kernel": sa.kernel_variant(m)},
+                   "envs": B, "dof": n, "links": L, "kernel": sa.kernel_variant(m),
+                   "shape": ({"lanes_per_env": tuned[B][0], "joints_per_lane": tuned[B][1],
+                              "envs_per_wavefront": 64 // tuned[B][0], "autotuned": True} if tuned else None)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel_ms": kern_ms, "bytes_per_launch": BYTES_PER_EVAL * B},

@@ -106,6 +106,25 @@ int sapamd_compute_inverse_kinematics(sapamd_model *m, int32_t link, const float
                                       double *q_out, int32_t *success, double err6[6]);
 
 /* ---------------------------------------------------------------- batched device path (FP32) */
+/* Kernel family used by the batched kinematics entry points (no reference
+ * counterpart; the reference has no GPU path). 0 = auto (lanes-per-joint for
+ * serial chains of <= 16 joints, else thread-per-environment), 1 =
+ * thread-per-environment, 2 = lanes-per-joint (fails for unsupported trees).
+ * Not synchronized with launches in flight on other threads. */
+enum { SAPAMD_KERNEL_AUTO = 0, SAPAMD_KERNEL_THREAD_PER_ENV = 1, SAPAMD_KERNEL_LANES_PER_JOINT = 2 };
+int sapamd_set_kernel_variant(sapamd_model *m, int32_t variant);
+/* The variant the batched kinematics entry points would launch now (1 or 2). */
+int sapamd_get_kernel_variant(const sapamd_model *m, int32_t *variant);
+/* Shapes of the lanes-per-joint chain kernel: G lanes per environment x C joints
+ * per lane. Writes up to cap (G, C) pairs and returns how many exist. */
+int sapamd_chain_shapes(int32_t *G, int32_t *C, int32_t cap);
+/* Use shape (G, C) for batches of at most max_batch environments (the
+ * autotuner's result; the smallest covering entry wins, larger batches use the
+ * last entry's default). G = C = 0 clears every entry. */
+int sapamd_set_chain_shape(sapamd_model *m, int64_t max_batch, int32_t G, int32_t C);
+/* The shape a batch of B environments would launch with. */
+int sapamd_get_chain_shape(const sapamd_model *m, int64_t B, int32_t *G, int32_t *C);
+
 /* Fused FK (every link pose) + full WORLD Jacobian + RNEA over B environments;
  * the throughput-critical entry point. Any of poses/J/tau may be NULL to skip it. */
 int sapamd_fk_jac_rnea_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd,

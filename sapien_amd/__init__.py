@@ -31,13 +31,28 @@ def native_fk_jac_rnea(model):
     return launcher_fk_jac_rnea(model)
 
 
+_VARIANTS = {"auto": 0, "generic": 1, "thread_per_env": 1, "fast": 2, "lanes_per_joint": 2}
+_VARIANT_NAMES = {1: "thread_per_env", 2: "lanes_per_joint"}
+
+
 def kernel_variant(model):
-    return "generic"
+    """Kernel family the batched kinematics methods launch for this model."""
+    import ctypes
+
+    from .native import check, handle, lib
+
+    v = ctypes.c_int32(0)
+    check(lib().sapamd_get_kernel_variant(handle(model), ctypes.byref(v)))
+    return _VARIANT_NAMES[v.value]
 
 
 def set_kernel_variant(model, name):
-    if name != "generic":
+    """Select 'auto', 'thread_per_env' (alias 'generic') or 'lanes_per_joint' (alias 'fast')."""
+    from .native import check, handle, lib
+
+    if name not in _VARIANTS:
         raise ValueError(f"unknown kernel variant {name!r}")
+    check(lib().sapamd_set_kernel_variant(handle(model), _VARIANTS[name]))
 
 
 __all__ = ["PinocchioModel", "Pose", "LIBRARY_PATH", "native_fk_jac_rnea", "kernel_variant", "set_kernel_variant"]

@@ -7,6 +7,7 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#include <algorithm>
 
 #include "../../include/sapien_amd.h"
 #include "launch.h"
@@ -79,6 +80,7 @@ struct sapamd_model {
   double *h_buf = nullptr, *d_buf = nullptr;
   size_t buf_doubles = 0;
   bool tables_ready = false;
+  int variant = SAPAMD_KERNEL_AUTO;
   std::vector<double> pose_cache, jw_cache, jl_cache;
 
   ~sapamd_model() {
@@ -107,6 +109,9 @@ struct sapamd_model {
     hdr->nv = host->nv();
     hdr->chain = host->is_chain() ? 1 : 0;
     hdr->max_depth = host->max_depth();
+    hdr->ident_order = 1;
+    for (int j = 0; j < nj; ++j)
+      if (host->vuser()[j] != j) hdr->ident_order = 0;
     for (int i = 0; i < 3; ++i) hdr->gravity[i] = T(host->gravity()[i]);
     auto *jr = const_cast<JointRec<T> *>(joints_of(hdr));
     std::vector<int> nchild(nj, 0);
@@ -143,6 +148,31 @@ struct sapamd_model {
       r.depth = jt.depth;
       r.nchild = nchild[j];
     }
+    // per-joint user link (for kernels that emit link poses from the joint's lane)
+    std::vector<int> per_joint(nj, 0);
+    int on_root = 0;
+    hdr->root_link = -1;
+    for (int j = 0; j < nj; ++j) {
+      jr[j].link = -1;
+      jr[j].diagI = (jr[j].Ic[1] == T(0) && jr[j].Ic[2] == T(0) && jr[j].Ic[4] == T(0)) ? 1 : 0;
+    }
+    for (int l = 0; l < nl; ++l) {
+      const Frame &f = F[lf[l]];
+      if (f.joint < 0) {
+        ++on_root;
+        hdr->root_link = l;
+        for (int k = 0; k < 9; ++k) hdr->root_R[k] = T(f.place.R[k]);
+        for (int k = 0; k < 3; ++k) hdr->root_p[k] = T(f.place.p[k]);
+      } else {
+        ++per_joint[f.joint];
+        jr[f.joint].link = l;
+        for (int k = 0; k < 9; ++k) jr[f.joint].lR[k] = T(f.place.R[k]);
+        for (int k = 0; k < 3; ++k) jr[f.joint].lp[k] = T(f.place.p[k]);
+      }
+    }
+    hdr->links_le1 = on_root <= 1;
+    for (int j = 0; j < nj; ++j)
+      if (per_joint[j] > 1) hdr->links_le1 = 0;
     auto *lr = const_cast<LinkRec<T> *>(links_of(hdr));
     for (int l = 0; l < nl; ++l) {
       const Frame &f = F[lf[l]];
@@ -261,6 +291,31 @@ struct sapamd_model {
     HIP_OK(hipMemcpyAsync(h_buf + off, d_buf + off, n * sizeof(double), hipMemcpyDeviceToHost, stream));
     HIP_OK(hipStreamSynchronize(stream));
     std::memcpy(dst, h_buf + off, n * sizeof(double));
+  }
+  int effective_variant() const {
+    bool fast_ok = host->njoints() >= 1 && host->is_chain() && fast_chain_group(host->njoints()) > 0;
+    if (variant == SAPAMD_KERNEL_THREAD_PER_ENV) return SAPAMD_KERNEL_THREAD_PER_ENV;
+    if (variant == SAPAMD_KERNEL_LANES_PER_JOINT) {
+      if (!fast_ok) throw std::runtime_error("lanes-per-joint kernels need a serial chain of at most 16 joints");
+      return SAPAMD_KERNEL_LANES_PER_JOINT;
+    }
+    return fast_ok ? SAPAMD_KERNEL_LANES_PER_JOINT : SAPAMD_KERNEL_THREAD_PER_ENV;
+  }
+  // Chain kernel shape (G lanes per environment x C joints per lane) per batch
+  // size bucket: tuned entries override the default.
+  struct ShapeEntry {
+    int64_t max_batch;
+    int G, C;
+  };
+  std::vector<ShapeEntry> shapes;
+  void shape_for(int64_t B, int *G, int *C) const {
+    for (const auto &s : shapes)
+      if (B <= s.max_batch) {
+        *G = s.G;
+        *C = s.C;
+        return;
+      }
+    default_chain_shape(host->njoints(), G, C);
   }
   const ModelTable<double> *t64() const { return static_cast<const ModelTable<double> *>(tab64); }
   const ModelTable<float> *t32() const { return static_cast<const ModelTable<float> *>(tab32); }
@@ -535,6 +590,70 @@ void need(const void *p, const char *what) {
 hipStream_t S(void *s) { return static_cast<hipStream_t>(s); }
 }  // namespace
 
+int sapamd_set_kernel_variant(sapamd_model *m, int32_t variant) {
+  return guarded([&] {
+    if (variant < 0 || variant > 2) throw std::invalid_argument("unknown kernel variant");
+    int old = m->variant;
+    m->variant = variant;
+    try {
+      (void)m->effective_variant();
+    } catch (...) {
+      m->variant = old;
+      throw;
+    }
+  });
+}
+
+int sapamd_get_kernel_variant(const sapamd_model *m, int32_t *variant) {
+  return guarded([&] { *variant = m->effective_variant(); });
+}
+
+int sapamd_chain_shapes(int32_t *G, int32_t *C, int32_t cap) {
+  int n = chain_shape_count();
+  for (int i = 0; i < n && i < cap; ++i) {
+    int g, c;
+    chain_shape(i, &g, &c);
+    G[i] = g;
+    C[i] = c;
+  }
+  return n;
+}
+
+int sapamd_set_chain_shape(sapamd_model *m, int64_t max_batch, int32_t G, int32_t C) {
+  return guarded([&] {
+    if (G == 0 && C == 0) {  // clear all tuned entries
+      m->shapes.clear();
+      return;
+    }
+    bool known = false;
+    for (int i = 0; i < chain_shape_count(); ++i) {
+      int g, c;
+      chain_shape(i, &g, &c);
+      known |= (g == G && c == C);
+    }
+    if (!known) throw std::invalid_argument("unsupported chain kernel shape");
+    if (G * C < m->host->njoints()) throw std::invalid_argument("shape covers fewer joints than the model has");
+    auto &v = m->shapes;
+    for (auto &s : v)
+      if (s.max_batch == max_batch) {
+        s.G = G;
+        s.C = C;
+        return;
+      }
+    v.push_back({max_batch, G, C});
+    std::sort(v.begin(), v.end(), [](const auto &a, const auto &b) { return a.max_batch < b.max_batch; });
+  });
+}
+
+int sapamd_get_chain_shape(const sapamd_model *m, int64_t B, int32_t *G, int32_t *C) {
+  return guarded([&] {
+    int g, c;
+    m->shape_for(B, &g, &c);
+    *G = g;
+    *C = c;
+  });
+}
+
 int sapamd_fk_jac_rnea_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd,
                                const float *qdd, float *poses, float *J, float *tau, void *stream) {
   return guarded([&] {
@@ -546,7 +665,13 @@ int sapamd_fk_jac_rnea_batched(const sapamd_model *m, int64_t B, const float *q,
     if (poses) { flags |= KF_POSES; o.poses = poses; if (m->L() == 0) throw std::runtime_error("link index out of bound"); }
     if (J) { flags |= KF_JFULL; o.jfull = J; }
     if (tau) { need(qd, "qd"); need(qdd, "qdd"); flags |= KF_RNEA; o.tau = tau; }
-    HIP_OK(launch_kinematics<float>(m->t32(), m->host->njoints(), B, flags, q, qd, qdd, o, S(stream)));
+    if (m->effective_variant() == SAPAMD_KERNEL_LANES_PER_JOINT) {
+      int G, C;
+      m->shape_for(B, &G, &C);
+      HIP_OK(launch_chain_fused(m->t32(), m->host->njoints(), B, flags, q, qd, qdd, o, S(stream), G, C));
+    }
+    else
+      HIP_OK(launch_kinematics<float>(m->t32(), m->host->njoints(), B, flags, q, qd, qdd, o, S(stream)));
   });
 }
 

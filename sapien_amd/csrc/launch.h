@@ -53,6 +53,16 @@ template <class T>
 hipError_t launch_clik(const ModelTable<T> *tab, int nj, int64_t B, const ClikParams &P, const T *target,
                        const T *q0, T *q_out, int32_t *success, T *err, int32_t *iters, hipStream_t s);
 
+// Lanes-per-joint fused FK + Jacobian + RNEA for serial chains of <= 16 joints
+// (FP32), G lanes per environment x C joints per lane. Returns
+// hipErrorNotSupported for other trees or unsupported shapes.
+int fast_chain_group(int nj);
+int chain_shape_count();
+void chain_shape(int i, int *G, int *C);
+void default_chain_shape(int nj, int *G, int *C);
+hipError_t launch_chain_fused(const ModelTable<float> *tab, int nj, int64_t B, int flags, const float *q,
+                              const float *qd, const float *qdd, KinOut<float> out, hipStream_t s, int G, int C);
+
 // dst[0:n] = value (a kernel, not hipMemsetAsync, so it is safe under graph capture)
 hipError_t launch_fill_i32(int32_t *dst, int32_t value, int64_t n, hipStream_t s);
 

@@ -150,6 +150,15 @@ def chain7(seed=0):
     return _finish(b, "chain7", [0] + idx, idx, ee_user=7)
 
 
+def chain(n, seed=0):
+    """Serial revolute chain of n joints (n + 1 links), generated like chain7."""
+    b = _Builder(seed)
+    b.link("link_0", None)
+    first = b.offset()
+    idx = _arm(b, "link_0", 1, n, False, first)
+    return _finish(b, f"chain{n}", [0] + idx, idx, ee_user=n)
+
+
 def arm9(seed=1):
     b = _Builder(seed)
     b.link("link_0", None)

@@ -230,3 +230,56 @@ def test_batched_clik():
         qr, okr, errr, itr = o.clik(7, tgt[b])
         agree += okr == ok[b]
     assert agree >= 28
+
+
+# ----------------------------------------------------------------- kernel families agree
+@pytest.mark.parametrize("n", [3, 7, 12, 16])
+def test_lanes_per_joint_matches_oracle_and_generic(n):
+    import sapien_amd as sa
+
+    rob = S.chain(n, seed=40 + n)
+    m, o = _pair(rob)
+    assert sa.kernel_variant(m) == "lanes_per_joint"
+    q, qd, qdd, _ = S.samples(rob, 1000, 1100 + n)
+    _check_kin(m, o, q, qd, qdd)
+    fast = m.compute_fk_jacobian_rnea_batched(_t(q), _t(qd), _t(qdd))
+    sa.set_kernel_variant(m, "thread_per_env")
+    assert sa.kernel_variant(m) == "thread_per_env"
+    gen = m.compute_fk_jacobian_rnea_batched(_t(q), _t(qd), _t(qdd))
+    for a, b in zip(fast, gen):
+        assert rel_err(a.cpu().numpy(), b.cpu().numpy().astype(np.float64)) <= REL_TOL
+    sa.set_kernel_variant(m, "auto")
+    # trees fall back to the thread-per-env kernels; forcing lanes-per-joint is an error
+    m9, _ = _pair(S.arm9())
+    assert sa.kernel_variant(m9) == "thread_per_env"
+    with pytest.raises(RuntimeError):
+        sa.set_kernel_variant(m9, "lanes_per_joint")
+
+
+def test_fast_kernel_partial_outputs_and_ragged_batches():
+    rob = S.chain7()
+    m, o = _pair(rob)
+    for B in (1, 5, 31, 33, 1000):
+        q, qd, qdd, _ = S.samples(rob, B, 1200 + B)
+        _check_kin(m, o, q, qd, qdd)
+        P = m.compute_forward_kinematics_batched(_t(q))
+        J = m.compute_full_jacobian_batched(_t(q))
+        T = m.compute_inverse_dynamics_batched(_t(q), _t(qd), _t(qdd))
+        P2, J2, T2 = m.compute_fk_jacobian_rnea_batched(_t(q), _t(qd), _t(qdd))
+        assert torch.equal(P, P2) and torch.equal(J, J2) and torch.equal(T, T2)
+
+
+@pytest.mark.parametrize("n", [7, 12])
+def test_every_chain_shape_matches_oracle(n):
+    from sapien_amd import tune
+
+    rob = S.chain7() if n == 7 else S.chain(n, seed=3)
+    m, o = _pair(rob)
+    q, qd, qdd, _ = S.samples(rob, 300, 1300 + n)
+    for G, C in tune.shapes():
+        if G * C < n:
+            continue
+        tune.set_shape(m, 1 << 40, G, C)
+        assert tune.get_shape(m, 300) == (G, C)
+        _check_kin(m, o, q, qd, qdd)
+    tune.set_shape(m, 0, 0, 0)  # clear tuned entries
