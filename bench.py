@@ -127,6 +127,53 @@ def main():
                      "kernel_ms": kern_ms, "bytes_per_launch": BYTES_PER_EVAL * B},
     }
 
+    # gradient throughput: 8192 environments of the 7-dof and 9-dof trees,
+    # loss = |p_ee - FK(q*)|^2 + |tau_RNEA(q, qd, qdd) - tau*|^2. Forward-only
+    # = the fused FK + RNEA evaluation of that loss's inputs; forward + backward
+    # = the fused loss-and-gradient kernel (dq, dqd, dqdd).
+    from sapien_amd.native import launcher_pose_torque_loss
+
+    grads = {}
+    for gname in ("chain7", "arm9"):
+        grob = S.robot(gname)
+        gm = m if gname == "chain7" else sa.PinocchioModel(grob.urdf, [0.0, 0.0, -9.81])
+        if gname != "chain7":
+            gm.set_joint_order(grob.joint_names)
+            gm.set_link_order(grob.link_names)
+        GB, gn, gL = 8192, grob.nv, grob.nlinks
+        gq, gqd, gqdd, gts = [torch.from_numpy(x).to(dev) for x in S.samples(grob, GB, 1003, dtype=np.float32)]
+        gt = torch.from_numpy(S.samples(grob, GB, 1013, dtype=np.float32)[0]).to(dev)
+        gt = gm.compute_forward_kinematics_batched(gt)[:, grob.ee_link, :3].contiguous()
+        bufs = dict(P=torch.empty(GB, gL, 7, device=dev), T=torch.empty(GB, gn, device=dev),
+                    loss=torch.empty(GB, device=dev), dq=torch.empty(GB, gn, device=dev),
+                    dqd=torch.empty(GB, gn, device=dev), dqdd=torch.empty(GB, gn, device=dev))
+        fwd = sa.native_fk_jac_rnea(gm)
+        fb = launcher_pose_torque_loss(gm)
+        if sa.kernel_variant(gm) == "lanes_per_joint":
+            tune.autotune(gm, batch_sizes=(GB,), use_cache=False)
+
+        def run_f():
+            fwd(GB, gq, gqd, gqdd, bufs["P"], None, bufs["T"], stream.cuda_stream)
+
+        def run_fb():
+            fb(GB, gq, gqd, gqdd, gts, grob.ee_link, gt, bufs["loss"], None, bufs["dq"], bufs["dqd"], bufs["dqdd"],
+               stream.cuda_stream)
+
+        res = {}
+        for key, fn in (("forward_evals_s", run_f), ("fwd_bwd_evals_s", run_fb)):
+            for _ in range(20):
+                fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(500):
+                fn()
+            torch.cuda.synchronize()
+            res[key] = GB * 500 / (time.perf_counter() - t0)
+        res["ratio"] = res["fwd_bwd_evals_s"] / res["forward_evals_s"]
+        res["kernel"] = sa.kernel_variant(gm)
+        grads[gname] = res
+    out["gradients"] = grads
+
     # plain per-call drop-in path on the GPU (FP64, one configuration per call)
     q64, qd64, qdd64, _ = S.samples(rob, 256, 1001)
     t0 = time.perf_counter()

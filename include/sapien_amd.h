@@ -155,6 +155,23 @@ int sapamd_clik_batched(const sapamd_model *m, int64_t B, int32_t link, const fl
                         const int32_t *mask, double eps, int32_t max_iterations, double dt, double damp,
                         float *q_out, int32_t *success, float *err, int32_t *iterations, void *stream);
 
+/* ---------------------------------------------------------------- adjoints (FP32, batched)
+ * No reference counterpart: the reference has no derivatives. Reverse mode of
+ * the batched kinematics above, for training losses through the model. */
+/* Vector-Jacobian product of link poses and RNEA torques: given upstream
+ * gradients gposes (B, L, 7) and/or gtau (B, nv) (either may be NULL; gtau
+ * needs qd/qdd), writes dq (B, nv) and, when gtau is given, dqd and dqdd
+ * (B, nv). The forward pass is recomputed inside the kernel. */
+int sapamd_kin_vjp_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd, const float *qdd,
+                           const float *gposes, const float *gtau, float *dq, float *dqd, float *dqdd, void *stream);
+/* Fused loss and gradient: loss_b = sum_i (tau_i - tau*_i)^2 + |p_ee - t|^2 with
+ * tau = RNEA(q, qd, qdd) and p_ee the position of user link ee_link; tau_star
+ * or p_target may be NULL to drop that term. Writes loss (B) and dq, dqd, dqdd
+ * (B, nv); tau (B, nv) may be non-NULL to also receive the torques. */
+int sapamd_pose_torque_loss_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd,
+                                    const float *qdd, const float *tau_star, int32_t ee_link, const float *p_target,
+                                    float *loss, float *tau, float *dq, float *dqd, float *dqdd, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -777,4 +777,87 @@ int sapamd_clik_batched(const sapamd_model *m, int64_t B, int32_t link, const fl
   });
 }
 
+namespace {
+// Chain kernels need one user link per joint for pose wrenches; otherwise the
+// thread-per-environment kernel runs.
+bool use_chain_vjp(const sapamd_model *m, bool poses) {
+  if (m->effective_variant() != SAPAMD_KERNEL_LANES_PER_JOINT) return false;
+  if (!poses) return true;
+  const auto &lf = m->host->link_frames();
+  std::vector<int> cnt(m->host->njoints(), 0);
+  int root = 0;
+  for (int f : lf) {
+    int j = m->host->frames()[f].joint;
+    if (j < 0) ++root;
+    else if (++cnt[j] > 1) return false;
+  }
+  return root <= 1;
+}
+void run_vjp(const sapamd_model *m, int64_t B, const float *q, const float *qd, const float *qdd,
+             VjpArgs<float> a, bool poses, void *stream) {
+  if (use_chain_vjp(m, poses)) {
+    int G, C;
+    m->shape_for(B, &G, &C);
+    HIP_OK(launch_chain_vjp(m->t32(), m->host->njoints(), B, q, qd, qdd, a, S(stream), G, C));
+  } else {
+    HIP_OK(launch_kin_vjp<float>(m->t32(), m->host->njoints(), B, q, qd, qdd, a, S(stream)));
+  }
+}
+}  // namespace
+
+int sapamd_kin_vjp_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd, const float *qdd,
+                           const float *gposes, const float *gtau, float *dq, float *dqd, float *dqdd, void *stream) {
+  return guarded([&] {
+    check_batched(m, B);
+    if (B == 0) return;
+    need(q, "q");
+    need(dq, "dq");
+    if (!gposes && !gtau) throw std::invalid_argument("give gposes and/or gtau");
+    if (gtau) {
+      need(qd, "qd");
+      need(qdd, "qdd");
+      need(dqd, "dqd");
+      need(dqdd, "dqdd");
+    }
+    if (gposes && m->L() == 0) throw std::runtime_error("link index out of bound");
+    VjpArgs<float> a;
+    a.gposes = gposes;
+    a.gtau = gtau;
+    a.dq = dq;
+    a.dqd = gtau ? dqd : nullptr;
+    a.dqdd = gtau ? dqdd : nullptr;
+    run_vjp(m, B, q, qd, qdd, a, gposes != nullptr, stream);
+  });
+}
+
+int sapamd_pose_torque_loss_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd,
+                                    const float *qdd, const float *tau_star, int32_t ee_link, const float *p_target,
+                                    float *loss, float *tau, float *dq, float *dqd, float *dqdd, void *stream) {
+  return guarded([&] {
+    check_batched(m, B);
+    if (p_target) m->check_link(ee_link);
+    if (B == 0) return;
+    need(q, "q");
+    need(loss, "loss");
+    need(dq, "dq");
+    if (!tau_star && !p_target) throw std::invalid_argument("give tau_star and/or p_target");
+    if (tau_star) {
+      need(qd, "qd");
+      need(qdd, "qdd");
+      need(dqd, "dqd");
+      need(dqdd, "dqdd");
+    }
+    VjpArgs<float> a;
+    a.tau_star = tau_star;
+    a.p_target = p_target;
+    a.ee_link = ee_link;
+    a.loss = loss;
+    a.tau = tau;
+    a.dq = dq;
+    a.dqd = tau_star ? dqd : nullptr;
+    a.dqdd = tau_star ? dqdd : nullptr;
+    run_vjp(m, B, q, qd, qdd, a, p_target != nullptr, stream);
+  });
+}
+
 }  // extern "C"

@@ -1,0 +1,157 @@
+// Cross-lane building blocks for the lanes-per-joint kernels: DPP row shifts,
+// ds_bpermute, SE(3) / spatial-vector scans over a group of G lanes inside a
+// 16-lane row, and per-joint helpers shared by the forward and adjoint kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "spatial.h"
+
+namespace sapamd {
+namespace lanes {
+
+template <int CTRL> __device__ __forceinline__ float dppf(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xF, 0xF, true));
+}
+// value held by lane g - D of the same 16-lane row (0 for lanes below D)
+template <int D> __device__ __forceinline__ float from_below(float x) { return dppf<0x110 + D>(x); }
+// value held by lane g + D of the same 16-lane row (0 past the row end)
+template <int D> __device__ __forceinline__ float from_above(float x) { return dppf<0x100 + D>(x); }
+// value held by wave lane src4 / 4
+__device__ __forceinline__ float bperm(int src4, float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src4, __builtin_bit_cast(int, x)));
+}
+
+template <int D> __device__ __forceinline__ S6<float> below6(S6<float> v) {
+  return {{from_below<D>(v.l.x), from_below<D>(v.l.y), from_below<D>(v.l.z)},
+          {from_below<D>(v.a.x), from_below<D>(v.a.y), from_below<D>(v.a.z)}};
+}
+template <int D> __device__ __forceinline__ S6<float> above6(S6<float> v) {
+  return {{from_above<D>(v.l.x), from_above<D>(v.l.y), from_above<D>(v.l.z)},
+          {from_above<D>(v.a.x), from_above<D>(v.a.y), from_above<D>(v.a.z)}};
+}
+__device__ __forceinline__ S6<float> sel(bool t, S6<float> a, S6<float> b) {
+  return {{t ? a.l.x : b.l.x, t ? a.l.y : b.l.y, t ? a.l.z : b.l.z},
+          {t ? a.a.x : b.a.x, t ? a.a.y : b.a.y, t ? a.a.z : b.a.z}};
+}
+
+struct Xf {
+  float R[9];
+  V3<float> p;
+};
+__device__ __forceinline__ Xf compose(const Xf &a, const Xf &b) {
+  Xf o;
+  mm(a.R, b.R, o.R);
+  o.p = mv(a.R, b.p) + a.p;
+  return o;
+}
+__device__ __forceinline__ Xf identity_xf() {
+  Xf o;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) o.R[k] = (k % 4 == 0) ? 1.f : 0.f;
+  o.p = {0.f, 0.f, 0.f};
+  return o;
+}
+template <int D> __device__ __forceinline__ Xf below_xf(const Xf &x) {
+  Xf o;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) o.R[k] = from_below<D>(x.R[k]);
+  o.p = {from_below<D>(x.p.x), from_below<D>(x.p.y), from_below<D>(x.p.z)};
+  return o;
+}
+__device__ __forceinline__ Xf sel_xf(bool t, const Xf &a, const Xf &b) {
+  Xf o;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) o.R[k] = t ? a.R[k] : b.R[k];
+  o.p = {t ? a.p.x : b.p.x, t ? a.p.y : b.p.y, t ? a.p.z : b.p.z};
+  return o;
+}
+
+// Inclusive prefix product over the group: x_g <- x_0 ... x_g.
+template <int G, int D = 1> __device__ __forceinline__ void scan_xf(int g, Xf &x) {
+  if constexpr (D < G) {
+    Xf y = below_xf<D>(x);
+    x = sel_xf(g >= D, compose(y, x), x);
+    scan_xf<G, 2 * D>(g, x);
+  }
+}
+template <int G, int D = 1> __device__ __forceinline__ void scan_sum(int g, S6<float> &x) {
+  if constexpr (D < G) {
+    x = sel(g >= D, x + below6<D>(x), x);
+    scan_sum<G, 2 * D>(g, x);
+  }
+}
+template <int G, int D = 1> __device__ __forceinline__ void scan_suffix(int g, S6<float> &x) {
+  if constexpr (D < G) {
+    x = sel(g + D < G, x + above6<D>(x), x);
+    scan_suffix<G, 2 * D>(g, x);
+  }
+}
+// exclusive values from the inclusive scan
+template <int G> __device__ __forceinline__ Xf excl_xf(int g, const Xf &incl) {
+  if constexpr (G == 1) return identity_xf();
+  else return sel_xf(g >= 1, below_xf<1>(incl), identity_xf());
+}
+template <int G> __device__ __forceinline__ S6<float> excl_sum(int g, S6<float> incl) {
+  if constexpr (G == 1) return zero6<float>();
+  else return sel(g >= 1, below6<1>(incl), zero6<float>());
+}
+template <int G> __device__ __forceinline__ S6<float> excl_suffix(int g, S6<float> incl) {
+  if constexpr (G == 1) return zero6<float>();
+  else return sel(g + 1 < G, above6<1>(incl), zero6<float>());
+}
+
+// Shoemake's quaternion with Eigen's branch rule (w first when the trace is
+// positive, else the largest diagonal entry), evaluated without divergence.
+__device__ __forceinline__ void quat_branchless(const float *R, float *q) {
+  float tr = R[0] + R[4] + R[8];
+  int i = R[4] > R[0] ? 1 : 0;
+  float di = i ? R[4] : R[0];
+  int k = tr > 0.f ? 0 : 1 + (R[8] > di ? 2 : i);
+  float t = k == 0 ? tr : k == 1 ? R[0] - R[4] - R[8] : k == 2 ? R[4] - R[8] - R[0] : R[8] - R[0] - R[4];
+  float r = sqrtf(t + 1.f);
+  float h = 0.5f / r;
+  float a = (R[7] - R[5]) * h, b = (R[2] - R[6]) * h, c = (R[3] - R[1]) * h;  // w-type
+  float d = (R[1] + R[3]) * h, e = (R[2] + R[6]) * h, f = (R[5] + R[7]) * h;  // off-diagonal sums
+  float big = 0.5f * r;
+  q[0] = k == 0 ? big : k == 1 ? a : k == 2 ? b : c;
+  q[1] = k == 0 ? a : k == 1 ? big : k == 2 ? d : e;
+  q[2] = k == 0 ? b : k == 1 ? d : k == 2 ? big : f;
+  q[3] = k == 0 ? c : k == 1 ? e : k == 2 ? f : big;
+}
+
+__device__ __forceinline__ void store_pose(float *P, const float *R, V3<float> p) {
+  float qq[4];
+  quat_branchless(R, qq);
+  P[0] = p.x; P[1] = p.y; P[2] = p.z;
+  P[3] = qq[0]; P[4] = qq[1]; P[5] = qq[2]; P[6] = qq[3];
+}
+
+// World inertia of a body whose rotational inertia is diagonal in its frame:
+// R diag(d) R^T = sum_k d_k r_k r_k^T (columns r_k of R).
+__device__ __forceinline__ WInertia<float> world_inertia_fast(const JointRec<float> &J, const float *R, V3<float> p) {
+  if (!J.diagI) return world_inertia(J, R, p);
+  WInertia<float> Y;
+  Y.m = J.m;
+  Y.c = mv(R, load3(J.c)) + p;
+  const float d0 = J.Ic[0], d1 = J.Ic[3], d2 = J.Ic[5];
+  Y.I[0] = d0 * R[0] * R[0] + d1 * R[1] * R[1] + d2 * R[2] * R[2];
+  Y.I[1] = d0 * R[0] * R[3] + d1 * R[1] * R[4] + d2 * R[2] * R[5];
+  Y.I[2] = d0 * R[0] * R[6] + d1 * R[1] * R[7] + d2 * R[2] * R[8];
+  Y.I[3] = d0 * R[3] * R[3] + d1 * R[4] * R[4] + d2 * R[5] * R[5];
+  Y.I[4] = d0 * R[3] * R[6] + d1 * R[4] * R[7] + d2 * R[5] * R[8];
+  Y.I[5] = d0 * R[6] * R[6] + d1 * R[7] * R[7] + d2 * R[8] * R[8];
+  return Y;
+}
+
+
+// Scalar sums over the group (the total lands in lane 0 of the group).
+template <int G, int D = 1> __device__ __forceinline__ void scan_suffix1(int g, float &x) {
+  if constexpr (D < G) {
+    const float y = from_above<D>(x);
+    x = g + D < G ? x + y : x;
+    scan_suffix1<G, 2 * D>(g, x);
+  }
+}
+
+}  // namespace lanes
+}  // namespace sapamd

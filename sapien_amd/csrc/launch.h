@@ -63,6 +63,29 @@ void default_chain_shape(int nj, int *G, int *C);
 hipError_t launch_chain_fused(const ModelTable<float> *tab, int nj, int64_t B, int flags, const float *q,
                               const float *qd, const float *qdd, KinOut<float> out, hipStream_t s, int G, int C);
 
+// Reverse mode (vector-Jacobian products) of FK link poses and RNEA.
+// VJP mode: upstream gradients gtau (B, nv) and/or gposes (B, L, 7).
+// Loss mode (tau_star / p_target given): the kernel forms them itself for
+//   loss = sum_i (tau_i - tau*_i)^2 + |p_ee - t|^2,
+// writes the per-environment loss and returns its gradients.
+// Gradient outputs: dq always; dqd, dqdd when the RNEA term is present.
+template <class T> struct VjpArgs {
+  const T *gtau = nullptr;      // (B, nv)
+  const T *gposes = nullptr;    // (B, L, 7)
+  const T *tau_star = nullptr;  // (B, nv), loss mode
+  const T *p_target = nullptr;  // (B, 3), loss mode
+  int ee_link = -1;             // user link of p_target
+  T *loss = nullptr;            // (B), loss mode
+  T *tau = nullptr;             // (B, nv), optional forward output
+  T *dq = nullptr, *dqd = nullptr, *dqdd = nullptr;
+};
+template <class T>
+hipError_t launch_kin_vjp(const ModelTable<T> *tab, int nj, int64_t B, const T *q, const T *qd, const T *qdd,
+                          VjpArgs<T> a, hipStream_t s);
+// Lanes-per-joint version of the same for serial chains (FP32).
+hipError_t launch_chain_vjp(const ModelTable<float> *tab, int nj, int64_t B, const float *q, const float *qd,
+                            const float *qdd, VjpArgs<float> a, hipStream_t s, int G, int C);
+
 // dst[0:n] = value (a kernel, not hipMemsetAsync, so it is safe under graph capture)
 hipError_t launch_fill_i32(int32_t *dst, int32_t value, int64_t n, hipStream_t s);
 

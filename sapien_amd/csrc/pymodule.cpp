@@ -352,6 +352,76 @@ class PyModel {
     return py::make_tuple(Q, S.attr("bool")(), E, I);
   }
 
+  // ---------------------------------------------------------------- adjoints (DLPack, FP32)
+  py::tuple kin_vjp_batched(py::object q, py::object qd, py::object qdd, py::object gposes, py::object gtau,
+                            py::object stream) {
+    Batch b(*this, q);
+    const bool rnea = !gtau.is_none();
+    py::object qdT, qddT, gT, gP;
+    std::unique_ptr<DLIn> vqd, vqdd, vgt, vgp;
+    if (rnea) {
+      qdT = b.input(qd);
+      qddT = b.input(qdd);
+      gT = b.input(gtau);
+      vqd = b.view(qdT);
+      vqdd = b.view(qddT);
+      vgt = b.view(gT);
+    }
+    if (!gposes.is_none()) {
+      int L = info().nlinks;
+      gP = b.as_shape(gposes, {b.B, L, 7});
+      vgp = b.view(gP);
+    }
+    py::object dq = b.empty({b.B, b.n}), dqd = py::none(), dqdd = py::none();
+    std::unique_ptr<DLIn> vdqd, vdqdd;
+    if (rnea) {
+      dqd = b.empty({b.B, b.n});
+      dqdd = b.empty({b.B, b.n});
+      vdqd = b.view(dqd);
+      vdqdd = b.view(dqdd);
+    }
+    auto vq = b.view(b.q), vdq = b.view(dq);
+    check(sapamd_kin_vjp_batched(m_, b.B, f(vq), rnea ? f(vqd) : nullptr, rnea ? f(vqdd) : nullptr,
+                                 vgp ? f(vgp) : nullptr, rnea ? f(vgt) : nullptr, f(vdq), rnea ? f(vdqd) : nullptr,
+                                 rnea ? f(vdqdd) : nullptr, b.stream(stream)));
+    return py::make_tuple(dq, dqd, dqdd);
+  }
+  py::tuple pose_torque_loss_batched(py::object q, py::object qd, py::object qdd, py::object tau_star,
+                                     int64_t ee_link, py::object p_target, py::object stream) {
+    Batch b(*this, q);
+    const bool rnea = !tau_star.is_none();
+    py::object qdT, qddT, tsT, ptT;
+    std::unique_ptr<DLIn> vqd, vqdd, vts, vpt;
+    if (rnea) {
+      qdT = b.input(qd);
+      qddT = b.input(qdd);
+      tsT = b.input(tau_star);
+      vqd = b.view(qdT);
+      vqdd = b.view(qddT);
+      vts = b.view(tsT);
+    }
+    if (!p_target.is_none()) {
+      ptT = b.input(p_target, 3);
+      vpt = b.view(ptT);
+    }
+    py::object loss = b.empty({b.B}), tau = py::none(), dq = b.empty({b.B, b.n}), dqd = py::none(), dqdd = py::none();
+    std::unique_ptr<DLIn> vtau, vdqd, vdqdd;
+    if (rnea) {
+      tau = b.empty({b.B, b.n});
+      dqd = b.empty({b.B, b.n});
+      dqdd = b.empty({b.B, b.n});
+      vtau = b.view(tau);
+      vdqd = b.view(dqd);
+      vdqdd = b.view(dqdd);
+    }
+    auto vq = b.view(b.q), vl = b.view(loss), vdq = b.view(dq);
+    check(sapamd_pose_torque_loss_batched(
+        m_, b.B, f(vq), rnea ? f(vqd) : nullptr, rnea ? f(vqdd) : nullptr, rnea ? f(vts) : nullptr,
+        (int32_t)clamp_index(ee_link), vpt ? f(vpt) : nullptr, f(vl), rnea ? f(vtau) : nullptr, f(vdq),
+        rnea ? f(vdqd) : nullptr, rnea ? f(vdqdd) : nullptr, b.stream(stream)));
+    return py::make_tuple(loss, tau, dq, dqd, dqdd);
+  }
+
   sapamd_model *raw() const { return m_; }
 
  private:
@@ -391,6 +461,13 @@ class PyModel {
       if (shp.size() != 2 || shp[1].cast<int64_t>() != ncols)
         throw std::runtime_error("batched input must have shape (B, " + std::to_string(ncols) + ")");
       if (B && shp[0].cast<int64_t>() != B) throw std::runtime_error("batched inputs disagree on B");
+      return x;
+    }
+    py::object as_shape(py::object x, std::vector<int64_t> shape) {
+      if (!py::isinstance(x, torch.attr("Tensor"))) x = torch.attr("as_tensor")(x);
+      x = x.attr("detach")().attr("to")(py::arg("device") = dev, py::arg("dtype") = torch.attr("float32"));
+      x = x.attr("contiguous")();
+      if (x.attr("shape").cast<std::vector<int64_t>>() != shape) throw std::runtime_error("unexpected gradient shape");
       return x;
     }
     py::object input_i32(py::object x, int64_t len) {
@@ -545,6 +622,16 @@ PYBIND11_MODULE(_pinocchio, m) {
            py::arg("stream") = py::none())
       .def("compute_forward_dynamics_batched", &PyModel::aba_batched, py::arg("qpos"), py::arg("qvel"), py::arg("qf"),
            py::arg("stream") = py::none())
+      .def("kinematics_vjp_batched", &PyModel::kin_vjp_batched, py::arg("qpos"), py::arg("qvel") = py::none(),
+           py::arg("qacc") = py::none(), py::arg("grad_poses") = py::none(), py::arg("grad_tau") = py::none(),
+           py::arg("stream") = py::none(),
+           "Reverse mode of link poses and RNEA: (dq, dqd, dqdd) for upstream gradients on poses (B, L, 7) "
+           "and/or torques (B, nv).")
+      .def("pose_torque_loss_batched", &PyModel::pose_torque_loss_batched, py::arg("qpos"),
+           py::arg("qvel") = py::none(), py::arg("qacc") = py::none(), py::arg("tau_target") = py::none(),
+           py::arg("ee_link") = 0, py::arg("p_target") = py::none(), py::arg("stream") = py::none(),
+           "loss = |tau(q, qd, qdd) - tau*|^2 + |p_ee(q) - t|^2 per environment with its gradients, one pass: "
+           "(loss (B,), tau (B, nv), dq, dqd, dqdd).")
       .def("compute_inverse_kinematics_batched", &PyModel::clik_batched, py::arg("link_index"), py::arg("poses"),
            py::arg("initial_qpos") = py::none(), py::arg("active_qmask") = py::none(), py::arg("eps") = 1e-4,
            py::arg("max_iterations") = 1000, py::arg("dt") = 0.1, py::arg("damp") = 1e-6,

@@ -52,6 +52,27 @@ def handle(model):
     return ctypes.c_void_p(_pinocchio._abi_handle(model))
 
 
+def launcher_pose_torque_loss(model):
+    """f(B, q, qd, qdd, tau_star, ee_link, p_target, loss, tau, dq, dqd, dqdd, stream) over tensors."""
+    L = lib()
+    h = handle(model)
+    fn = L.sapamd_pose_torque_loss_batched
+    P, I64, I32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
+    fn.argtypes = [P, I64, P, P, P, P, I32, P, P, P, P, P, P, P]
+    fn.restype = ctypes.c_int
+
+    def ptr(x):
+        return None if x is None else (x if isinstance(x, int) else x.data_ptr())
+
+    def launch(B, q, qd, qdd, tau_star, ee_link, p_target, loss, tau, dq, dqd, dqdd, stream):
+        rc = fn(h, B, ptr(q), ptr(qd), ptr(qdd), ptr(tau_star), ee_link, ptr(p_target), ptr(loss), ptr(tau), ptr(dq),
+                ptr(dqd), ptr(dqdd), stream)
+        if rc:
+            check(rc)
+
+    return launch
+
+
 def launcher_fk_jac_rnea(model):
     """f(B, q, qd, qdd, P, J, T, stream) over torch tensors (or raw ints)."""
     L = lib()
