@@ -155,6 +155,15 @@ int sapamd_clik_batched(const sapamd_model *m, int64_t B, int32_t link, const fl
                         const int32_t *mask, double eps, int32_t max_iterations, double dt, double damp,
                         float *q_out, int32_t *success, float *err, int32_t *iterations, void *stream);
 
+/* Heterogeneous batch in ONE launch: models[i] (trees of 1 to 32 joints, at
+ * most 8 models) each with B[i] environments and its own device buffers; per
+ * model any of poses/J/tau/M may be NULL (arrays of pointers may be NULL to
+ * skip that output everywhere). Computes FK link poses, WORLD Jacobian, RNEA
+ * and CRBA (no reference counterpart: the reference evaluates one model per call). */
+int sapamd_hetero_batched(const sapamd_model *const *models, int32_t nmodels, const int64_t *B,
+                          const float *const *q, const float *const *qd, const float *const *qdd,
+                          float *const *poses, float *const *J, float *const *tau, float *const *M, void *stream);
+
 /* ---------------------------------------------------------------- adjoints (FP32, batched)
  * No reference counterpart: the reference has no derivatives. Reverse mode of
  * the batched kinematics above, for training losses through the model. */

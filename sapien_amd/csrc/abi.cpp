@@ -170,6 +170,17 @@ struct sapamd_model {
         for (int k = 0; k < 3; ++k) jr[f.joint].lp[k] = T(f.place.p[k]);
       }
     }
+    // pointer-jumping ancestors and depth-first subtree sizes
+    for (int j = 0; j < nj; ++j) {
+      int a = J[j].parent;
+      for (int k = 0; k < 5; ++k) {
+        jr[j].anc[k] = a;
+        for (int s = 0; s < (1 << k) && a >= 0; ++s) a = J[a].parent;  // advance 2^k more steps
+      }
+      jr[j].subtree = 1;
+    }
+    for (int j = nj - 1; j >= 0; --j)
+      if (J[j].parent >= 0) jr[J[j].parent].subtree += jr[j].subtree;
     hdr->links_le1 = on_root <= 1;
     for (int j = 0; j < nj; ++j)
       if (per_joint[j] > 1) hdr->links_le1 = 0;
@@ -292,11 +303,13 @@ struct sapamd_model {
     HIP_OK(hipStreamSynchronize(stream));
     std::memcpy(dst, h_buf + off, n * sizeof(double));
   }
+  // serial chain of <= 16 joints: the G x C chain kernels apply
+  bool chain_kernel() const { return host->njoints() >= 1 && host->is_chain() && host->njoints() <= 16; }
   int effective_variant() const {
-    bool fast_ok = host->njoints() >= 1 && host->is_chain() && fast_chain_group(host->njoints()) > 0;
+    bool fast_ok = chain_kernel() || tree_group(host->njoints()) > 0;
     if (variant == SAPAMD_KERNEL_THREAD_PER_ENV) return SAPAMD_KERNEL_THREAD_PER_ENV;
     if (variant == SAPAMD_KERNEL_LANES_PER_JOINT) {
-      if (!fast_ok) throw std::runtime_error("lanes-per-joint kernels need a serial chain of at most 16 joints");
+      if (!fast_ok) throw std::runtime_error("lanes-per-joint kernels need a tree of at most 32 joints");
       return SAPAMD_KERNEL_LANES_PER_JOINT;
     }
     return fast_ok ? SAPAMD_KERNEL_LANES_PER_JOINT : SAPAMD_KERNEL_THREAD_PER_ENV;
@@ -666,12 +679,17 @@ int sapamd_fk_jac_rnea_batched(const sapamd_model *m, int64_t B, const float *q,
     if (J) { flags |= KF_JFULL; o.jfull = J; }
     if (tau) { need(qd, "qd"); need(qdd, "qdd"); flags |= KF_RNEA; o.tau = tau; }
     if (m->effective_variant() == SAPAMD_KERNEL_LANES_PER_JOINT) {
-      int G, C;
-      m->shape_for(B, &G, &C);
-      HIP_OK(launch_chain_fused(m->t32(), m->host->njoints(), B, flags, q, qd, qdd, o, S(stream), G, C));
-    }
-    else
+      if (m->chain_kernel()) {
+        int G, C;
+        m->shape_for(B, &G, &C);
+        HIP_OK(launch_chain_fused(m->t32(), m->host->njoints(), B, flags, q, qd, qdd, o, S(stream), G, C));
+      } else {
+        HIP_OK(launch_tree_fused(m->t32(), m->host->njoints(), B, flags, q, qd, qdd, poses, J, tau, nullptr,
+                                 S(stream)));
+      }
+    } else {
       HIP_OK(launch_kinematics<float>(m->t32(), m->host->njoints(), B, flags, q, qd, qdd, o, S(stream)));
+    }
   });
 }
 
@@ -711,7 +729,51 @@ int sapamd_crba_batched(const sapamd_model *m, int64_t B, const float *q, float 
     if (B == 0) return;
     need(q, "q");
     need(M, "M");
-    HIP_OK(launch_crba<float>(m->t32(), m->host->njoints(), B, q, M, S(stream)));
+    if (m->effective_variant() == SAPAMD_KERNEL_LANES_PER_JOINT && tree_group(m->host->njoints()))
+      HIP_OK(launch_tree_fused(m->t32(), m->host->njoints(), B, KF_CRBA, q, nullptr, nullptr, nullptr, nullptr,
+                               nullptr, M, S(stream)));
+    else
+      HIP_OK(launch_crba<float>(m->t32(), m->host->njoints(), B, q, M, S(stream)));
+  });
+}
+
+int sapamd_hetero_batched(const sapamd_model *const *models, int32_t nmodels, const int64_t *B,
+                          const float *const *q, const float *const *qd, const float *const *qdd,
+                          float *const *poses, float *const *J, float *const *tau, float *const *M, void *stream) {
+  return guarded([&] {
+    if (nmodels < 1 || nmodels > kMaxSegments) throw std::invalid_argument("1 to 8 models per heterogeneous launch");
+    HeteroArgs args{};
+    args.nseg = 0;
+    int64_t blocks = 0;
+    int flags = KF_POSES | KF_JFULL | KF_RNEA | KF_CRBA;
+    for (int i = 0; i < nmodels; ++i) {
+      const sapamd_model *m = models[i];
+      check_batched(m, B[i]);
+      const int nj = m->host->njoints(), G = tree_group(nj);
+      if (!G) throw std::runtime_error("heterogeneous launches need trees of 1 to 32 joints");
+      if (B[i] == 0) continue;
+      need(q[i], "q");
+      if (tau && tau[i]) {
+        need(qd ? qd[i] : nullptr, "qd");
+        need(qdd ? qdd[i] : nullptr, "qdd");
+      }
+      if (poses && poses[i] && m->L() == 0) throw std::runtime_error("link index out of bound");
+      HeteroSegment &s = args.seg[args.nseg++];
+      s.tab = m->t32();
+      s.G = G;
+      s.nj = nj;
+      s.B = B[i];
+      s.block0 = blocks;
+      s.q = q[i];
+      s.qd = qd ? qd[i] : nullptr;
+      s.qdd = qdd ? qdd[i] : nullptr;
+      s.poses = poses ? poses[i] : nullptr;
+      s.J = J ? J[i] : nullptr;
+      s.tau = tau ? tau[i] : nullptr;
+      s.M = M ? M[i] : nullptr;
+      blocks += (B[i] * G + 255) / 256;
+    }
+    HIP_OK(launch_hetero(args, blocks, flags, S(stream)));
   });
 }
 
@@ -780,9 +842,7 @@ int sapamd_clik_batched(const sapamd_model *m, int64_t B, int32_t link, const fl
 namespace {
 // Chain kernels need one user link per joint for pose wrenches; otherwise the
 // thread-per-environment kernel runs.
-bool use_chain_vjp(const sapamd_model *m, bool poses) {
-  if (m->effective_variant() != SAPAMD_KERNEL_LANES_PER_JOINT) return false;
-  if (!poses) return true;
+bool links_one_per_joint(const sapamd_model *m) {
   const auto &lf = m->host->link_frames();
   std::vector<int> cnt(m->host->njoints(), 0);
   int root = 0;
@@ -795,10 +855,13 @@ bool use_chain_vjp(const sapamd_model *m, bool poses) {
 }
 void run_vjp(const sapamd_model *m, int64_t B, const float *q, const float *qd, const float *qdd,
              VjpArgs<float> a, bool poses, void *stream) {
-  if (use_chain_vjp(m, poses)) {
+  const bool lanes = m->effective_variant() == SAPAMD_KERNEL_LANES_PER_JOINT && (!poses || links_one_per_joint(m));
+  if (lanes && m->chain_kernel()) {
     int G, C;
     m->shape_for(B, &G, &C);
     HIP_OK(launch_chain_vjp(m->t32(), m->host->njoints(), B, q, qd, qdd, a, S(stream), G, C));
+  } else if (lanes && tree_group(m->host->njoints())) {
+    HIP_OK(launch_tree_vjp(m->t32(), m->host->njoints(), B, q, qd, qdd, a, S(stream)));
   } else {
     HIP_OK(launch_kin_vjp<float>(m->t32(), m->host->njoints(), B, q, qd, qdd, a, S(stream)));
   }

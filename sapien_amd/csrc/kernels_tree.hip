@@ -1,0 +1,434 @@
+// Lanes-per-joint kernels for kinematic trees (FP32), and the heterogeneous
+// launch that runs several trees in one grid.
+//
+// An environment occupies G lanes (G = 8, 16 or 32 >= nj); lane j owns joint j
+// of the depth-first (preorder) joint order. Two tree scans replace the serial
+// sweeps:
+//  * root-to-joint prefix (FK products, velocities, accelerations): pointer
+//    jumping over the precomputed 2^k-th ancestors, x_j <- x_anc_k(j) (+) x_j
+//    for k = 0 .. log2(depth), exchanging registers with ds_bpermute;
+//  * subtree sums (forces, composite inertias): in preorder the subtree of j is
+//    the lane range [j, j + size_j). Doubling builds block sums
+//    B_k[j] = sum of 2^k lanes from j, and F_j adds the blocks of the binary
+//    decomposition of size_j. Exact, without prefix differences (no cancellation).
+// The CRBA lane j writes row j of M: for i an ancestor-or-self of j the entry is
+// col_i . (Yc_j col_j), for i a descendant it is col_j . (Yc_i col_i), else 0.
+#include <hip/hip_runtime.h>
+
+#include "lanes.h"
+#include "launch.h"
+#include "spatial.h"
+
+namespace sapamd {
+
+namespace {
+
+using namespace lanes;
+
+constexpr int kMaxTreeJoints = 32;
+constexpr int kTreeTableWords4 = (int)((kHeaderBytes + sizeof(JointRec<float>) * kMaxTreeJoints) / 16);
+
+__device__ __forceinline__ S6<float> bperm6(int src4, S6<float> x) {
+  return {{bperm(src4, x.l.x), bperm(src4, x.l.y), bperm(src4, x.l.z)},
+          {bperm(src4, x.a.x), bperm(src4, x.a.y), bperm(src4, x.a.z)}};
+}
+__device__ __forceinline__ Xf bperm_xf(int src4, const Xf &x) {
+  Xf o;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) o.R[k] = bperm(src4, x.R[k]);
+  o.p = {bperm(src4, x.p.x), bperm(src4, x.p.y), bperm(src4, x.p.z)};
+  return o;
+}
+
+// log2(G) rounds of pointer jumping toward the root.
+template <int G> struct Tree {
+  static constexpr int K = G == 8 ? 3 : G == 16 ? 4 : 5;
+  int base4;
+  int anc[K];
+  int size;
+  int g;
+  __device__ __forceinline__ void prefix(Xf &x) const {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int a = anc[k];
+      const Xf y = bperm_xf(base4 + 4 * (a < 0 ? 0 : a), x);
+      x = a >= 0 ? compose(y, x) : x;
+    }
+  }
+  __device__ __forceinline__ void prefix(S6<float> &x) const {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int a = anc[k];
+      const S6<float> y = bperm6(base4 + 4 * (a < 0 ? 0 : a), x);
+      x = sel(a >= 0, x + y, x);
+    }
+  }
+  // subtree sum of x over [g, g + size) (x = 0 on lanes without a joint)
+  template <int N> __device__ __forceinline__ void subtree(float (&x)[N]) const {
+    float blk[K][N];
+    float cur[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) cur[i] = x[i];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) blk[k][i] = cur[i];
+      const int src = g + (1 << k);
+      const int s4 = base4 + 4 * (src < G ? src : 0);
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const float y = bperm(s4, cur[i]);
+        cur[i] = src < G ? cur[i] + y : cur[i];
+      }
+    }
+    // blocks of size 2^K cover the whole group: size <= G
+    float acc[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) acc[i] = 0.f;
+    // every lane runs the query (ds_bpermute must not sit in divergent code);
+    // size == G (a single-root tree filling the group) has no bits below K and
+    // takes the full block instead
+    int pos = g;
+#pragma unroll
+    for (int k = K - 1; k >= 0; --k) {
+      const bool take = (size >> k) & 1;
+      const int s4 = base4 + 4 * (pos < G ? pos : 0);
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const float y = bperm(s4, blk[k][i]);
+        acc[i] = take ? acc[i] + y : acc[i];
+      }
+      pos += take ? (1 << k) : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = size >= G ? cur[i] : acc[i];
+  }
+  __device__ __forceinline__ void subtree(S6<float> &f) const {
+    float x[6] = {f.l.x, f.l.y, f.l.z, f.a.x, f.a.y, f.a.z};
+    subtree<6>(x);
+    f = {{x[0], x[1], x[2]}, {x[3], x[4], x[5]}};
+  }
+};
+
+struct TreeIO {
+  const float *q, *qd, *qdd;
+  float *poses, *J, *tau, *M;
+};
+
+// One environment per G-lane group; M is the LDS copy of the model table.
+template <int G>
+__device__ __forceinline__ void tree_body(const ModelTable<float> *M, const ModelTable<float> *Mg, int64_t B,
+                                          int64_t env, int flags, TreeIO io) {
+  const int g = threadIdx.x & (G - 1);
+  const bool live = env < B;
+  const int64_t e = live ? env : B - 1;
+  const int nj = M->nj, nv = M->nv, L = M->nlinks;
+  const bool dyn = flags & KF_RNEA;
+  const bool valid = g < nj;
+  const JointRec<float> &J = joints_of(M)[valid ? g : nj - 1];
+  Tree<G> T;
+  T.g = g;
+  T.base4 = ((threadIdx.x & 63) & ~(G - 1)) * 4;
+#pragma unroll
+  for (int k = 0; k < Tree<G>::K; ++k) T.anc[k] = valid ? J.anc[k] : -1;
+  T.size = valid ? J.subtree : 0;
+
+  // dofs: lane g loads user dof g, then the joint -> dof permutation via bpermute
+  const int64_t qi = e * nv + (g < nv ? g : nv - 1);
+  float qu = io.q[qi], qdu = io.qd[qi], qddu = io.qdd[qi];
+  const int src = T.base4 + 4 * J.vuser;
+  float qj = bperm(src, qu), qdj = bperm(src, qdu), qddj = bperm(src, qddu);
+  if (!valid) qj = qdj = qddj = 0.f;
+  if (!dyn) qdj = qddj = 0.f;
+
+  float s, c;
+  sincosf(qj, &s, &c);
+  Xf X;
+  joint_world(J, qj, s, c, static_cast<const float *>(nullptr), V3<float>{0, 0, 0}, true, X.R, X.p);
+  if (!valid) X = identity_xf();
+  T.prefix(X);
+  const S6<float> col = valid ? jcol(J, X.R, X.p) : zero6<float>();
+  const int u = J.vuser;
+
+  if (dyn) {
+    S6<float> v = qdj * col;
+    T.prefix(v);
+    S6<float> a = qddj * col + qdj * crossm(v, col);
+    T.prefix(a);
+    a.l = a.l - V3<float>{M->gravity[0], M->gravity[1], M->gravity[2]};
+    const WInertia<float> Y = world_inertia_fast(J, X.R, X.p);
+    S6<float> f = valid ? inertia_apply(Y, a) + crossf(v, inertia_apply(Y, v)) : zero6<float>();
+    T.subtree(f);
+    if (live && valid) io.tau[e * nv + u] = dot6(col, f);
+  }
+  if ((flags & KF_JFULL) && live && valid) {
+    float *Jb = io.J + e * 6 * nv + u;
+    Jb[0 * nv] = col.l.x; Jb[1 * nv] = col.l.y; Jb[2 * nv] = col.l.z;
+    Jb[3 * nv] = col.a.x; Jb[4 * nv] = col.a.y; Jb[5 * nv] = col.a.z;
+  }
+  if (flags & KF_CRBA) {
+    // composite inertia of the subtree (additive form about the world origin)
+    AInertia<float> Ya = additive(world_inertia_fast(J, X.R, X.p));
+    float y[10] = {Ya.m, Ya.h.x, Ya.h.y, Ya.h.z, Ya.I[0], Ya.I[1], Ya.I[2], Ya.I[3], Ya.I[4], Ya.I[5]};
+    if (!valid)
+#pragma unroll
+      for (int i = 0; i < 10; ++i) y[i] = 0.f;
+    T.template subtree<10>(y);
+    AInertia<float> Yc;
+    Yc.m = y[0];
+    Yc.h = {y[1], y[2], y[3]};
+#pragma unroll
+    for (int i = 0; i < 6; ++i) Yc.I[i] = y[4 + i];
+    const S6<float> Fj = apply(Yc, col);
+    for (int i = 0; i < nj; ++i) {  // uniform loop over the other joints
+      const int s4 = T.base4 + 4 * i;
+      const S6<float> ci = bperm6(s4, col), Fi = bperm6(s4, Fj);
+      const JointRec<float> &Ji = joints_of(M)[i];
+      const bool i_anc_j = i <= g && g < i + Ji.subtree;
+      const bool j_anc_i = g < i && i < g + T.size;
+      const float mij = i_anc_j ? dot6(ci, Fj) : j_anc_i ? dot6(col, Fi) : 0.f;
+      if (live && valid) io.M[(e * nv + u) * nv + Ji.vuser] = mij;
+    }
+  }
+  if (!(flags & KF_POSES)) return;
+  if (M->links_le1) {
+    if (!live) return;
+    if (valid && J.link >= 0) {
+      float Rw[9];
+      mm(X.R, J.lR, Rw);
+      store_pose(io.poses + (e * L + J.link) * 7, Rw, mv(X.R, load3(J.lp)) + X.p);
+    }
+    if (g == 0 && M->root_link >= 0) store_pose(io.poses + (e * L + M->root_link) * 7, M->root_R, load3(M->root_p));
+    return;
+  }
+  for (int l0 = 0; l0 < L; l0 += G) {
+    const int l = l0 + g;
+    const LinkRec<float> &LL = links_of(Mg)[l < L ? l : L - 1];
+    const int jl = LL.joint < 0 ? 0 : LL.joint;
+    const Xf W = bperm_xf(T.base4 + 4 * jl, X);
+    float Rw[9];
+    mm(W.R, LL.R, Rw);
+    V3<float> pw = mv(W.R, load3(LL.p)) + W.p;
+    if (LL.joint < 0) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Rw[k] = LL.R[k];
+      pw = load3(LL.p);
+    }
+    if (live && l < L) store_pose(io.poses + (e * L + l) * 7, Rw, pw);
+  }
+}
+
+// Sum over the G lanes of the group, result in lane 0.
+template <int G> __device__ __forceinline__ float group_sum(int g, int base4, float x) {
+#pragma unroll
+  for (int d = 1; d < G; d *= 2) {
+    const int src = g + d;
+    const float y = bperm(base4 + 4 * (src < G ? src : 0), x);
+    x = src < G ? x + y : x;
+  }
+  return x;
+}
+
+// Forward recomputation + reverse pass (formulas in kernels_adjoint.hip).
+// Pose gradients need one user link per joint (links_le1).
+template <int G>
+__device__ __forceinline__ void tree_vjp_body(const ModelTable<float> *M, int64_t B, int64_t env, const float *q,
+                                              const float *qd, const float *qdd, const float *gsrc,
+                                              const VjpArgs<float> &A) {
+  const int g = threadIdx.x & (G - 1);
+  const bool live = env < B;
+  const int64_t e = live ? env : B - 1;
+  const int nj = M->nj, nv = M->nv, L = M->nlinks;
+  const bool dyn = A.gtau || A.tau_star;
+  const bool valid = g < nj;
+  const JointRec<float> &J = joints_of(M)[valid ? g : nj - 1];
+  Tree<G> T;
+  T.g = g;
+  T.base4 = ((threadIdx.x & 63) & ~(G - 1)) * 4;
+#pragma unroll
+  for (int k = 0; k < Tree<G>::K; ++k) T.anc[k] = valid ? J.anc[k] : -1;
+  T.size = valid ? J.subtree : 0;
+
+  const int64_t qi = e * nv + (g < nv ? g : nv - 1);
+  const float qu = q[qi], qdu = qd[qi], qddu = qdd[qi], gu = gsrc[qi];
+  V3<float> ptarget = {0.f, 0.f, 0.f};
+  if (A.p_target) ptarget = {A.p_target[e * 3], A.p_target[e * 3 + 1], A.p_target[e * 3 + 2]};
+  const int src = T.base4 + 4 * J.vuser;
+  float qj = bperm(src, qu), qdj = bperm(src, qdu), qddj = bperm(src, qddu), gj = bperm(src, gu);
+  if (!valid) qj = qdj = qddj = gj = 0.f;
+  if (!dyn) qdj = qddj = gj = 0.f;
+
+  float s, c;
+  sincosf(qj, &s, &c);
+  Xf X;
+  joint_world(J, qj, s, c, static_cast<const float *>(nullptr), V3<float>{0, 0, 0}, true, X.R, X.p);
+  if (!valid) X = identity_xf();
+  T.prefix(X);
+  const S6<float> col = valid ? jcol(J, X.R, X.p) : zero6<float>();
+  const int u = J.vuser;
+
+  float loss = 0.f;
+  S6<float> FP = zero6<float>();
+  if (valid && J.link >= 0) {
+    const int l = J.link;
+    float Rw[9];
+    mm(X.R, J.lR, Rw);
+    const V3<float> pw = mv(X.R, load3(J.lp)) + X.p;
+    if (A.gposes) {
+      float quat[4];
+      quat_branchless(Rw, quat);
+      const float *gp7 = A.gposes + (e * L + l) * 7;
+      const V3<float> gp = {gp7[0], gp7[1], gp7[2]}, gq = {gp7[4], gp7[5], gp7[6]};
+      const V3<float> uq = {quat[1], quat[2], quat[3]};
+      FP = {gp, cross(pw, gp) + 0.5f * (quat[0] * gq - gp7[3] * uq + cross(uq, gq))};
+    } else if (A.p_target && l == A.ee_link) {
+      const V3<float> d = pw - ptarget;
+      loss += dot(d, d);
+      const V3<float> gp = 2.f * d;
+      FP = {gp, cross(pw, gp)};
+    }
+  }
+  S6<float> Hw = zero6<float>(), QS = zero6<float>(), v = zero6<float>(), a = zero6<float>(), F = zero6<float>(),
+            w = zero6<float>();
+  if (dyn) {
+    v = qdj * col;
+    T.prefix(v);
+    a = qddj * col + qdj * crossm(v, col);
+    T.prefix(a);
+    a.l = a.l - V3<float>{M->gravity[0], M->gravity[1], M->gravity[2]};
+    const WInertia<float> Y = world_inertia_fast(J, X.R, X.p);
+    F = valid ? inertia_apply(Y, a) + crossf(v, inertia_apply(Y, v)) : zero6<float>();
+    T.subtree(F);
+    const float tj = dot6(col, F);
+    if (A.tau && live && valid) A.tau[e * nv + u] = tj;
+    if (A.tau_star) {
+      const float d = valid ? tj - gj : 0.f;
+      loss += d * d;
+      gj = 2.f * d;
+    }
+    w = gj * col;
+    T.prefix(w);
+    const S6<float> Iw = inertia_apply(Y, w), Iv = inertia_apply(Y, v);
+    if (valid) {
+      Hw = Iw;
+      QS = crossf(w, Iv) - crossf(v, Iw) + inertia_apply(Y, crossm(v, w));
+    }
+  }
+  float acc[18] = {Hw.l.x, Hw.l.y, Hw.l.z, Hw.a.x, Hw.a.y, Hw.a.z, QS.l.x, QS.l.y, QS.l.z,
+                   QS.a.x, QS.a.y, QS.a.z, FP.l.x, FP.l.y, FP.l.z, FP.a.x, FP.a.y, FP.a.z};
+  T.template subtree<18>(acc);
+  Hw = {{acc[0], acc[1], acc[2]}, {acc[3], acc[4], acc[5]}};
+  QS = {{acc[6], acc[7], acc[8]}, {acc[9], acc[10], acc[11]}};
+  FP = {{acc[12], acc[13], acc[14]}, {acc[15], acc[16], acc[17]}};
+  if (live && valid) {
+    float dq = dot6(col, FP);
+    if (dyn) {
+      const S6<float> wl = w - gj * col, vl = v - qdj * col;
+      const S6<float> al = a - qddj * col - qdj * crossm(v, col);
+      dq += dot6(col, crossf(vl, QS + crossf(vl, Hw)) - crossf(wl, F) - crossf(al, Hw));
+      if (A.dqd) A.dqd[e * nv + u] = -dot6(col, QS + 2.f * crossf(vl, Hw));
+      if (A.dqdd) A.dqdd[e * nv + u] = dot6(col, Hw);
+    }
+    if (A.dq) A.dq[e * nv + u] = dq;
+  }
+  if ((A.tau_star || A.p_target) && A.loss) {
+    loss = group_sum<G>(g, T.base4, loss);
+    if (live && g == 0) A.loss[e] = loss;
+  }
+}
+
+__device__ __forceinline__ void stage_table(float4 *s_tab, const ModelTable<float> *Mg) {
+  const int n4 = (int)((kHeaderBytes + sizeof(JointRec<float>) * Mg->nj) / 16);
+  const float4 *src = reinterpret_cast<const float4 *>(Mg);
+  for (int i = threadIdx.x; i < n4; i += blockDim.x) s_tab[i] = src[i];
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void k_tree_fused(const ModelTable<float> *__restrict__ Mg, int64_t B, int flags,
+                                                    TreeIO io) {
+  __shared__ float4 s_tab[kTreeTableWords4];
+  stage_table(s_tab, Mg);
+  __syncthreads();
+  const int64_t env = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
+  tree_body<G>(reinterpret_cast<const ModelTable<float> *>(s_tab), Mg, B, env, flags, io);
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void k_tree_vjp(const ModelTable<float> *__restrict__ Mg, int64_t B,
+                                                  const float *__restrict__ q, const float *__restrict__ qd,
+                                                  const float *__restrict__ qdd, const float *__restrict__ gsrc,
+                                                  VjpArgs<float> A) {
+  __shared__ float4 s_tab[kTreeTableWords4];
+  stage_table(s_tab, Mg);
+  __syncthreads();
+  const int64_t env = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
+  tree_vjp_body<G>(reinterpret_cast<const ModelTable<float> *>(s_tab), B, env, q, qd, qdd, gsrc, A);
+}
+
+// Heterogeneous launch: block ranges map to segments (one tree type each).
+__global__ __launch_bounds__(256) void k_hetero(HeteroArgs args, int flags) {
+  __shared__ float4 s_tab[kTreeTableWords4];
+  int s = 0;
+  while (s + 1 < args.nseg && (int64_t)blockIdx.x >= args.seg[s + 1].block0) ++s;
+  const HeteroSegment &S = args.seg[s];
+  stage_table(s_tab, S.tab);
+  __syncthreads();
+  const int64_t local = (int64_t)(blockIdx.x - S.block0) * blockDim.x + threadIdx.x;
+  const ModelTable<float> *M = reinterpret_cast<const ModelTable<float> *>(s_tab);
+  TreeIO io{S.q, S.qd ? S.qd : S.q, S.qdd ? S.qdd : S.q, S.poses, S.J, S.tau, S.M};
+  int f = flags;
+  if (!S.poses) f &= ~KF_POSES;
+  if (!S.J) f &= ~KF_JFULL;
+  if (!S.tau) f &= ~KF_RNEA;
+  if (!S.M) f &= ~KF_CRBA;
+  switch (S.G) {  // uniform per block
+    case 8: tree_body<8>(M, S.tab, S.B, local / 8, f, io); break;
+    case 16: tree_body<16>(M, S.tab, S.B, local / 16, f, io); break;
+    default: tree_body<32>(M, S.tab, S.B, local / 32, f, io); break;
+  }
+}
+
+}  // namespace
+
+int tree_group(int nj) { return nj < 1 ? 0 : nj <= 8 ? 8 : nj <= 16 ? 16 : nj <= 32 ? 32 : 0; }
+
+hipError_t launch_tree_fused(const ModelTable<float> *tab, int nj, int64_t B, int flags, const float *q,
+                             const float *qd, const float *qdd, float *poses, float *J, float *tau, float *M,
+                             hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  const int G = tree_group(nj);
+  if (!G) return hipErrorNotSupported;
+  TreeIO io{q, (flags & KF_RNEA) && qd ? qd : q, (flags & KF_RNEA) && qdd ? qdd : q, poses, J, tau, M};
+  const int64_t per_block = 256 / G;
+  dim3 grid((unsigned)((B + per_block - 1) / per_block));
+  if (G == 8) hipLaunchKernelGGL(k_tree_fused<8>, grid, dim3(256), 0, s, tab, B, flags, io);
+  else if (G == 16) hipLaunchKernelGGL(k_tree_fused<16>, grid, dim3(256), 0, s, tab, B, flags, io);
+  else hipLaunchKernelGGL(k_tree_fused<32>, grid, dim3(256), 0, s, tab, B, flags, io);
+  return hipGetLastError();
+}
+
+hipError_t launch_tree_vjp(const ModelTable<float> *tab, int nj, int64_t B, const float *q, const float *qd,
+                           const float *qdd, VjpArgs<float> a, hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  const int G = tree_group(nj);
+  if (!G) return hipErrorNotSupported;
+  const bool dyn = a.gtau || a.tau_star;
+  if (!dyn) qd = qdd = q;
+  const float *gsrc = a.tau_star ? a.tau_star : a.gtau ? a.gtau : q;
+  const int64_t per_block = 256 / G;
+  dim3 grid((unsigned)((B + per_block - 1) / per_block));
+  if (G == 8) hipLaunchKernelGGL(k_tree_vjp<8>, grid, dim3(256), 0, s, tab, B, q, qd, qdd, gsrc, a);
+  else if (G == 16) hipLaunchKernelGGL(k_tree_vjp<16>, grid, dim3(256), 0, s, tab, B, q, qd, qdd, gsrc, a);
+  else hipLaunchKernelGGL(k_tree_vjp<32>, grid, dim3(256), 0, s, tab, B, q, qd, qdd, gsrc, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_hetero(const HeteroArgs &args, int64_t total_blocks, int flags, hipStream_t s) {
+  if (total_blocks <= 0) return hipSuccess;
+  for (int i = 0; i < args.nseg; ++i)
+    if (!tree_group(args.seg[i].nj) || args.seg[i].G != tree_group(args.seg[i].nj)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_hetero, dim3((unsigned)total_blocks), dim3(256), 0, s, args, flags);
+  return hipGetLastError();
+}
+
+}  // namespace sapamd

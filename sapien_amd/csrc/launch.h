@@ -11,7 +11,36 @@
 
 namespace sapamd {
 
-enum KinFlags : int { KF_POSES = 1, KF_JFULL = 2, KF_JLINKS = 4, KF_RNEA = 8 };
+enum KinFlags : int { KF_POSES = 1, KF_JFULL = 2, KF_JLINKS = 4, KF_RNEA = 8, KF_CRBA = 64 };
+
+// One tree type of a heterogeneous batch: its table and buffers (device
+// pointers, user order), its lanes per environment and its first block.
+struct HeteroSegment {
+  const ModelTable<float> *tab;
+  int32_t G, nj;
+  int64_t B, block0;
+  const float *q, *qd, *qdd;
+  float *poses, *J, *tau, *M;
+};
+// Lanes-per-joint kernels for trees of <= 32 joints (FP32): FK + Jacobian +
+// RNEA + CRBA in one pass (flags select outputs).
+int tree_group(int nj);
+hipError_t launch_tree_fused(const ModelTable<float> *tab, int nj, int64_t B, int flags, const float *q,
+                             const float *qd, const float *qdd, float *poses, float *J, float *tau, float *M,
+                             hipStream_t s);
+template <class T> struct VjpArgs;
+// Tree version of the fused forward + reverse pass (pose gradients need one
+// user link per joint).
+hipError_t launch_tree_vjp(const ModelTable<float> *tab, int nj, int64_t B, const float *q, const float *qd,
+                           const float *qdd, VjpArgs<float> a, hipStream_t s);
+// All segments in one launch (passed by value, up to kMaxSegments); the grid is
+// the sum of the segments' block counts (256-thread blocks).
+constexpr int kMaxSegments = 8;
+struct HeteroArgs {
+  HeteroSegment seg[kMaxSegments];
+  int32_t nseg;
+};
+hipError_t launch_hetero(const HeteroArgs &args, int64_t total_blocks, int flags, hipStream_t s);
 
 template <class T> struct KinOut {
   T *poses = nullptr, *jfull = nullptr, *jw_links = nullptr, *jl_links = nullptr, *tau = nullptr;

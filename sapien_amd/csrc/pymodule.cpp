@@ -424,6 +424,57 @@ class PyModel {
 
   sapamd_model *raw() const { return m_; }
 
+  // Heterogeneous batch: several models, one launch (FK + J + RNEA + CRBA).
+  static py::list hetero_batched(py::list models, py::list qs, py::object qds, py::object qdds, py::object stream) {
+    const size_t n = models.size();
+    if (qs.size() != n) throw py::value_error("one q batch per model");
+    std::vector<std::unique_ptr<Batch>> batches;
+    std::vector<py::object> keep;
+    std::vector<std::unique_ptr<DLIn>> views;
+    std::vector<const sapamd_model *> ms;
+    std::vector<int64_t> Bs;
+    std::vector<const float *> q, qd, qdd;
+    std::vector<float *> P, J, T, M;
+    py::list out;
+    auto fp = [&](py::object t) {
+      views.push_back(take(t, dlabi::kFloat, t.attr("shape").cast<std::vector<int64_t>>(),
+                           batches.back()->device));
+      return static_cast<float *>(views.back()->ptr);
+    };
+    void *s = nullptr;
+    for (size_t i = 0; i < n; ++i) {
+      PyModel &pm = models[i].cast<PyModel &>();
+      batches.push_back(std::make_unique<Batch>(pm, qs[i]));
+      Batch &b = *batches.back();
+      if (i == 0) s = b.stream(stream);
+      const int L = pm.info().nlinks;
+      py::object qdT = qds.is_none() ? py::object(b.torch.attr("zeros_like")(b.q)) : b.input(qds.cast<py::list>()[i]);
+      py::object qddT =
+          qdds.is_none() ? py::object(b.torch.attr("zeros_like")(b.q)) : b.input(qdds.cast<py::list>()[i]);
+      py::object Pt = b.empty({b.B, L, 7}), Jt = b.empty({b.B, 6, b.n}), Tt = b.empty({b.B, b.n}),
+                 Mt = b.empty({b.B, b.n, b.n});
+      keep.insert(keep.end(), {b.q, qdT, qddT});
+      ms.push_back(pm.raw());
+      Bs.push_back(b.B);
+      q.push_back(fp(b.q));
+      qd.push_back(fp(qdT));
+      qdd.push_back(fp(qddT));
+      P.push_back(fp(Pt));
+      J.push_back(fp(Jt));
+      T.push_back(fp(Tt));
+      M.push_back(fp(Mt));
+      py::dict d;
+      d["poses"] = Pt;
+      d["J"] = Jt;
+      d["tau"] = Tt;
+      d["M"] = Mt;
+      out.append(d);
+    }
+    check(sapamd_hetero_batched(ms.data(), (int32_t)n, Bs.data(), q.data(), qd.data(), qdd.data(), P.data(),
+                                J.data(), T.data(), M.data(), s));
+    return out;
+  }
+
  private:
   sapamd_model *m_ = nullptr;
   std::mt19937_64 rng_{std::random_device{}()};
@@ -501,6 +552,10 @@ PYBIND11_MODULE(_pinocchio, m) {
   m.def("version", [] { return std::string(sapamd_version()); });
   m.def("_abi_handle", [](PyModel &pm) { return reinterpret_cast<intptr_t>(pm.raw()); },
         "raw sapamd_model* of a model, for ctypes callers of the C ABI");
+  m.def("hetero_batched", &PyModel::hetero_batched, py::arg("models"), py::arg("qpos"), py::arg("qvel") = py::none(),
+        py::arg("qacc") = py::none(), py::arg("stream") = py::none(),
+        "Heterogeneous batch of several PinocchioModels (trees of <= 32 joints) in ONE kernel launch: per model "
+        "a dict of link poses (B, L, 7), WORLD Jacobian (B, 6, nv), RNEA torques (B, nv) and mass matrix (B, nv, nv).");
 
   py::class_<Pose>(m, "Pose")
       .def(py::init([](std::vector<float> p, std::vector<float> q) {
@@ -601,6 +656,8 @@ PYBIND11_MODULE(_pinocchio, m) {
       .def_property_readonly("nv", [](const PyModel &s) { return s.info().nv; })
       .def_property_readonly("nq", [](const PyModel &s) { return s.info().nq; })
       .def_property_readonly("nlinks", [](const PyModel &s) { return s.info().nlinks; })
+      .def_property_readonly("is_chain", [](const PyModel &s) { return bool(s.info().chain); })
+      .def_property_readonly("max_depth", [](const PyModel &s) { return s.info().max_depth; })
       // batched FP32 methods over (B, dof) tensors (DLPack, kDLROCM)
       .def("compute_forward_kinematics_batched", &PyModel::fk_batched, py::arg("qpos"), py::arg("stream") = py::none(),
            "Link poses (B, L, 7) = p xyz | q wxyz for a (B, nv) batch.")

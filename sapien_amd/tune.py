@@ -72,7 +72,7 @@ def autotune(model, batch_sizes=(16384,), iters=100, use_cache=True, report=None
     from . import kernel_variant
     from .native import launcher_fk_jac_rnea
 
-    if kernel_variant(model) != "lanes_per_joint":
+    if kernel_variant(model) != "lanes_per_joint" or not model.is_chain or model.nv > 16:
         return {}
     nj = model.nv
     dev = torch.device("cuda", torch.cuda.current_device())

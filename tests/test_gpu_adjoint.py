@@ -57,7 +57,7 @@ def test_rnea_vjp_matches_exact_derivatives(tree):
     q, qd, qdd, _ = S.samples(rob, 64, 1400)
     g = np.random.default_rng(1).normal(size=q.shape)
     Dq, Dqd, Dqdd = o.rnea_derivs(q, qd, qdd)
-    variants = ["auto", "thread_per_env"] if tree == "chain7" else ["auto"]
+    variants = ["auto", "thread_per_env"]
     for v in variants:
         sa.set_kernel_variant(m, v)
         dq, dqd, dqdd = m.kinematics_vjp_batched(_t(q), _t(qd), _t(qdd), grad_tau=_t(g))
@@ -75,7 +75,7 @@ def test_fk_pose_vjp_matches_central_differences(tree):
     q = S.samples(rob, 8, 1401)[0]
     gP = np.random.default_rng(2).normal(size=(8, rob.nlinks, 7))
     ref = _fk_vjp_fd(o, q, gP)
-    for v in (["auto", "thread_per_env"] if tree == "chain7" else ["auto"]):
+    for v in ("auto", "thread_per_env"):
         sa.set_kernel_variant(m, v)
         dq = m.kinematics_vjp_batched(_t(q), grad_poses=_t(gP))[0].cpu().numpy()
         assert rel_err(dq, ref) <= GRAD_TOL, v
@@ -108,7 +108,7 @@ def test_pose_torque_loss_gradients(tree):
             args_m = [q - e * (x == 0), qd - e * (x == 1), qdd - e * (x == 2)]
             d[:, k] = (loss64(*args_p) - loss64(*args_m)) / (2 * h)
         fd.append(d)
-    for v in (["auto", "thread_per_env"] if tree == "chain7" else ["auto"]):
+    for v in ("auto", "thread_per_env"):
         sa.set_kernel_variant(m, v)
         loss, tau, dq, dqd, dqdd = m.pose_torque_loss_batched(_t(q), _t(qd), _t(qdd), _t(tau_star), ee, _t(t))
         assert np.allclose(loss.cpu().numpy(), ref_loss, rtol=1e-4), v

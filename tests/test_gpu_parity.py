@@ -249,11 +249,14 @@ def test_lanes_per_joint_matches_oracle_and_generic(n):
     for a, b in zip(fast, gen):
         assert rel_err(a.cpu().numpy(), b.cpu().numpy().astype(np.float64)) <= REL_TOL
     sa.set_kernel_variant(m, "auto")
-    # trees fall back to the thread-per-env kernels; forcing lanes-per-joint is an error
+    # trees run the tree lanes-per-joint kernel; beyond 32 joints only thread-per-env exists
     m9, _ = _pair(S.arm9())
-    assert sa.kernel_variant(m9) == "thread_per_env"
+    assert sa.kernel_variant(m9) == "lanes_per_joint" and not m9.is_chain
+    big = S.chain(40, seed=5)
+    mb, _ = _pair(big)
+    assert sa.kernel_variant(mb) == "thread_per_env"
     with pytest.raises(RuntimeError):
-        sa.set_kernel_variant(m9, "lanes_per_joint")
+        sa.set_kernel_variant(mb, "lanes_per_joint")
 
 
 def test_fast_kernel_partial_outputs_and_ragged_batches():
@@ -283,3 +286,71 @@ def test_every_chain_shape_matches_oracle(n):
         assert tune.get_shape(m, 300) == (G, C)
         _check_kin(m, o, q, qd, qdd)
     tune.set_shape(m, 0, 0, 0)  # clear tuned entries
+
+
+# ----------------------------------------------------------------- trees and heterogeneous launches
+@pytest.mark.parametrize("tree", ["arm9", "quad18", "torso30"])
+def test_tree_kernel_matches_oracle_and_generic(tree):
+    import sapien_amd as sa
+
+    rob = S.robot(tree)
+    m, o = _pair(rob)
+    assert sa.kernel_variant(m) == "lanes_per_joint" and not m.is_chain
+    q, qd, qdd, _ = S.samples(rob, 777, 1500)
+    _check_kin(m, o, q, qd, qdd)
+    M = m.compute_generalized_mass_matrix_batched(_t(q)).cpu().numpy()
+    assert rel_err(M, o.crba(q)) <= REL_TOL
+    fast = m.compute_fk_jacobian_rnea_batched(_t(q), _t(qd), _t(qdd)) + (m.compute_generalized_mass_matrix_batched(_t(q)),)
+    sa.set_kernel_variant(m, "thread_per_env")
+    gen = m.compute_fk_jacobian_rnea_batched(_t(q), _t(qd), _t(qdd)) + (m.compute_generalized_mass_matrix_batched(_t(q)),)
+    for a, b in zip(fast, gen):
+        assert rel_err(a.cpu().numpy(), b.cpu().numpy().astype(np.float64)) <= REL_TOL
+
+
+def test_tree_kernel_general_link_lists(golden_dir):
+    """movo_simple: several links per joint (fixed frames) takes the general link path."""
+    import xml.etree.ElementTree as ET
+
+    import sapien_amd as sa
+
+    text = open(os.path.join(golden_dir, "movo_simple.urdf")).read()
+    root = ET.fromstring(text)
+    jn = [j.get("name") for j in root.findall("joint") if j.get("type") != "fixed"]
+    ln = [l.get("name") for l in root.findall("link")]
+    m = sa.PinocchioModel(text, [0, 0, -9.81])
+    m.set_joint_order(jn)
+    m.set_link_order(ln)
+    o = OracleModel(text)
+    o.set_joint_order(jn)
+    o.set_link_order(ln)
+    rng = np.random.default_rng(8)
+    q = rng.uniform(-1, 1, size=(200, 13))
+    q[:, 0] = rng.uniform(0, 0.4, size=200)
+    qd, qdd = rng.normal(size=(200, 13)), rng.normal(size=(200, 13))
+    assert sa.kernel_variant(m) == "lanes_per_joint"
+    _check_kin(m, o, q, qd, qdd)
+    assert rel_err(m.compute_generalized_mass_matrix_batched(_t(q)).cpu().numpy(), o.crba(q)) <= REL_TOL
+
+
+def test_heterogeneous_batch_one_launch():
+    """The mixed 8192 batch (4:3:2:1) of the four trees in ONE kernel launch."""
+    import sapien_amd as sa
+
+    pairs, qs, qds, qdds, refs = [], [], [], [], []
+    for i, tree in enumerate(["chain7", "arm9", "quad18", "torso30"]):
+        rob = S.robot(tree)
+        m, o = _pair(rob)
+        q, qd, qdd, _ = S.samples(rob, MIX[tree], 1600 + i)
+        pairs.append((m, o))
+        qs.append(_t(q))
+        qds.append(_t(qd))
+        qdds.append(_t(qdd))
+        refs.append((o.link_poses(q), o.jacobian_world(q), o.rnea(q, qd, qdd), o.crba(q)))
+    outs = sa._pinocchio.hetero_batched([p[0] for p in pairs], qs, qds, qdds)
+    torch.cuda.synchronize()
+    for out, (P, J, T, M) in zip(outs, refs):
+        pos, ang = pose_errors(out["poses"].cpu().numpy(), P)
+        assert pos <= POSE_TOL_M and ang <= POSE_TOL_RAD
+        assert rel_err(out["J"].cpu().numpy(), J) <= REL_TOL
+        assert rel_err(out["tau"].cpu().numpy(), T) <= REL_TOL
+        assert rel_err(out["M"].cpu().numpy(), M) <= REL_TOL
