@@ -38,15 +38,102 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", default="auto", help="auto | generic | fast (kernel variant of the fused path)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
+    ap.add_argument("--probe-traffic", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
+
+
+COPY_BYTES = 256 << 20  # calibration copy: 256 MiB read + 256 MiB written
+
+
+def probe_traffic():
+    """Child of a rocprofv3 --pmc pass: 10 launches of the fused kernel on one
+    buffer set, then 5 device copies of 256 MiB for the counter calibration."""
+    import numpy as np
+    import torch
+
+    import sapien_amd as sa
+    from sapien_amd import synthetic as S
+    from sapien_amd import tune
+
+    dev = torch.device("cuda:0")
+    rob = S.chain7()
+    m = sa.PinocchioModel(rob.urdf, [0.0, 0.0, -9.81])
+    m.set_joint_order(rob.joint_names)
+    m.set_link_order(rob.link_names)
+    if sa.kernel_variant(m) == "lanes_per_joint":
+        tune.autotune(m, batch_sizes=(B,), use_cache=True)
+    q, qd, qdd, _ = (torch.from_numpy(x).to(dev) for x in S.samples(rob, B, 1001, dtype=np.float32))
+    P, J, T = (torch.empty(B, rob.nlinks, 7, device=dev), torch.empty(B, 6, rob.nv, device=dev),
+               torch.empty(B, rob.nv, device=dev))
+    launch = sa.native_fk_jac_rnea(m)
+    stream = torch.cuda.current_stream(dev)
+    for _ in range(10):
+        launch(B, q, qd, qdd, P, J, T, stream.cuda_stream)
+    src = torch.ones(COPY_BYTES // 4, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+
+
+def measure_traffic(timeout=120):
+    """HBM bytes per launch of the fused kernel from two rocprofv3 --pmc passes
+    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by the factor
+    that makes the same counter read 256 MiB for the 256 MiB copy kernel.
+    Runs before this process touches the GPU; returns None if rocprofv3 fails."""
+    import csv
+    import glob
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+
+    if shutil.which("rocprofv3") is None:
+        return None
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="sapamd_pmc_")
+        cmd = ["rocprofv3", "--pmc", counter, "-d", d, "-o", "pmc", "--output-format", "csv", "--",
+               sys.executable, os.path.abspath(__file__), "--probe-traffic"]
+        p = subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, start_new_session=True)
+        try:
+            p.wait(timeout=timeout)
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, signal.SIGKILL)
+            p.wait()
+            return None
+        kern, copy = [], []
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(f) as fh:
+                for row in csv.DictReader(fh):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    name, v = row.get("Kernel_Name", ""), float(row.get("Counter_Value", "nan"))
+                    if "k_chain_fused" in name or "k_kinematics" in name or "k_tree_fused" in name:
+                        kern.append(v)
+                    elif "copy" in name.lower() and "rocclr" not in name.lower():
+                        copy.append(v)
+        shutil.rmtree(d, ignore_errors=True)
+        if not kern or not copy:
+            return None
+        kern = sorted(kern)[len(kern) // 2]
+        copy = sorted(copy)[len(copy) // 2]
+        scale = (COPY_BYTES / 1024.0) / copy
+        res[counter] = {"kib_raw": kern, "copy_kib_raw": copy, "scale": scale, "bytes": kern * 1024.0 * scale}
+    return res
 
 
 def main():
     args = parse()
+    if args.probe_traffic:
+        probe_traffic()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X "
               "(the environments shard trivially across GPUs; no multi-GPU run is made).", file=sys.stderr)
         sys.exit(2)
+    traffic = None if args.no_traffic else measure_traffic()
     import numpy as np
     import torch
 
@@ -123,9 +210,16 @@ def main():
                    "shape": ({"lanes_per_env": tuned[B][0], "joints_per_lane": tuned[B][1],
                               "envs_per_wavefront": 64 // tuned[B][0], "autotuned": True} if tuned else None)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (traffic["FETCH_SIZE"]["bytes"] + traffic["WRITE_SIZE"]["bytes"]) if traffic else None,
                      "kernel_ms": kern_ms, "bytes_per_launch": BYTES_PER_EVAL * B},
     }
+    if traffic:
+        out["roofline"]["traffic_detail"] = {
+            "fetch_bytes": traffic["FETCH_SIZE"]["bytes"], "write_bytes": traffic["WRITE_SIZE"]["bytes"],
+            "fetch_scale": traffic["FETCH_SIZE"]["scale"], "write_scale": traffic["WRITE_SIZE"]["scale"],
+            "method": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch, median of 10 launches on one "
+                      "buffer set), scaled so a 256 MiB device copy reads 256 MiB"}
 
     # gradient throughput: 8192 environments of the 7-dof and 9-dof trees,
     # loss = |p_ee - FK(q*)|^2 + |tau_RNEA(q, qd, qdd) - tau*|^2. Forward-only
@@ -196,15 +290,23 @@ def main():
         o.set_joint_order(rob.joint_names)
         o.set_link_order(rob.link_names)
         threads = min(16, os.cpu_count() or 1)
-        q, qd, qdd, _ = S.samples(rob, 4096, 1001)
-        secs = o.fk_jac_rnea_timed(q, qd, qdd, threads)[0]
-        nsamp = int(min(4_000_000, max(4096, 4096 * args.cpu_seconds / max(secs, 1e-6))))
-        q, qd, qdd, _ = S.samples(rob, nsamp, 1001)
-        secs = o.fk_jac_rnea_timed(q, qd, qdd, threads)[0]
+        # one seeded chunk of 1M samples, evaluated repeatedly until the sample
+        # holds about cpu_seconds of CPU work
+        chunk = 1 << 20
+        q, qd, qdd, _ = S.samples(rob, chunk, 1001)
+        q, qd, qdd = (np.ascontiguousarray(x) for x in (q, qd, qdd))
+        bufs = o.fk_jac_rnea_timed(q[:1], qd[:1], qdd[:1], threads)[1:]
+        bufs = tuple(np.zeros((chunk,) + b.shape[1:]) for b in bufs)
+        o.fk_jac_rnea_timed(q, qd, qdd, threads, out=bufs)  # warm-up: touches every page once
+        secs, reps = 0.0, 0
+        while secs < args.cpu_seconds and reps < 1000:
+            secs += o.fk_jac_rnea_timed(q, qd, qdd, threads, out=bufs)[0]
+            reps += 1
+        nsamp = chunk * reps
         out["cpu_baseline"] = {"value": nsamp / secs, "unit": "evals/s", "cores": threads, "kind": "port",
-                               "sample": f"{nsamp} seeded chain7 samples, FP64 FK + 8 link poses + WORLD Jacobian + "
-                                         f"RNEA per sample, C++ oracle statically split over {threads} threads "
-                                         f"({secs:.1f} s)"}
+                               "sample": f"{reps} x {chunk} seeded chain7 samples, FP64 FK + 8 link poses + WORLD "
+                                         f"Jacobian + RNEA per sample, C++ oracle statically split over {threads} "
+                                         f"threads ({secs:.1f} s)"}
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out))
 

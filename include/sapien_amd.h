@@ -181,6 +181,27 @@ int sapamd_pose_torque_loss_batched(const sapamd_model *m, int64_t B, const floa
                                     const float *qdd, const float *tau_star, int32_t ee_link, const float *p_target,
                                     float *loss, float *tau, float *dq, float *dqd, float *dqdd, void *stream);
 
+/* ---------------------------------------------------------------- rollouts (FP32, batched)
+ * Semi-implicit Euler with ABA forward dynamics: qdd_t = ABA(q_t, qd_t, tau_t),
+ * qd_{t+1} = qd_t + dt qdd_t, q_{t+1} = q_t + dt qd_{t+1}. No reference
+ * counterpart (the reference exposes single ABA calls). Every step is one
+ * launch on `stream`, so a caller may capture a whole call in a HIP graph;
+ * nothing is allocated. tau is (B, T, nv). */
+/* One step in place on q, qd (B, nv) with torques tau_t rows at tau_stride. */
+int sapamd_rollout_step(const sapamd_model *m, int64_t B, float dt, float *q, float *qd, const float *tau_t,
+                        int64_t tau_stride, void *stream);
+/* T steps in place on q, qd; the pre-step state (q | qd) of steps 0, K, 2K, ...
+ * is saved to ckpt (B, ceil(T / K), 2 nv) for the backward pass. */
+int sapamd_rollout_forward(const sapamd_model *m, int64_t B, int32_t T, int32_t K, float dt, float *q, float *qd,
+                           const float *tau, float *ckpt, void *stream);
+/* Backward through T steps from the checkpoints: on entry aq, aqd (B, nv) hold
+ * dL/dq_T and dL/dqd_T, on exit dL/dq_0 and dL/dqd_0; gtau (B, T, nv) receives
+ * dL/dtau. Segments of K steps are recomputed from their checkpoint into
+ * seg (B, K, 2 nv) using q_tmp, qd_tmp (B, nv) as scratch. */
+int sapamd_rollout_backward(const sapamd_model *m, int64_t B, int32_t T, int32_t K, float dt, const float *tau,
+                            const float *ckpt, float *seg, float *q_tmp, float *qd_tmp, float *aq, float *aqd,
+                            float *gtau, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

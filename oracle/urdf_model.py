@@ -434,12 +434,14 @@ class OracleModel:
                                   _ptr(dqT), _ptr(dqdT))
         return dqT, dqdT
 
-    def fk_jac_rnea_timed(self, q, qd, qdd, threads=1):
+    def fk_jac_rnea_timed(self, q, qd, qdd, threads=1, out=None):
+        """Times the C++ loop only; pass out=(poses, J, tau) from an earlier call
+        to reuse (already touched) output memory across repeated timings."""
         q, qd, qdd = self._q(q), self._q(qd), self._q(qdd)
         B = q.shape[0]
-        poses = np.zeros((B, self.nlinks, 7))
-        J = np.zeros((B, 6, self.nj))
-        tau = np.zeros((B, self.nj))
+        if out is None:
+            out = (np.zeros((B, self.nlinks, 7)), np.zeros((B, 6, self.nj)), np.zeros((B, self.nj)))
+        poses, J, tau = out
         secs = _lib().oracle_fk_jac_rnea_timed(self.handle(), ctypes.c_int64(B), _ptr(q), _ptr(qd), _ptr(qdd),
                                                _ptr(poses), _ptr(J), _ptr(tau), ctypes.c_int(threads))
         return secs, poses, J, tau

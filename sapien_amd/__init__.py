@@ -19,7 +19,7 @@ except ImportError as e:  # pragma: no cover - exercised only on broken installs
         "sapien_amd native extension is not built; run `python -m sapien_amd.build` "
         f"(or __graft_entry__.build()) first: {e}") from e
 
-from ._pinocchio import PinocchioModel, Pose  # noqa: E402
+from ._pinocchio import PinocchioModel, Pose, hetero_batched  # noqa: E402
 
 LIBRARY_PATH = os.path.join(_HERE, "libsapien_amd.so")
 
@@ -55,4 +55,4 @@ def set_kernel_variant(model, name):
     check(lib().sapamd_set_kernel_variant(handle(model), _VARIANTS[name]))
 
 
-__all__ = ["PinocchioModel", "Pose", "LIBRARY_PATH", "native_fk_jac_rnea", "kernel_variant", "set_kernel_variant"]
+__all__ = ["PinocchioModel", "Pose", "hetero_batched", "LIBRARY_PATH", "native_fk_jac_rnea", "kernel_variant", "set_kernel_variant"]

@@ -5,6 +5,10 @@
 * ``pose_torque_loss(model, q, qd, qdd, tau_target, ee_link, p_target)``
       -> (B,) loss_b = |tau - tau*|^2 + |p_ee - t|^2; the forward kernel returns
       the loss and its gradients in one pass, backward only scales them.
+* ``rollout(model, q0, qd0, tau, dt, checkpoint_every=16)`` -> final (q_T, qd_T)
+      of semi-implicit Euler with ABA over tau (B, T, nv); the forward keeps the
+      state every K steps, backward recomputes each K-step segment from its
+      checkpoint and runs the step adjoint kernel backwards through it.
 
 Backward passes recompute the forward inside the adjoint kernel from the saved
 inputs (no intermediates are stored between forward and backward). Kernels run
@@ -70,6 +74,24 @@ class _PoseTorqueLoss(torch.autograd.Function):
         return tuple(out) + (None, None, None)
 
 
+class _Rollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, model, q0, qd0, tau, dt, K):
+        qT, qdT, ckpt = model.rollout_batched(q0.detach(), qd0.detach(), tau.detach(), dt, K)
+        ctx.model, ctx.dt, ctx.K = model, dt, K
+        ctx.dtypes = (q0.dtype, qd0.dtype, tau.dtype)
+        ctx.save_for_backward(tau.detach().to(torch.float32).contiguous(), ckpt)
+        return qT.to(q0.dtype), qdT.to(qd0.dtype)
+
+    @staticmethod
+    def backward(ctx, gq, gqd):
+        tau, ckpt = ctx.saved_tensors
+        gq = torch.zeros_like(tau[:, 0]) if gq is None else gq.detach().to(torch.float32).contiguous()
+        gqd = torch.zeros_like(tau[:, 0]) if gqd is None else gqd.detach().to(torch.float32).contiguous()
+        dq0, dqd0, dtau = ctx.model.rollout_backward_batched(tau, ckpt, ctx.dt, gq, gqd, ctx.K)
+        return None, dq0.to(ctx.dtypes[0]), dqd0.to(ctx.dtypes[1]), dtau.to(ctx.dtypes[2]), None, None
+
+
 def link_poses(model, q):
     return _LinkPoses.apply(model, q)
 
@@ -80,3 +102,7 @@ def inverse_dynamics(model, q, qd, qdd):
 
 def pose_torque_loss(model, q, qd=None, qdd=None, tau_target=None, ee_link=0, p_target=None):
     return _PoseTorqueLoss.apply(model, q, qd, qdd, tau_target, ee_link, p_target)
+
+
+def rollout(model, q0, qd0, tau, dt, checkpoint_every=16):
+    return _Rollout.apply(model, q0, qd0, tau, dt, checkpoint_every)

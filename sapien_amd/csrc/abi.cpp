@@ -923,4 +923,62 @@ int sapamd_pose_torque_loss_batched(const sapamd_model *m, int64_t B, const floa
   });
 }
 
+int sapamd_rollout_step(const sapamd_model *m, int64_t B, float dt, float *q, float *qd, const float *tau_t,
+                        int64_t tau_stride, void *stream) {
+  return guarded([&] {
+    check_batched(m, B);
+    if (B == 0) return;
+    need(q, "q");
+    need(qd, "qd");
+    need(tau_t, "tau_t");
+    HIP_OK(launch_rollout_step<float>(m->t32(), m->host->njoints(), B, dt, q, qd, tau_t, tau_stride, nullptr, 0,
+                                      S(stream)));
+  });
+}
+
+int sapamd_rollout_forward(const sapamd_model *m, int64_t B, int32_t T, int32_t K, float dt, float *q, float *qd,
+                           const float *tau, float *ckpt, void *stream) {
+  return guarded([&] {
+    check_batched(m, B);
+    if (T < 0 || K < 1) throw std::invalid_argument("T >= 0 and K >= 1 required");
+    if (B == 0 || T == 0) return;
+    need(q, "q");
+    need(qd, "qd");
+    need(tau, "tau");
+    const int nv = m->nv(), nj = m->host->njoints();
+    const int64_t S_ = (T + K - 1) / K;
+    for (int t = 0; t < T; ++t) {
+      float *save = (ckpt && t % K == 0) ? ckpt + (int64_t)(t / K) * 2 * nv : nullptr;
+      HIP_OK(launch_rollout_step<float>(m->t32(), nj, B, dt, q, qd, tau + (int64_t)t * nv, (int64_t)T * nv, save,
+                                        S_ * 2 * nv, S(stream)));
+    }
+  });
+}
+
+int sapamd_rollout_backward(const sapamd_model *m, int64_t B, int32_t T, int32_t K, float dt, const float *tau,
+                            const float *ckpt, float *seg, float *q_tmp, float *qd_tmp, float *aq, float *aqd,
+                            float *gtau, void *stream) {
+  return guarded([&] {
+    check_batched(m, B);
+    if (T < 0 || K < 1) throw std::invalid_argument("T >= 0 and K >= 1 required");
+    if (B == 0 || T == 0) return;
+    for (const void *p : {(const void *)tau, (const void *)ckpt, (const void *)seg, (const void *)q_tmp,
+                          (const void *)qd_tmp, (const void *)aq, (const void *)aqd, (const void *)gtau})
+      need(p, "rollout buffer");
+    const int nv = m->nv(), nj = m->host->njoints();
+    const int64_t S_ = (T + K - 1) / K;
+    for (int64_t s = S_ - 1; s >= 0; --s) {
+      const int t0 = (int)(s * K), t1 = std::min(T, t0 + K);
+      HIP_OK(launch_restore_state(B, nv, ckpt + s * 2 * nv, S_ * 2 * nv, q_tmp, qd_tmp, S(stream)));
+      for (int t = t0; t < t1; ++t)  // recompute the segment, keeping every pre-step state
+        HIP_OK(launch_rollout_step<float>(m->t32(), nj, B, dt, q_tmp, qd_tmp, tau + (int64_t)t * nv, (int64_t)T * nv,
+                                          seg + (int64_t)(t - t0) * 2 * nv, (int64_t)K * 2 * nv, S(stream)));
+      for (int t = t1 - 1; t >= t0; --t)
+        HIP_OK(launch_rollout_step_vjp<float>(m->t32(), nj, B, dt, seg + (int64_t)(t - t0) * 2 * nv,
+                                              (int64_t)K * 2 * nv, tau + (int64_t)t * nv, (int64_t)T * nv, aq, aqd,
+                                              gtau + (int64_t)t * nv, S(stream)));
+    }
+  });
+}
+
 }  // extern "C"

@@ -115,6 +115,23 @@ hipError_t launch_kin_vjp(const ModelTable<T> *tab, int nj, int64_t B, const T *
 hipError_t launch_chain_vjp(const ModelTable<float> *tab, int nj, int64_t B, const float *q, const float *qd,
                             const float *qdd, VjpArgs<float> a, hipStream_t s, int G, int C);
 
+// One semi-implicit Euler step with ABA, in place on q, qd (B, nv); tau_t rows
+// at tau_stride; when save != nullptr the pre-step state (q | qd) is written
+// to save rows at save_stride.
+template <class T>
+hipError_t launch_rollout_step(const ModelTable<T> *tab, int nj, int64_t B, T dt, T *q, T *qd, const T *tau,
+                               int64_t tau_stride, T *save, int64_t save_stride, hipStream_t s);
+// Adjoint of one step: state rows (q | qd) before the step, aq/aqd (B, nv)
+// hold the adjoint of the post-step state on entry and of the pre-step state
+// on exit; gtau_t rows (at tau_stride) receive dL/dtau_t.
+template <class T>
+hipError_t launch_rollout_step_vjp(const ModelTable<T> *tab, int nj, int64_t B, T dt, const T *state,
+                                   int64_t state_stride, const T *tau, int64_t tau_stride, T *aq, T *aqd, T *gtau,
+                                   hipStream_t s);
+// rows of (q | qd) at stride -> q, qd (B, nv)
+hipError_t launch_restore_state(int64_t B, int nv, const float *src, int64_t stride, float *q, float *qd,
+                                hipStream_t s);
+
 // dst[0:n] = value (a kernel, not hipMemsetAsync, so it is safe under graph capture)
 hipError_t launch_fill_i32(int32_t *dst, int32_t value, int64_t n, hipStream_t s);
 

@@ -422,6 +422,59 @@ class PyModel {
     return py::make_tuple(loss, tau, dq, dqd, dqdd);
   }
 
+  // ---------------------------------------------------------------- rollouts (DLPack, FP32)
+  // (T, K) of a (B, T, nv) torque tensor and a checkpoint interval
+  static int32_t ckpt_count(int64_t T, int64_t K) { return (int32_t)((T + K - 1) / K); }
+  py::tuple rollout_batched(py::object q0, py::object qd0, py::object tau, double dt, int64_t K, py::object stream) {
+    if (K < 1) throw py::value_error("checkpoint_every must be >= 1");
+    Batch b(*this, q0);
+    py::object qdT = b.input(qd0);
+    auto shp = b.torch.attr("as_tensor")(tau).attr("shape").cast<std::vector<int64_t>>();
+    if (shp.size() != 3 || shp[0] != b.B || shp[2] != b.n)
+      throw std::runtime_error("tau must have shape (B, T, " + std::to_string(b.n) + ")");
+    const int64_t T = shp[1];
+    py::object tauT = b.as_shape(tau, shp);
+    py::object Q = b.q.attr("clone")(), QD = qdT.attr("clone")();
+    py::object C = b.empty({b.B, T ? ckpt_count(T, K) : 0, 2 * b.n});
+    auto vQ = b.view(Q), vQD = b.view(QD), vt = b.view(tauT), vC = b.view(C);
+    check(sapamd_rollout_forward(m_, b.B, (int32_t)T, (int32_t)K, (float)dt, f(vQ), f(vQD), f(vt), f(vC),
+                                 b.stream(stream)));
+    return py::make_tuple(Q, QD, C);
+  }
+  py::tuple rollout_backward_batched(py::object tau, py::object ckpt, double dt, py::object gq, py::object gqd,
+                                     int64_t K, py::object stream) {
+    if (K < 1) throw py::value_error("checkpoint_every must be >= 1");
+    Batch b(*this, gq);
+    py::object aq = b.q.attr("clone")(), aqd = b.input(gqd).attr("clone")();
+    auto shp = b.torch.attr("as_tensor")(tau).attr("shape").cast<std::vector<int64_t>>();
+    if (shp.size() != 3 || shp[0] != b.B || shp[2] != b.n)
+      throw std::runtime_error("tau must have shape (B, T, " + std::to_string(b.n) + ")");
+    const int64_t T = shp[1];
+    py::object tauT = b.as_shape(tau, shp);
+    py::object C = b.as_shape(ckpt, {b.B, T ? ckpt_count(T, K) : 0, 2 * b.n});
+    py::object G = b.torch.attr("zeros")(py::cast(shp), py::arg("device") = b.dev,
+                                         py::arg("dtype") = b.torch.attr("float32"));
+    py::object seg = b.empty({b.B, K, 2 * b.n}), qt = b.empty({b.B, b.n}), qdt = b.empty({b.B, b.n});
+    auto vt = b.view(tauT), vC = b.view(C), vG = b.view(G), vs = b.view(seg), vqt = b.view(qt), vqdt = b.view(qdt),
+         vaq = b.view(aq), vaqd = b.view(aqd);
+    check(sapamd_rollout_backward(m_, b.B, (int32_t)T, (int32_t)K, (float)dt, f(vt), f(vC), f(vs), f(vqt), f(vqdt),
+                                  f(vaq), f(vaqd), f(vG), b.stream(stream)));
+    return py::make_tuple(aq, aqd, G);
+  }
+  // In place on q, qd (contiguous float32 (B, nv) tensors on the model's GPU):
+  // no copies, no allocation, so a caller can capture it in a CUDA/HIP graph.
+  void rollout_step_(py::object q, py::object qd, py::object tau_t, double dt, py::object stream) {
+    Batch b(*this, q);
+    for (const py::object *t : {&q, &qd, &tau_t}) {
+      if (!py::isinstance(*t, b.torch.attr("Tensor")) || !t->attr("is_contiguous")().cast<bool>() ||
+          !t->attr("dtype").equal(b.torch.attr("float32")) || !t->attr("is_cuda").cast<bool>())
+        throw std::runtime_error("rollout_step_ needs contiguous float32 GPU tensors");
+    }
+    auto vq = b.view(q), vqd = take(qd, dlabi::kFloat, {b.B, b.n}, b.device),
+         vt = take(tau_t, dlabi::kFloat, {b.B, b.n}, b.device);
+    check(sapamd_rollout_step(m_, b.B, (float)dt, f(vq), f(vqd), f(vt), b.n, b.stream(stream)));
+  }
+
   sapamd_model *raw() const { return m_; }
 
   // Heterogeneous batch: several models, one launch (FK + J + RNEA + CRBA).
@@ -684,6 +737,14 @@ PYBIND11_MODULE(_pinocchio, m) {
            py::arg("stream") = py::none(),
            "Reverse mode of link poses and RNEA: (dq, dqd, dqdd) for upstream gradients on poses (B, L, 7) "
            "and/or torques (B, nv).")
+      .def("rollout_batched", &PyModel::rollout_batched, py::arg("qpos"), py::arg("qvel"), py::arg("qf"),
+           py::arg("dt"), py::arg("checkpoint_every") = 16, py::arg("stream") = py::none(),
+           "Semi-implicit Euler rollouts with ABA: qf (B, T, nv) -> final (q, qd) and checkpoints (B, ceil(T/K), 2nv)")
+      .def("rollout_backward_batched", &PyModel::rollout_backward_batched, py::arg("qf"), py::arg("checkpoints"),
+           py::arg("dt"), py::arg("grad_qpos"), py::arg("grad_qvel"), py::arg("checkpoint_every") = 16,
+           py::arg("stream") = py::none(), "Adjoint of rollout_batched: (dL/dq0, dL/dqd0, dL/dqf)")
+      .def("rollout_step_", &PyModel::rollout_step_, py::arg("qpos"), py::arg("qvel"), py::arg("qf"), py::arg("dt"),
+           py::arg("stream") = py::none(), "One in-place rollout step (graph-capturable)")
       .def("pose_torque_loss_batched", &PyModel::pose_torque_loss_batched, py::arg("qpos"),
            py::arg("qvel") = py::none(), py::arg("qacc") = py::none(), py::arg("tau_target") = py::none(),
            py::arg("ee_link") = 0, py::arg("p_target") = py::none(), py::arg("stream") = py::none(),

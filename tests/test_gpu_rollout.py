@@ -1,0 +1,168 @@
+"""GPU: batched ABA rollouts (semi-implicit Euler) and their checkpointed adjoint.
+
+Forward: final states against the FP64 oracle rollout, relative error
+(max over the batch of |x - x_ref| / max|x_ref|) <= 1e-4 over 37 steps.
+Backward: dot-product test against the FP64 oracle tangent (forward-mode)
+rollout, <g, J v> = <J^T g, v>, relative error <= 1e-3.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from oracle.urdf_model import OracleModel  # noqa: E402
+from parity import rel_err  # noqa: E402
+from sapien_amd import synthetic as S  # noqa: E402
+
+DEV = "cuda:0"
+DT = 1.0 / 240.0
+
+
+def _pair(rob):
+    import sapien_amd as sa
+
+    m = sa.PinocchioModel(rob.urdf, [0.0, 0.0, -9.81])
+    m.set_joint_order(rob.joint_names)
+    m.set_link_order(rob.link_names)
+    o = OracleModel(rob.urdf)
+    o.set_joint_order(rob.joint_names)
+    o.set_link_order(rob.link_names)
+    return m, o
+
+
+def _t(x):
+    return torch.tensor(np.asarray(x), dtype=torch.float32, device=DEV)
+
+
+def _problem(rob, B, T, seed):
+    q0, qd0, _, _ = S.samples(rob, B, seed)
+    tau = np.random.default_rng(seed + 1).normal(scale=5.0, size=(B, T, rob.nv))
+    return q0, qd0, tau
+
+
+# quad18's mass matrix reaches cond ~1e6 (light feet under a heavy floating
+# base): its FP64 rollout moves by 1e-4 relative under a 1e-6 relative change
+# of q0, so FP32 states carry ~1e-3 after 37 steps.
+FWD_TOL = {"chain7": 1e-4, "arm9": 1e-4, "quad18": 2e-3}
+
+
+@pytest.mark.parametrize("tree", ["chain7", "arm9", "quad18"])
+def test_rollout_forward_matches_oracle(tree):
+    rob = S.robot(tree)
+    m, o = _pair(rob)
+    q0, qd0, tau = _problem(rob, 32, 37, 1500)
+    qT, qdT, ckpt = m.rollout_batched(_t(q0), _t(qd0), _t(tau), DT, 8)
+    rq, rqd = o.rollout(q0, qd0, tau, DT)
+    eq, eqd = rel_err(qT.cpu().numpy(), rq), rel_err(qdT.cpu().numpy(), rqd)
+    print(f"{tree}: rel err q {eq:.2e} qd {eqd:.2e}")
+    assert eq <= FWD_TOL[tree] and eqd <= FWD_TOL[tree]
+    # checkpoints: the pre-step state of steps 0, 8, ..., 32
+    assert tuple(ckpt.shape) == (32, 5, 2 * rob.nv)
+    assert np.allclose(ckpt[:, 0].cpu().numpy(), np.concatenate([q0, qd0], 1), atol=1e-6)
+    q8, qd8 = o.rollout(q0, qd0, tau[:, :8], DT)
+    assert rel_err(ckpt[:, 1].cpu().numpy(), np.concatenate([q8, qd8], 1)) <= FWD_TOL[tree]
+
+
+@pytest.mark.parametrize("tree", ["chain7", "arm9"])
+def test_rollout_adjoint_dot_product(tree):
+    rob = S.robot(tree)
+    m, o = _pair(rob)
+    B, T = 6, 37
+    q0, qd0, tau = _problem(rob, B, T, 1501)
+    rng = np.random.default_rng(7)
+    gq, gqd = rng.normal(size=(B, rob.nv)), rng.normal(size=(B, rob.nv))
+    vq, vqd, vtau = rng.normal(size=(B, rob.nv)), rng.normal(size=(B, rob.nv)), rng.normal(size=(B, T, rob.nv))
+    _, _, ckpt = m.rollout_batched(_t(q0), _t(qd0), _t(tau), DT, 8)
+    aq, aqd, atau = (x.cpu().numpy().astype(np.float64)
+                     for x in m.rollout_backward_batched(_t(tau), ckpt, DT, _t(gq), _t(gqd), 8))
+    for b in range(B):
+        jq, jqd = o.rollout_jvp(q0[b], qd0[b], tau[b], vq[b], vqd[b], vtau[b], DT)
+        lhs = gq[b] @ jq + gqd[b] @ jqd
+        rhs = aq[b] @ vq[b] + aqd[b] @ vqd[b] + (atau[b] * vtau[b]).sum()
+        scale = np.linalg.norm(np.concatenate([gq[b], gqd[b]])) * np.linalg.norm(np.concatenate([jq, jqd]))
+        assert abs(lhs - rhs) <= 1e-3 * max(scale, 1e-12), (b, lhs, rhs)
+
+
+def test_rollout_checkpoint_interval_invariance():
+    """The gradient does not depend on the checkpoint interval (1, 5, 16, > T)."""
+    rob = S.chain7()
+    m, _ = _pair(rob)
+    q0, qd0, tau = _problem(rob, 64, 23, 1502)
+    g = _t(np.random.default_rng(3).normal(size=(64, rob.nv)))
+    outs = []
+    for K in (1, 5, 16, 40):
+        _, _, ck = m.rollout_batched(_t(q0), _t(qd0), _t(tau), DT, K)
+        outs.append([x.cpu().numpy() for x in m.rollout_backward_batched(_t(tau), ck, DT, g, g, K)])
+    for o_ in outs[1:]:
+        for a, b in zip(outs[0], o_):
+            assert np.array_equal(a, b)
+
+
+def test_rollout_autograd_and_graph_replay():
+    from sapien_amd import autograd as A
+
+    rob = S.chain7()
+    m, o = _pair(rob)
+    B, T = 16, 20
+    q0, qd0, tau = _problem(rob, B, T, 1503)
+    qt, qdt, taut = (_t(x).requires_grad_() for x in (q0, qd0, tau))
+    qT, qdT = A.rollout(m, qt, qdt, taut, DT, 8)
+    (qT.square().sum() + 0.5 * qdT.sum()).backward()
+    aq, aqd, atau = m.rollout_backward_batched(_t(tau), m.rollout_batched(_t(q0), _t(qd0), _t(tau), DT, 8)[2], DT,
+                                               2 * qT.detach(), torch.full_like(qdT, 0.5), 8)
+    assert torch.equal(qt.grad, aq) and torch.equal(qdt.grad, aqd) and torch.equal(taut.grad, atau)
+
+    # one step captured in a graph, replayed T times with eager work between
+    # replays; every replay equals the eager step bit for bit
+    q, qd, tau_t = _t(q0), _t(qd0), torch.zeros(B, rob.nv, device=DEV)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        m.rollout_step_(q.clone(), qd.clone(), tau_t, DT)  # warm-up outside capture
+    torch.cuda.current_stream().wait_stream(s)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        m.rollout_step_(q, qd, tau_t, DT)
+    q.copy_(_t(q0))
+    qd.copy_(_t(qd0))
+    qe, qde = _t(q0), _t(qd0)
+    for t in range(T):
+        tau_t.copy_(_t(tau[:, t]))
+        g.replay()
+        m.rollout_step_(qe, qde, tau_t, DT)
+        torch.cuda.synchronize()
+        assert torch.equal(q, qe) and torch.equal(qd, qde), t
+    rq, rqd = o.rollout(q0, qd0, tau, DT)
+    assert rel_err(q.cpu().numpy(), rq) <= 1e-4 and rel_err(qd.cpu().numpy(), rqd) <= 1e-4
+
+
+def test_rollout_metric_size_finite_and_consistent():
+    """4096 rollouts x 128 steps of chain7, tau ~ N(0, 5^2): a sample matches the
+    oracle and the adjoint of (q_T, qd_T) -> sum is finite everywhere."""
+    rob = S.chain7()
+    m, o = _pair(rob)
+    B, T = 4096, 128
+    q0, qd0, tau = _problem(rob, B, T, 1504)
+    qT, qdT, ck = m.rollout_batched(_t(q0), _t(qd0), _t(tau), DT, 16)
+    idx = np.arange(0, B, 257)
+    rq, rqd = o.rollout(q0[idx], qd0[idx], tau[idx], DT)
+    assert rel_err(qT.cpu().numpy()[idx], rq) <= 1e-3
+    assert rel_err(qdT.cpu().numpy()[idx], rqd) <= 1e-3
+    one = torch.ones(B, rob.nv, device=DEV)
+    aq, aqd, atau = m.rollout_backward_batched(_t(tau), ck, DT, one, one, 16)
+    assert all(torch.isfinite(x).all() for x in (aq, aqd, atau))
+
+
+def test_rollout_errors():
+    rob = S.chain7()
+    m, _ = _pair(rob)
+    with pytest.raises(Exception):
+        m.rollout_batched(torch.zeros(4, 7, device=DEV), torch.zeros(4, 7, device=DEV),
+                          torch.zeros(4, 3, 6, device=DEV), DT)
+    with pytest.raises(Exception):
+        m.rollout_batched(torch.zeros(4, 7, device=DEV), torch.zeros(4, 7, device=DEV),
+                          torch.zeros(4, 3, 7, device=DEV), DT, 0)
+    qT, qdT, ck = m.rollout_batched(torch.zeros(4, 7, device=DEV), torch.ones(4, 7, device=DEV),
+                                    torch.zeros(4, 0, 7, device=DEV), DT)
+    assert torch.equal(qT, torch.zeros(4, 7, device=DEV)) and tuple(ck.shape) == (4, 0, 14)
