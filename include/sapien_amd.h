@@ -164,6 +164,20 @@ int sapamd_hetero_batched(const sapamd_model *const *models, int32_t nmodels, co
                           const float *const *q, const float *const *qd, const float *const *qdd,
                           float *const *poses, float *const *J, float *const *tau, float *const *M, void *stream);
 
+/* CLIK problems of several models in ONE persistent launch (at most 8
+ * models): model i solves B[i] problems for its link links[i], with targets[i]
+ * (B[i], 7), optional q0[i] (B[i], nv_i) and masks[i] (nv_i), writing
+ * q_out[i], success[i], err[i], iterations[i] (may be NULL) like
+ * sapamd_clik_batched. Lanes pull problems from one queue, so early-exiting
+ * problems of any model free their lane for the next one (no reference
+ * counterpart: the reference solves one problem per call,
+ * pinocchio_model.cpp:228-300). The queue counter belongs to models[0]. */
+int sapamd_clik_hetero_batched(const sapamd_model *const *models, int32_t nmodels, const int64_t *B,
+                               const int32_t *links, const float *const *targets, const float *const *q0,
+                               const int32_t *const *masks, double eps, int32_t max_iterations, double dt,
+                               double damp, float *const *q_out, int32_t *const *success, float *const *err,
+                               int32_t *const *iterations, void *stream);
+
 /* ---------------------------------------------------------------- adjoints (FP32, batched)
  * No reference counterpart: the reference has no derivatives. Reverse mode of
  * the batched kinematics above, for training losses through the model. */
@@ -201,6 +215,18 @@ int sapamd_rollout_forward(const sapamd_model *m, int64_t B, int32_t T, int32_t 
 int sapamd_rollout_backward(const sapamd_model *m, int64_t B, int32_t T, int32_t K, float dt, const float *tau,
                             const float *ckpt, float *seg, float *q_tmp, float *qd_tmp, float *aq, float *aqd,
                             float *gtau, void *stream);
+
+/* ---------------------------------------------------------------- forward-dynamics precision
+ * Batched forward dynamics (sapamd_aba_batched, rollouts) runs FP32 ABA unless
+ * the model's mass matrix is ill-conditioned: at set_joint_order the library
+ * estimates cond(M) (FP64 CRBA on 64 seeded configurations within the joint
+ * limits) and, above 1e6, switches to FP64 CRBA + RNEA bias + LTDL
+ * factorization (FP32 in and out). The reference computes ABA in FP64 always
+ * (pinocchio_model.cpp:221-226); this keeps its accuracy where FP32 cannot. */
+enum { SAPAMD_DYNAMICS_AUTO = 0, SAPAMD_DYNAMICS_FP32 = 1, SAPAMD_DYNAMICS_FP64 = 2 };
+int sapamd_set_dynamics_precision(sapamd_model *m, int32_t mode);
+/* Effective precision (FP32 or FP64) and the cond(M) estimate. */
+int sapamd_get_dynamics_precision(const sapamd_model *m, int32_t *mode, double *cond_estimate);
 
 #ifdef __cplusplus
 }

@@ -16,10 +16,10 @@ try:
     from . import _pinocchio  # noqa: F401
 except ImportError as e:  # pragma: no cover - exercised only on broken installs
     raise ImportError(
-        "sapien_amd native extension is not built; run `python -m sapien_amd.build` "
+        "sapien_amd native extension is not built or is stale; run `python sapien_amd/build.py` "
         f"(or __graft_entry__.build()) first: {e}") from e
 
-from ._pinocchio import PinocchioModel, Pose, hetero_batched  # noqa: E402
+from ._pinocchio import PinocchioModel, Pose, clik_hetero_batched, hetero_batched  # noqa: E402
 
 LIBRARY_PATH = os.path.join(_HERE, "libsapien_amd.so")
 
@@ -55,4 +55,30 @@ def set_kernel_variant(model, name):
     check(lib().sapamd_set_kernel_variant(handle(model), _VARIANTS[name]))
 
 
-__all__ = ["PinocchioModel", "Pose", "hetero_batched", "LIBRARY_PATH", "native_fk_jac_rnea", "kernel_variant", "set_kernel_variant"]
+_PRECISIONS = {"auto": 0, "fp32": 1, "fp64": 2}
+
+
+def dynamics_precision(model):
+    """('fp32' | 'fp64', cond_estimate): the arithmetic of batched forward
+    dynamics and rollouts for this model, and the largest cond(M) seen over
+    64 seeded configurations within the joint limits."""
+    import ctypes
+
+    from .native import check, handle, lib
+
+    mode, cond = ctypes.c_int32(0), ctypes.c_double(0)
+    check(lib().sapamd_get_dynamics_precision(handle(model), ctypes.byref(mode), ctypes.byref(cond)))
+    return ("fp64" if mode.value == 2 else "fp32"), cond.value
+
+
+def set_dynamics_precision(model, name):
+    """'auto' (FP64 factorization when cond(M) > 1e6), 'fp32' (ABA) or 'fp64'."""
+    from .native import check, handle, lib
+
+    if name not in _PRECISIONS:
+        raise ValueError(f"unknown dynamics precision {name!r}")
+    check(lib().sapamd_set_dynamics_precision(handle(model), _PRECISIONS[name]))
+
+
+__all__ = ["PinocchioModel", "Pose", "hetero_batched", "clik_hetero_batched", "LIBRARY_PATH", "native_fk_jac_rnea", "kernel_variant",
+           "set_kernel_variant", "dynamics_precision", "set_dynamics_precision"]

@@ -8,7 +8,8 @@
 Every HIP translation unit is built with packed FP32 disabled
 (``-Xclang -target-feature -Xclang -packed-fp32-ops``).
 
-Usage: ``python -m sapien_amd.build`` (or ``build()`` from Python).
+Usage: ``python sapien_amd/build.py`` (by path: importing the package needs the
+extension this builds), or ``__graft_entry__.build()``.
 """
 from __future__ import annotations
 

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <random>
 
 #include "../../include/sapien_amd.h"
 #include "launch.h"
@@ -82,6 +83,15 @@ struct sapamd_model {
   bool tables_ready = false;
   int variant = SAPAMD_KERNEL_AUTO;
   std::vector<double> pose_cache, jw_cache, jl_cache;
+  // Largest cond(M) over sampled configurations (FP64, at refresh) and the
+  // precision policy of batched forward dynamics: FP32 ABA, or FP64
+  // CRBA + LTDL when the estimate exceeds kCondLimit.
+  double cond_est = 0;
+  int dyn_precision = SAPAMD_DYNAMICS_AUTO;
+  static constexpr double kCondLimit = 1e6;
+  bool fd64() const {
+    return dyn_precision == SAPAMD_DYNAMICS_FP64 || (dyn_precision == SAPAMD_DYNAMICS_AUTO && cond_est > kCondLimit);
+  }
 
   ~sapamd_model() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -246,6 +256,73 @@ struct sapamd_model {
       buf_doubles = need;
     }
     tables_ready = true;
+    cond_est = estimate_condition();
+  }
+
+  // cond(M) = lambda_max / lambda_min over 64 seeded configurations drawn
+  // uniformly within the joint limits (continuous joints in [-pi, pi]): M from
+  // the FP64 CRBA kernel, eigenvalues by cyclic Jacobi rotations on the host.
+  double estimate_condition() {
+    const int n = nv(), S = 64;
+    if (n == 0) return 1.0;
+    std::vector<double> q((size_t)S * n), Mh((size_t)S * n * n);
+    std::mt19937_64 rng(0x5eed);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    const auto &J = host->joints();
+    for (int b = 0; b < S; ++b)
+      for (size_t j = 0; j < J.size(); ++j) {
+        const int u = host->vuser()[j];
+        if (u < 0) continue;
+        const bool cont = J[j].kind == JK_CONTINUOUS;
+        const double lo = cont ? -M_PI : J[j].lower, hi = cont ? M_PI : J[j].upper;
+        q[(size_t)b * n + u] = lo + (hi - lo) * U(rng);
+      }
+    double *dq = nullptr, *dM = nullptr;
+    HIP_OK(hipMalloc(&dq, q.size() * sizeof(double)));
+    HIP_OK(hipMalloc(&dM, Mh.size() * sizeof(double)));
+    hipError_t e = hipMemcpy(dq, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_crba<double>(t64(), host->njoints(), S, dq, dM, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = hipMemcpy(Mh.data(), dM, Mh.size() * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(dq);
+    (void)hipFree(dM);
+    HIP_OK(e);
+    double worst = 1.0;
+    std::vector<double> A((size_t)n * n);
+    for (int b = 0; b < S; ++b) {
+      std::copy(Mh.begin() + (size_t)b * n * n, Mh.begin() + (size_t)(b + 1) * n * n, A.begin());
+      for (int sweep = 0; sweep < 50; ++sweep) {
+        double off = 0;
+        for (int i = 0; i < n; ++i)
+          for (int k = i + 1; k < n; ++k) off += A[i * n + k] * A[i * n + k];
+        if (off < 1e-30) break;
+        for (int i = 0; i < n; ++i)
+          for (int k = i + 1; k < n; ++k) {
+            const double aik = A[i * n + k];
+            if (std::abs(aik) < 1e-300) continue;
+            const double th = (A[k * n + k] - A[i * n + i]) / (2 * aik);
+            const double t = (th >= 0 ? 1.0 : -1.0) / (std::abs(th) + std::sqrt(th * th + 1));
+            const double c = 1 / std::sqrt(t * t + 1), sn = t * c;
+            for (int r = 0; r < n; ++r) {  // columns i, k
+              const double ari = A[r * n + i], ark = A[r * n + k];
+              A[r * n + i] = c * ari - sn * ark;
+              A[r * n + k] = sn * ari + c * ark;
+            }
+            for (int r = 0; r < n; ++r) {  // rows i, k
+              const double air = A[i * n + r], akr = A[k * n + r];
+              A[i * n + r] = c * air - sn * akr;
+              A[k * n + r] = sn * air + c * akr;
+            }
+          }
+      }
+      double lo = A[0], hi = A[0];
+      for (int i = 1; i < n; ++i) {
+        lo = std::min(lo, A[i * n + i]);
+        hi = std::max(hi, A[i * n + i]);
+      }
+      worst = std::max(worst, lo > 0 ? hi / lo : INFINITY);
+    }
+    return worst;
   }
 
   void reset_caches() {
@@ -777,6 +854,68 @@ int sapamd_hetero_batched(const sapamd_model *const *models, int32_t nmodels, co
   });
 }
 
+int sapamd_clik_hetero_batched(const sapamd_model *const *models, int32_t nmodels, const int64_t *B,
+                               const int32_t *links, const float *const *targets, const float *const *q0,
+                               const int32_t *const *masks, double eps, int32_t max_iterations, double dt,
+                               double damp, float *const *q_out, int32_t *const *success, float *const *err,
+                               int32_t *const *iterations, void *stream) {
+  return guarded([&] {
+    if (nmodels < 1 || nmodels > kMaxSegments) throw std::invalid_argument("1 to 8 models per heterogeneous launch");
+    need(B, "B");
+    need(links, "links");
+    need(targets, "targets");
+    need(q_out, "q_out");
+    need(success, "success");
+    need(err, "err");
+    HeteroClikArgs A{};
+    A.nseg = 0;
+    A.total = 0;
+    int max_chain = 1;
+    for (int i = 0; i < nmodels; ++i) {
+      const sapamd_model *m = models[i];
+      check_batched(m, B[i]);
+      m->check_link(links[i]);
+      if (B[i] == 0) continue;
+      need(targets[i], "target");
+      need(q_out[i], "q_out");
+      need(success[i], "success");
+      need(err[i], "err");
+      ClikSegment &g = A.seg[A.nseg++];
+      g.tab = m->t32();
+      g.P.link = links[i];
+      g.P.chain = m->d_chain + m->chain_off[links[i]];
+      g.P.nchain = m->chain_len[links[i]];
+      g.P.eps = eps;
+      g.P.dt = dt;
+      g.P.damp = damp;
+      g.P.max_iter = max_iterations;
+      g.P.mask = masks ? masks[i] : nullptr;
+      g.P.mask_ld = 0;
+      g.P.counter = nullptr;
+      g.P.persistent_waves = 0;
+      g.target = targets[i];
+      g.q0 = q0 ? q0[i] : nullptr;
+      g.q_out = q_out[i];
+      g.success = success[i];
+      g.err = err[i];
+      g.iters = iterations ? iterations[i] : nullptr;
+      g.B = B[i];
+      g.first = A.total;
+      A.total += B[i];
+      max_chain = std::max(max_chain, g.P.nchain);
+    }
+    if (A.total == 0) return;
+    // the queue counter of the first model: overlapping launches on it race
+    A.counter = models[0]->d_counter;
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, models[0]->device) == hipSuccess) cus = prop.multiProcessorCount;
+    const int64_t waves = std::min<int64_t>(4 * cus, (A.total + 63) / 64);
+    HIP_OK(launch_fill_i32(A.counter, 0, 1, S(stream)));
+    HIP_OK(launch_clik_hetero(A, max_chain, (int)waves, S(stream)));
+  });
+}
+
 int sapamd_coriolis_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd, float *C,
                             void *stream) {
   return guarded([&] {
@@ -798,7 +937,10 @@ int sapamd_aba_batched(const sapamd_model *m, int64_t B, const float *q, const f
     need(qd, "qd");
     need(tau, "tau");
     need(qdd, "qdd");
-    HIP_OK(launch_aba<float>(m->t32(), m->host->njoints(), B, q, qd, tau, qdd, S(stream)));
+    if (m->fd64())
+      HIP_OK(launch_fd_ldlt(m->t64(), m->host->njoints(), B, q, qd, tau, qdd, S(stream)));
+    else
+      HIP_OK(launch_aba<float>(m->t32(), m->host->njoints(), B, q, qd, tau, qdd, S(stream)));
   });
 }
 
@@ -931,8 +1073,8 @@ int sapamd_rollout_step(const sapamd_model *m, int64_t B, float dt, float *q, fl
     need(q, "q");
     need(qd, "qd");
     need(tau_t, "tau_t");
-    HIP_OK(launch_rollout_step<float>(m->t32(), m->host->njoints(), B, dt, q, qd, tau_t, tau_stride, nullptr, 0,
-                                      S(stream)));
+    HIP_OK(launch_rollout_step<float>(m->t32(), m->fd64() ? m->t64() : nullptr, m->host->njoints(), B, dt, q, qd,
+                                      tau_t, tau_stride, nullptr, 0, S(stream)));
   });
 }
 
@@ -946,11 +1088,12 @@ int sapamd_rollout_forward(const sapamd_model *m, int64_t B, int32_t T, int32_t 
     need(qd, "qd");
     need(tau, "tau");
     const int nv = m->nv(), nj = m->host->njoints();
+    const ModelTable<double> *t64 = m->fd64() ? m->t64() : nullptr;
     const int64_t S_ = (T + K - 1) / K;
     for (int t = 0; t < T; ++t) {
       float *save = (ckpt && t % K == 0) ? ckpt + (int64_t)(t / K) * 2 * nv : nullptr;
-      HIP_OK(launch_rollout_step<float>(m->t32(), nj, B, dt, q, qd, tau + (int64_t)t * nv, (int64_t)T * nv, save,
-                                        S_ * 2 * nv, S(stream)));
+      HIP_OK(launch_rollout_step<float>(m->t32(), t64, nj, B, dt, q, qd, tau + (int64_t)t * nv, (int64_t)T * nv,
+                                        save, S_ * 2 * nv, S(stream)));
     }
   });
 }
@@ -966,18 +1109,37 @@ int sapamd_rollout_backward(const sapamd_model *m, int64_t B, int32_t T, int32_t
                           (const void *)qd_tmp, (const void *)aq, (const void *)aqd, (const void *)gtau})
       need(p, "rollout buffer");
     const int nv = m->nv(), nj = m->host->njoints();
+    const ModelTable<double> *t64 = m->fd64() ? m->t64() : nullptr;
     const int64_t S_ = (T + K - 1) / K;
     for (int64_t s = S_ - 1; s >= 0; --s) {
       const int t0 = (int)(s * K), t1 = std::min(T, t0 + K);
       HIP_OK(launch_restore_state(B, nv, ckpt + s * 2 * nv, S_ * 2 * nv, q_tmp, qd_tmp, S(stream)));
       for (int t = t0; t < t1; ++t)  // recompute the segment, keeping every pre-step state
-        HIP_OK(launch_rollout_step<float>(m->t32(), nj, B, dt, q_tmp, qd_tmp, tau + (int64_t)t * nv, (int64_t)T * nv,
-                                          seg + (int64_t)(t - t0) * 2 * nv, (int64_t)K * 2 * nv, S(stream)));
+        HIP_OK(launch_rollout_step<float>(m->t32(), t64, nj, B, dt, q_tmp, qd_tmp, tau + (int64_t)t * nv,
+                                          (int64_t)T * nv, seg + (int64_t)(t - t0) * 2 * nv, (int64_t)K * 2 * nv,
+                                          S(stream)));
       for (int t = t1 - 1; t >= t0; --t)
-        HIP_OK(launch_rollout_step_vjp<float>(m->t32(), nj, B, dt, seg + (int64_t)(t - t0) * 2 * nv,
+        HIP_OK(launch_rollout_step_vjp<float>(m->t32(), t64, nj, B, dt, seg + (int64_t)(t - t0) * 2 * nv,
                                               (int64_t)K * 2 * nv, tau + (int64_t)t * nv, (int64_t)T * nv, aq, aqd,
                                               gtau + (int64_t)t * nv, S(stream)));
     }
+  });
+}
+
+int sapamd_set_dynamics_precision(sapamd_model *m, int32_t mode) {
+  return guarded([&] {
+    if (!m) throw std::invalid_argument("null model");
+    if (mode < SAPAMD_DYNAMICS_AUTO || mode > SAPAMD_DYNAMICS_FP64)
+      throw std::invalid_argument("unknown dynamics precision");
+    m->dyn_precision = mode;
+  });
+}
+
+int sapamd_get_dynamics_precision(const sapamd_model *m, int32_t *mode, double *cond_estimate) {
+  return guarded([&] {
+    if (!m) throw std::invalid_argument("null model");
+    if (mode) *mode = m->fd64() ? SAPAMD_DYNAMICS_FP64 : SAPAMD_DYNAMICS_FP32;
+    if (cond_estimate) *cond_estimate = m->cond_est;
   });
 }
 

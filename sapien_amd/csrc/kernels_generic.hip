@@ -468,16 +468,15 @@ template <class T> __device__ __forceinline__ void pose_to_mat(const T *P, T *R,
   p = {P[0], P[1], P[2]};
 }
 
+// One CLIK problem (pinocchio_model.cpp:228-300) of a batch sharing the target link.
 template <class T, int MJ>
-__global__ __launch_bounds__(64) void k_clik(const ModelTable<T> *__restrict__ M, int64_t B, ClikParams P,
-                                             const T *__restrict__ target, const T *__restrict__ q0,
-                                             T *__restrict__ q_out, int32_t *__restrict__ success,
-                                             T *__restrict__ err_out, int32_t *__restrict__ iters) {
+__device__ void clik_problem(const ModelTable<T> *__restrict__ M, const ClikParams &P, const T *__restrict__ target,
+                             const T *__restrict__ q0, T *__restrict__ q_out, int32_t *__restrict__ success,
+                             T *__restrict__ err_out, int32_t *__restrict__ iters, int64_t prob) {
   const int nv = M->nv, nc = P.nchain;
   const JointRec<T> *JR = joints_of(M);
   const LinkRec<T> &Lk = links_of(M)[P.link];
-  int64_t prob = P.persistent_waves ? (int64_t)atomicAdd(P.counter, 1) : blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  while (prob < B) {
+  {
     // desired pose of the link's joint frame: oMdes = target * link2joint^-1
     T Rt[9], Rd[9];
     V3<T> pt;
@@ -534,9 +533,11 @@ __global__ __launch_bounds__(64) void k_clik(const ModelTable<T> *__restrict__ M
         for (int j = 0; j < 3; ++j) Rinv[3 * i + j] = Ri[3 * j + i];
       T Jl[36];
       jlog6_dev(Rinv, T(-1) * mtv(Ri, pi), Jl);
+      // J J^T + damp I and its LDL^T solve in FP64 whatever T is: with
+      // damp = 1e-6 the 6x6 system near a singular pose has cond ~1e6+.
       T Jk[MJ][6];
-      T JJt[36];
-      for (int k = 0; k < 36; ++k) JJt[k] = T(0);
+      double JJt[36];
+      for (int k = 0; k < 36; ++k) JJt[k] = 0.0;
       for (int k = 0; k < nc; ++k) {
         S6<T> cw = jcol(JR[P.chain[k]], oR[k], op[k]);
         S6<T> cl = to_local(Rj, pj, cw);
@@ -548,18 +549,18 @@ __global__ __launch_bounds__(64) void k_clik(const ModelTable<T> *__restrict__ M
           Jk[k][r] = -acc * msk[k];
         }
         for (int r = 0; r < 6; ++r)
-          for (int m = 0; m <= r; ++m) JJt[r * 6 + m] += Jk[k][r] * Jk[k][m];
+          for (int m = 0; m <= r; ++m) JJt[r * 6 + m] += double(Jk[k][r]) * double(Jk[k][m]);
       }
       for (int r = 0; r < 6; ++r) {
-        JJt[r * 6 + r] += T(P.damp);
+        JJt[r * 6 + r] += P.damp;
         for (int m = 0; m < r; ++m) JJt[m * 6 + r] = JJt[r * 6 + m];
       }
-      T y[6];
-      for (int r = 0; r < 6; ++r) y[r] = ev[r];
+      double y[6];
+      for (int r = 0; r < 6; ++r) y[r] = double(ev[r]);
       ldlt6_solve(JJt, y);
       for (int k = 0; k < nc; ++k) {
         T v = T(0);
-        for (int r = 0; r < 6; ++r) v -= Jk[k][r] * y[r];
+        for (int r = 0; r < 6; ++r) v -= Jk[k][r] * T(y[r]);
         qc[k] += v * T(P.dt);
       }
     }
@@ -579,8 +580,32 @@ __global__ __launch_bounds__(64) void k_clik(const ModelTable<T> *__restrict__ M
     success[prob] = ok ? 1 : 0;
     for (int r = 0; r < 6; ++r) err_out[prob * 6 + r] = bestErr[r];
     if (iters) iters[prob] = it;
-    if (!P.persistent_waves) break;
-    prob = atomicAdd(P.counter, 1);
+  }
+}
+
+template <class T, int MJ>
+__global__ __launch_bounds__(64) void k_clik(const ModelTable<T> *__restrict__ M, int64_t B, ClikParams P,
+                                             const T *__restrict__ target, const T *__restrict__ q0,
+                                             T *__restrict__ q_out, int32_t *__restrict__ success,
+                                             T *__restrict__ err_out, int32_t *__restrict__ iters) {
+  if (!P.persistent_waves) {
+    const int64_t prob = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (prob < B) clik_problem<T, MJ>(M, P, target, q0, q_out, success, err_out, iters, prob);
+    return;
+  }
+  // persistent: every lane pulls problems from the queue until it runs dry
+  for (int64_t prob = atomicAdd(P.counter, 1); prob < B; prob = atomicAdd(P.counter, 1))
+    clik_problem<T, MJ>(M, P, target, q0, q_out, success, err_out, iters, prob);
+}
+
+// Problems of several models (segments) in one persistent launch: a lane takes
+// the next global problem index and runs it on the segment that owns it.
+template <int MJ> __global__ __launch_bounds__(64) void k_clik_hetero(HeteroClikArgs A) {
+  for (int64_t g = atomicAdd(A.counter, 1); g < A.total; g = atomicAdd(A.counter, 1)) {
+    int s = 0;
+    while (s + 1 < A.nseg && g >= A.seg[s + 1].first) ++s;
+    const ClikSegment &S = A.seg[s];
+    clik_problem<float, MJ>(S.tab, S.P, S.target, S.q0, S.q_out, S.success, S.err, S.iters, g - S.first);
   }
 }
 
@@ -649,6 +674,16 @@ hipError_t launch_clik(const ModelTable<T> *tab, int nj, int64_t B, const ClikPa
   if (B <= 0) return hipSuccess;
   dim3 grid = P.persistent_waves ? dim3((unsigned)P.persistent_waves) : grid_for(B);
   SAPAMD_DISPATCH_MJ(P.nchain, k_clik, grid, dim3(64), 0, s, tab, B, P, target, q0, q_out, success, err, iters);
+  return hipGetLastError();
+}
+
+hipError_t launch_clik_hetero(const HeteroClikArgs &A, int max_chain, int waves, hipStream_t s) {
+  if (A.total <= 0) return hipSuccess;
+  const dim3 grid((unsigned)waves);
+  if (max_chain <= 8) hipLaunchKernelGGL((k_clik_hetero<8>), grid, dim3(64), 0, s, A);
+  else if (max_chain <= 16) hipLaunchKernelGGL((k_clik_hetero<16>), grid, dim3(64), 0, s, A);
+  else if (max_chain <= 32) hipLaunchKernelGGL((k_clik_hetero<32>), grid, dim3(64), 0, s, A);
+  else hipLaunchKernelGGL((k_clik_hetero<64>), grid, dim3(64), 0, s, A);
   return hipGetLastError();
 }
 

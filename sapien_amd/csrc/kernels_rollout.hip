@@ -1,5 +1,6 @@
 // Semi-implicit Euler rollouts with ABA forward dynamics and their adjoint
-// (thread-per-environment, any tree up to 64 joints, FP32 / FP64).
+// (thread-per-environment, any tree up to 64 joints; FP32 ABA, or FP64
+// factorized forward dynamics for ill-conditioned trees).
 //
 // Step t:  qdd_t = ABA(q_t, qd_t, tau_t);  qd_{t+1} = qd_t + dt qdd_t;  q_{t+1} = q_t + dt qd_{t+1}.
 // Adjoint of a step, from RNEA(q, qd, qdd) = tau (implicit differentiation):
@@ -147,8 +148,105 @@ __device__ __forceinline__ void rnea_vjp_sub(const ModelTable<T> *M, const T (&o
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP64 forward dynamics by factorization, for trees whose mass matrix is
+// ill-conditioned (cond(M) > 1e6, where FP32 ABA loses too many digits):
+// M(q) by the composite-rigid-body algorithm, the bias b(q, qd) = RNEA(q, qd, 0)
+// (gravity included), then Featherstone's LTDL factorization M = L^T D L,
+// which keeps the tree's sparsity (row k of L is nonzero only at ancestors of
+// k, so there is no fill-in), and the matching back-substitution.
+// Everything in FP64; joints in internal (depth-first) order, parents first.
+template <int MJ> struct Fd64 {
+  double H[MJ][MJ];  // H[k][i] for i ancestor-or-self of k; LTDL factors in place
+  int par[MJ];
+  int nj;
+
+  __device__ __forceinline__ void factor() {
+    for (int k = nj - 1; k >= 0; --k)
+      for (int i = par[k]; i >= 0; i = par[i]) {
+        const double a = H[k][i] / H[k][k];
+        for (int j = i; j >= 0; j = par[j]) H[i][j] -= a * H[k][j];
+        H[k][i] = a;
+      }
+  }
+  // y <- M^-1 y (internal order)
+  __device__ __forceinline__ void solve(double *y) const {
+    for (int i = nj - 1; i >= 0; --i)
+      for (int j = par[i]; j >= 0; j = par[j]) y[j] -= H[i][j] * y[i];
+    for (int i = 0; i < nj; ++i) y[i] /= H[i][i];
+    for (int i = 0; i < nj; ++i)
+      for (int j = par[i]; j >= 0; j = par[j]) y[i] -= H[i][j] * y[j];
+  }
+};
+
+// Builds M (into F.H) and b = RNEA(q, qd, 0) (internal order) for user-order q, qd.
+template <int MJ>
+__device__ __forceinline__ void mass_and_bias_f64(const ModelTable<double> *M, const double *q, const double *qd,
+                                                  Fd64<MJ> &F, double *bias) {
+  const JointRec<double> *JR = joints_of(M);
+  const int nj = M->nj;
+  F.nj = nj;
+  double oR[MJ][9];
+  V3<double> op[MJ];
+  S6<double> col[MJ], vel[MJ], acc[MJ], f[MJ];
+  AInertia<double> Yc[MJ];
+  const S6<double> a0 = {{-M->gravity[0], -M->gravity[1], -M->gravity[2]}, {0.0, 0.0, 0.0}};
+  for (int j = 0; j < nj; ++j) {
+    const JointRec<double> &J = JR[j];
+    const int p = J.parent, u = J.vuser;
+    F.par[j] = p;
+    double s, c;
+    sincos_t(q[u], &s, &c);
+    joint_world(J, q[u], s, c, p >= 0 ? oR[p] : static_cast<const double *>(nullptr),
+                p >= 0 ? op[p] : V3<double>{0, 0, 0}, p < 0, oR[j], op[j]);
+    col[j] = jcol(J, oR[j], op[j]);
+    const WInertia<double> Y = world_inertia(J, oR[j], op[j]);
+    vel[j] = (p >= 0 ? vel[p] : zero6<double>()) + qd[u] * col[j];
+    acc[j] = (p >= 0 ? acc[p] : a0) + qd[u] * crossm(vel[j], col[j]);
+    f[j] = inertia_apply(Y, acc[j]) + crossf(vel[j], inertia_apply(Y, vel[j]));
+    Yc[j] = additive(Y);
+  }
+  for (int j = nj - 1; j > 0; --j) {
+    const int p = JR[j].parent;
+    if (p < 0) continue;
+    add_to(Yc[p], Yc[j]);
+    f[p] = f[p] + f[j];
+  }
+  for (int j = 0; j < nj; ++j) {
+    bias[j] = dot6(col[j], f[j]);
+    const S6<double> Fj = apply(Yc[j], col[j]);
+    for (int i = j; i >= 0; i = JR[i].parent) F.H[j][i] = dot6(col[i], Fj);
+  }
+}
+
+// qdd = M^-1 (tau - b) and, when rhs2 != nullptr, out2 = M^-1 rhs2 (user order,
+// FP32 in and out, FP64 inside).
 template <class T, int MJ>
-__global__ __launch_bounds__(64) void k_rollout_step(const ModelTable<T> *__restrict__ M, int64_t B, T dt,
+__device__ __forceinline__ void fd_ldlt(const ModelTable<double> *M, const T *q, const T *qd, const T *tau,
+                                        const T *rhs2, T *qdd, T *out2) {
+  const JointRec<double> *JR = joints_of(M);
+  const int nj = M->nj;
+  double qq[MJ], qv[MJ], y[MJ], b[MJ];
+  for (int i = 0; i < nj; ++i) {
+    qq[i] = double(q[i]);
+    qv[i] = double(qd[i]);
+  }
+  Fd64<MJ> F;
+  mass_and_bias_f64<MJ>(M, qq, qv, F, b);
+  F.factor();
+  for (int j = 0; j < nj; ++j) y[j] = double(tau[JR[j].vuser]) - b[j];
+  F.solve(y);
+  for (int j = 0; j < nj; ++j) qdd[JR[j].vuser] = T(y[j]);
+  if (rhs2) {
+    for (int j = 0; j < nj; ++j) y[j] = double(rhs2[JR[j].vuser]);
+    F.solve(y);
+    for (int j = 0; j < nj; ++j) out2[JR[j].vuser] = T(y[j]);
+  }
+}
+
+template <class T, int MJ, bool F64>
+__global__ __launch_bounds__(64) void k_rollout_step(const ModelTable<T> *__restrict__ M,
+                                                     const ModelTable<double> *__restrict__ M64, int64_t B, T dt,
                                                      T *__restrict__ q, T *__restrict__ qd,
                                                      const T *__restrict__ tau, int64_t tau_stride,
                                                      T *__restrict__ save, int64_t save_stride) {
@@ -167,11 +265,15 @@ __global__ __launch_bounds__(64) void k_rollout_step(const ModelTable<T> *__rest
       save[b * save_stride + nv + i] = qdb[i];
     }
   }
-  T oR[MJ][9];
-  V3<T> op[MJ];
-  S6<T> col[MJ];
-  fk_cols<T, MJ>(M, qb, oR, op, col);
-  aba_world<T, MJ>(M, oR, op, col, qdb, tb, true, qdd);
+  if constexpr (F64) {
+    fd_ldlt<T, MJ>(M64, qb, qdb, tb, static_cast<const T *>(nullptr), qdd, static_cast<T *>(nullptr));
+  } else {
+    T oR[MJ][9];
+    V3<T> op[MJ];
+    S6<T> col[MJ];
+    fk_cols<T, MJ>(M, qb, oR, op, col);
+    aba_world<T, MJ>(M, oR, op, col, qdb, tb, true, qdd);
+  }
   for (int i = 0; i < nv; ++i) {
     const T v = qdb[i] + dt * qdd[i];
     qd[b * nv + i] = v;
@@ -179,8 +281,9 @@ __global__ __launch_bounds__(64) void k_rollout_step(const ModelTable<T> *__rest
   }
 }
 
-template <class T, int MJ>
-__global__ __launch_bounds__(64) void k_rollout_step_vjp(const ModelTable<T> *__restrict__ M, int64_t B, T dt,
+template <class T, int MJ, bool F64>
+__global__ __launch_bounds__(64) void k_rollout_step_vjp(const ModelTable<T> *__restrict__ M,
+                                                         const ModelTable<double> *__restrict__ M64, int64_t B, T dt,
                                                          const T *__restrict__ state, int64_t state_stride,
                                                          const T *__restrict__ tau, int64_t tau_stride,
                                                          T *__restrict__ aq, T *__restrict__ aqd,
@@ -202,14 +305,36 @@ __global__ __launch_bounds__(64) void k_rollout_step_vjp(const ModelTable<T> *__
   V3<T> op[MJ];
   S6<T> col[MJ];
   fk_cols<T, MJ>(M, qb, oR, op, col);
-  aba_world<T, MJ>(M, oR, op, col, qdb, tb, true, qdd);
-  aba_world<T, MJ>(M, oR, op, col, static_cast<const T *>(nullptr), rhs, false, lam);
+  if constexpr (F64) {
+    fd_ldlt<T, MJ>(M64, qb, qdb, tb, rhs, qdd, lam);
+  } else {
+    aba_world<T, MJ>(M, oR, op, col, qdb, tb, true, qdd);
+    aba_world<T, MJ>(M, oR, op, col, static_cast<const T *>(nullptr), rhs, false, lam);
+  }
   rnea_vjp_sub<T, MJ>(M, oR, op, col, qdb, qdd, lam, dq, dqd);
   for (int i = 0; i < nv; ++i) {
     aq[b * nv + i] = dq[i];
     aqd[b * nv + i] = dqd[i];
     gtau[b * tau_stride + i] = lam[i];
   }
+}
+
+// Batched forward dynamics through the FP64 factorization (FP32 in and out).
+template <int MJ>
+__global__ __launch_bounds__(64) void k_fd_ldlt(const ModelTable<double> *__restrict__ M64, int64_t B,
+                                                const float *__restrict__ q, const float *__restrict__ qd,
+                                                const float *__restrict__ tau, float *__restrict__ qdd) {
+  const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const int nv = M64->nv;
+  float qb[MJ], qdb[MJ], tb[MJ], out[MJ];
+  for (int i = 0; i < nv; ++i) {
+    qb[i] = q[b * nv + i];
+    qdb[i] = qd[b * nv + i];
+    tb[i] = tau[b * nv + i];
+  }
+  fd_ldlt<float, MJ>(M64, qb, qdb, tb, static_cast<const float *>(nullptr), out, static_cast<float *>(nullptr));
+  for (int i = 0; i < nv; ++i) qdd[b * nv + i] = out[i];
 }
 
 inline int pick(int nj) { return nj <= 8 ? 8 : nj <= 16 ? 16 : nj <= 32 ? 32 : 64; }
@@ -232,39 +357,57 @@ hipError_t launch_restore_state(int64_t B, int nv, const float *src, int64_t str
   return hipGetLastError();
 }
 
+#define SAPAMD_ROLLOUT_DISPATCH(KERNEL, ...)                                                          \
+  switch (pick(nj) * 2 + (tab64 ? 1 : 0)) {                                                            \
+    case 16: hipLaunchKernelGGL((KERNEL<T, 8, false>), grid, dim3(64), 0, s, __VA_ARGS__); break;      \
+    case 17: hipLaunchKernelGGL((KERNEL<T, 8, true>), grid, dim3(64), 0, s, __VA_ARGS__); break;       \
+    case 32: hipLaunchKernelGGL((KERNEL<T, 16, false>), grid, dim3(64), 0, s, __VA_ARGS__); break;     \
+    case 33: hipLaunchKernelGGL((KERNEL<T, 16, true>), grid, dim3(64), 0, s, __VA_ARGS__); break;      \
+    case 64: hipLaunchKernelGGL((KERNEL<T, 32, false>), grid, dim3(64), 0, s, __VA_ARGS__); break;     \
+    case 65: hipLaunchKernelGGL((KERNEL<T, 32, true>), grid, dim3(64), 0, s, __VA_ARGS__); break;      \
+    case 128: hipLaunchKernelGGL((KERNEL<T, 64, false>), grid, dim3(64), 0, s, __VA_ARGS__); break;    \
+    default: hipLaunchKernelGGL((KERNEL<T, 64, true>), grid, dim3(64), 0, s, __VA_ARGS__); break;      \
+  }
+
 template <class T>
-hipError_t launch_rollout_step(const ModelTable<T> *tab, int nj, int64_t B, T dt, T *q, T *qd, const T *tau,
-                               int64_t tau_stride, T *save, int64_t save_stride, hipStream_t s) {
+hipError_t launch_rollout_step(const ModelTable<T> *tab, const ModelTable<double> *tab64, int nj, int64_t B, T dt,
+                               T *q, T *qd, const T *tau, int64_t tau_stride, T *save, int64_t save_stride,
+                               hipStream_t s) {
   if (B <= 0) return hipSuccess;
   dim3 grid((unsigned)((B + 63) / 64));
-  switch (pick(nj)) {
-    case 8: hipLaunchKernelGGL((k_rollout_step<T, 8>), grid, dim3(64), 0, s, tab, B, dt, q, qd, tau, tau_stride, save, save_stride); break;
-    case 16: hipLaunchKernelGGL((k_rollout_step<T, 16>), grid, dim3(64), 0, s, tab, B, dt, q, qd, tau, tau_stride, save, save_stride); break;
-    case 32: hipLaunchKernelGGL((k_rollout_step<T, 32>), grid, dim3(64), 0, s, tab, B, dt, q, qd, tau, tau_stride, save, save_stride); break;
-    default: hipLaunchKernelGGL((k_rollout_step<T, 64>), grid, dim3(64), 0, s, tab, B, dt, q, qd, tau, tau_stride, save, save_stride); break;
-  }
+  SAPAMD_ROLLOUT_DISPATCH(k_rollout_step, tab, tab64, B, dt, q, qd, tau, tau_stride, save, save_stride);
   return hipGetLastError();
 }
 
 template <class T>
-hipError_t launch_rollout_step_vjp(const ModelTable<T> *tab, int nj, int64_t B, T dt, const T *state,
-                                   int64_t state_stride, const T *tau, int64_t tau_stride, T *aq, T *aqd, T *gtau,
-                                   hipStream_t s) {
+hipError_t launch_rollout_step_vjp(const ModelTable<T> *tab, const ModelTable<double> *tab64, int nj, int64_t B,
+                                   T dt, const T *state, int64_t state_stride, const T *tau, int64_t tau_stride,
+                                   T *aq, T *aqd, T *gtau, hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  dim3 grid((unsigned)((B + 63) / 64));
+  SAPAMD_ROLLOUT_DISPATCH(k_rollout_step_vjp, tab, tab64, B, dt, state, state_stride, tau, tau_stride, aq, aqd,
+                          gtau);
+  return hipGetLastError();
+}
+
+hipError_t launch_fd_ldlt(const ModelTable<double> *tab64, int nj, int64_t B, const float *q, const float *qd,
+                          const float *tau, float *qdd, hipStream_t s) {
   if (B <= 0) return hipSuccess;
   dim3 grid((unsigned)((B + 63) / 64));
   switch (pick(nj)) {
-    case 8: hipLaunchKernelGGL((k_rollout_step_vjp<T, 8>), grid, dim3(64), 0, s, tab, B, dt, state, state_stride, tau, tau_stride, aq, aqd, gtau); break;
-    case 16: hipLaunchKernelGGL((k_rollout_step_vjp<T, 16>), grid, dim3(64), 0, s, tab, B, dt, state, state_stride, tau, tau_stride, aq, aqd, gtau); break;
-    case 32: hipLaunchKernelGGL((k_rollout_step_vjp<T, 32>), grid, dim3(64), 0, s, tab, B, dt, state, state_stride, tau, tau_stride, aq, aqd, gtau); break;
-    default: hipLaunchKernelGGL((k_rollout_step_vjp<T, 64>), grid, dim3(64), 0, s, tab, B, dt, state, state_stride, tau, tau_stride, aq, aqd, gtau); break;
+    case 8: hipLaunchKernelGGL((k_fd_ldlt<8>), grid, dim3(64), 0, s, tab64, B, q, qd, tau, qdd); break;
+    case 16: hipLaunchKernelGGL((k_fd_ldlt<16>), grid, dim3(64), 0, s, tab64, B, q, qd, tau, qdd); break;
+    case 32: hipLaunchKernelGGL((k_fd_ldlt<32>), grid, dim3(64), 0, s, tab64, B, q, qd, tau, qdd); break;
+    default: hipLaunchKernelGGL((k_fd_ldlt<64>), grid, dim3(64), 0, s, tab64, B, q, qd, tau, qdd); break;
   }
   return hipGetLastError();
 }
 
-template hipError_t launch_rollout_step<float>(const ModelTable<float> *, int, int64_t, float, float *, float *,
-                                               const float *, int64_t, float *, int64_t, hipStream_t);
-template hipError_t launch_rollout_step_vjp<float>(const ModelTable<float> *, int, int64_t, float, const float *,
-                                                   int64_t, const float *, int64_t, float *, float *, float *,
-                                                   hipStream_t);
+template hipError_t launch_rollout_step<float>(const ModelTable<float> *, const ModelTable<double> *, int, int64_t,
+                                               float, float *, float *, const float *, int64_t, float *, int64_t,
+                                               hipStream_t);
+template hipError_t launch_rollout_step_vjp<float>(const ModelTable<float> *, const ModelTable<double> *, int,
+                                                   int64_t, float, const float *, int64_t, const float *, int64_t,
+                                                   float *, float *, float *, hipStream_t);
 
 }  // namespace sapamd

@@ -82,6 +82,24 @@ template <class T>
 hipError_t launch_clik(const ModelTable<T> *tab, int nj, int64_t B, const ClikParams &P, const T *target,
                        const T *q0, T *q_out, int32_t *success, T *err, int32_t *iters, hipStream_t s);
 
+// CLIK problems of several models in one persistent launch (FP32). Segment s
+// owns global problems [first, first + B); A.counter is zeroed before the launch.
+struct ClikSegment {
+  const ModelTable<float> *tab;
+  ClikParams P;
+  const float *target, *q0;
+  float *q_out, *err;
+  int32_t *success, *iters;
+  int64_t B, first;
+};
+struct HeteroClikArgs {
+  ClikSegment seg[kMaxSegments];
+  int32_t nseg;
+  int64_t total;
+  int32_t *counter;
+};
+hipError_t launch_clik_hetero(const HeteroClikArgs &A, int max_chain, int waves, hipStream_t s);
+
 // Lanes-per-joint fused FK + Jacobian + RNEA for serial chains of <= 16 joints
 // (FP32), G lanes per environment x C joints per lane. Returns
 // hipErrorNotSupported for other trees or unsupported shapes.
@@ -117,17 +135,22 @@ hipError_t launch_chain_vjp(const ModelTable<float> *tab, int nj, int64_t B, con
 
 // One semi-implicit Euler step with ABA, in place on q, qd (B, nv); tau_t rows
 // at tau_stride; when save != nullptr the pre-step state (q | qd) is written
-// to save rows at save_stride.
+// to save rows at save_stride. tab64 != nullptr selects FP64 factorized
+// forward dynamics (CRBA + RNEA bias + LTDL) instead of FP32 ABA.
 template <class T>
-hipError_t launch_rollout_step(const ModelTable<T> *tab, int nj, int64_t B, T dt, T *q, T *qd, const T *tau,
-                               int64_t tau_stride, T *save, int64_t save_stride, hipStream_t s);
+hipError_t launch_rollout_step(const ModelTable<T> *tab, const ModelTable<double> *tab64, int nj, int64_t B, T dt,
+                               T *q, T *qd, const T *tau, int64_t tau_stride, T *save, int64_t save_stride,
+                               hipStream_t s);
 // Adjoint of one step: state rows (q | qd) before the step, aq/aqd (B, nv)
 // hold the adjoint of the post-step state on entry and of the pre-step state
 // on exit; gtau_t rows (at tau_stride) receive dL/dtau_t.
 template <class T>
-hipError_t launch_rollout_step_vjp(const ModelTable<T> *tab, int nj, int64_t B, T dt, const T *state,
-                                   int64_t state_stride, const T *tau, int64_t tau_stride, T *aq, T *aqd, T *gtau,
-                                   hipStream_t s);
+hipError_t launch_rollout_step_vjp(const ModelTable<T> *tab, const ModelTable<double> *tab64, int nj, int64_t B,
+                                   T dt, const T *state, int64_t state_stride, const T *tau, int64_t tau_stride,
+                                   T *aq, T *aqd, T *gtau, hipStream_t s);
+// Batched forward dynamics in FP64 (CRBA + RNEA bias + LTDL), FP32 in and out.
+hipError_t launch_fd_ldlt(const ModelTable<double> *tab64, int nj, int64_t B, const float *q, const float *qd,
+                          const float *tau, float *qdd, hipStream_t s);
 // rows of (q | qd) at stride -> q, qd (B, nv)
 hipError_t launch_restore_state(int64_t B, int nv, const float *src, int64_t stride, float *q, float *qd,
                                 hipStream_t s);

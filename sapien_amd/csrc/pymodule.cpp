@@ -528,6 +528,73 @@ class PyModel {
     return out;
   }
 
+  // Heterogeneous CLIK: several models' problem batches in one persistent launch.
+  static py::list clik_hetero_batched(py::list models, py::list links, py::list poses, py::object q0s,
+                                      py::object masks, double eps, int max_iterations, double dt, double damp,
+                                      py::object stream) {
+    const size_t n = models.size();
+    if (links.size() != n || poses.size() != n) throw py::value_error("one link and one pose batch per model");
+    std::vector<std::unique_ptr<Batch>> batches;
+    std::vector<py::object> keep;
+    std::vector<std::unique_ptr<DLIn>> views;
+    std::vector<const sapamd_model *> ms;
+    std::vector<int64_t> Bs;
+    std::vector<int32_t> lk;
+    std::vector<const float *> tg, q0;
+    std::vector<const int32_t *> mk;
+    std::vector<float *> Q, E;
+    std::vector<int32_t *> Sx, It;
+    bool any_q0 = false, any_mask = false;
+    py::list out;
+    void *s = nullptr;
+    for (size_t i = 0; i < n; ++i) {
+      PyModel &pm = models[i].cast<PyModel &>();
+      batches.push_back(std::make_unique<Batch>(pm, poses[i], 7));
+      Batch &b = *batches.back();
+      if (i == 0) s = b.stream(stream);
+      ms.push_back(pm.raw());
+      Bs.push_back(b.B);
+      lk.push_back((int32_t)clamp_index(links[i].cast<int64_t>()));
+      keep.push_back(b.q);
+      views.push_back(b.view(b.q));
+      tg.push_back(f(views.back()));
+      py::object q0i = q0s.is_none() ? py::none() : py::object(q0s.cast<py::list>()[i]);
+      if (!q0i.is_none()) {
+        any_q0 = true;
+        py::object t = b.input(q0i);
+        keep.push_back(t);
+        views.push_back(b.view(t));
+        q0.push_back(f(views.back()));
+      } else {
+        q0.push_back(nullptr);
+      }
+      py::object mi = masks.is_none() ? py::none() : py::object(masks.cast<py::list>()[i]);
+      if (!mi.is_none()) {
+        any_mask = true;
+        py::object t = b.input_i32(mi, b.n);
+        keep.push_back(t);
+        views.push_back(take(t, dlabi::kInt, {b.n}, b.device));
+        mk.push_back(static_cast<const int32_t *>(views.back()->ptr));
+      } else {
+        mk.push_back(nullptr);
+      }
+      py::object Qt = b.empty({b.B, b.n}), Et = b.empty({b.B, 6}), St = b.empty_i32({b.B}), It_ = b.empty_i32({b.B});
+      views.push_back(b.view(Qt));
+      Q.push_back(f(views.back()));
+      views.push_back(b.view(Et));
+      E.push_back(f(views.back()));
+      views.push_back(take(St, dlabi::kInt, {b.B}, b.device));
+      Sx.push_back(static_cast<int32_t *>(views.back()->ptr));
+      views.push_back(take(It_, dlabi::kInt, {b.B}, b.device));
+      It.push_back(static_cast<int32_t *>(views.back()->ptr));
+      out.append(py::make_tuple(Qt, St.attr("bool")(), Et, It_));
+    }
+    check(sapamd_clik_hetero_batched(ms.data(), (int32_t)n, Bs.data(), lk.data(), tg.data(),
+                                     any_q0 ? q0.data() : nullptr, any_mask ? mk.data() : nullptr, eps,
+                                     max_iterations, dt, damp, Q.data(), Sx.data(), E.data(), It.data(), s));
+    return out;
+  }
+
  private:
   sapamd_model *m_ = nullptr;
   std::mt19937_64 rng_{std::random_device{}()};
@@ -609,6 +676,13 @@ PYBIND11_MODULE(_pinocchio, m) {
         py::arg("qacc") = py::none(), py::arg("stream") = py::none(),
         "Heterogeneous batch of several PinocchioModels (trees of <= 32 joints) in ONE kernel launch: per model "
         "a dict of link poses (B, L, 7), WORLD Jacobian (B, 6, nv), RNEA torques (B, nv) and mass matrix (B, nv, nv).");
+
+  m.def("clik_hetero_batched", &PyModel::clik_hetero_batched, py::arg("models"), py::arg("link_indices"),
+        py::arg("poses"), py::arg("initial_qpos") = py::none(), py::arg("active_qmask") = py::none(),
+        py::arg("eps") = 1e-4, py::arg("max_iterations") = 1000, py::arg("dt") = 0.1, py::arg("damp") = 1e-6,
+        py::arg("stream") = py::none(),
+        "CLIK for several PinocchioModels in ONE persistent launch: per model (qpos (B, nv), success (B,), "
+        "error (B, 6), iterations (B,)).");
 
   py::class_<Pose>(m, "Pose")
       .def(py::init([](std::vector<float> p, std::vector<float> q) {

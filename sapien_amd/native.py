@@ -21,7 +21,7 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise ImportError(f"{LIB_PATH} is not built; run `python -m sapien_amd.build`")
+            raise ImportError(f"{LIB_PATH} is not built; run `python sapien_amd/build.py`")
         L = ctypes.CDLL(LIB_PATH)
         P, I64, I32, D = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double
         L.sapamd_last_error.restype = ctypes.c_char_p

@@ -354,3 +354,29 @@ def test_heterogeneous_batch_one_launch():
         assert rel_err(out["J"].cpu().numpy(), J) <= REL_TOL
         assert rel_err(out["tau"].cpu().numpy(), T) <= REL_TOL
         assert rel_err(out["M"].cpu().numpy(), M) <= REL_TOL
+
+
+def test_heterogeneous_clik_one_launch():
+    """CLIK problems of the four trees (4:3:2:1) in ONE persistent launch: the
+    same per-problem results as each model's own batched CLIK, and converged
+    problems reach their targets."""
+    import sapien_amd as sa
+
+    models, links, tgts, refs = [], [], [], []
+    for i, (tree, nprob) in enumerate(zip(["chain7", "arm9", "quad18", "torso30"], [800, 600, 400, 200])):
+        rob = S.robot(tree)
+        m, o = _pair(rob)
+        qs = S.samples(rob, nprob, 1700 + i)[0]
+        t = o.link_poses(qs)[:, rob.ee_link]
+        models.append((m, o, rob))
+        links.append(rob.ee_link)
+        tgts.append(_t(t))
+    outs = sa.clik_hetero_batched([x[0] for x in models], links, tgts, max_iterations=300)
+    for (m, o, rob), link, t, (Q, ok, err, it) in zip(models, links, tgts, outs):
+        Qr, okr, errr, itr = m.compute_inverse_kinematics_batched(link, t, max_iterations=300)
+        assert torch.equal(ok, okr) and torch.equal(it, itr), rob.name
+        assert torch.allclose(Q, Qr, atol=1e-5) and torch.allclose(err, errr, atol=1e-6), rob.name
+        okn = ok.cpu().numpy()
+        assert okn.mean() > 0.3, (rob.name, okn.mean())
+        reached = o.link_poses(Q.cpu().numpy()[okn])[:, link, :3]
+        assert np.abs(reached - t.cpu().numpy()[okn, :3]).max() < 2e-4

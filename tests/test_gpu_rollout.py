@@ -166,3 +166,79 @@ def test_rollout_errors():
     qT, qdT, ck = m.rollout_batched(torch.zeros(4, 7, device=DEV), torch.ones(4, 7, device=DEV),
                                     torch.zeros(4, 0, 7, device=DEV), DT)
     assert torch.equal(qT, torch.zeros(4, 7, device=DEV)) and tuple(ck.shape) == (4, 0, 14)
+
+
+def _tiny_tip(rob):
+    """The same chain with a 1e-5 kg tip link whose inertia is 1e-10 kg m^2:
+    cond(M) far above 1e6."""
+    import re
+
+    u = rob.urdf
+    i = u.rfind("<inertial>")
+    head, tail = u[:i], u[i:]
+    tail = re.sub(r'<mass value="[^"]*"/>', '<mass value="1e-05"/>', tail, count=1)
+    tail = re.sub(r'(ixx|iyy|izz)="[^"]*"', r'\1="1e-10"', tail, count=3)
+    return head + tail
+
+
+def _f32(x):
+    return np.asarray(x, np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("tree", ["quad18", "torso30"])
+def test_fp64_factorized_forward_dynamics(tree):
+    """Forced FP64 (CRBA + LTDL): matches the FP64 oracle on the same
+    FP32-rounded inputs to output rounding; the default stays FP32 ABA."""
+    import sapien_amd as sa
+
+    rob = S.robot(tree)
+    m, o = _pair(rob)
+    prec, cond = sa.dynamics_precision(m)
+    assert prec == "fp32" and 1.0 <= cond < 1e6
+    q, qd, _, tau = (_f32(x) for x in S.samples(rob, 512, 1505))
+    ref = o.aba(q, qd, tau)
+    a32 = m.compute_forward_dynamics_batched(_t(q), _t(qd), _t(tau)).cpu().numpy()
+    sa.set_dynamics_precision(m, "fp64")
+    assert sa.dynamics_precision(m)[0] == "fp64"
+    a64 = m.compute_forward_dynamics_batched(_t(q), _t(qd), _t(tau)).cpu().numpy()
+    e32, e64 = rel_err(a32, ref), rel_err(a64, ref)
+    print(f"{tree}: cond {cond:.3g}  FP32 ABA {e32:.2e}  FP64 LTDL {e64:.2e}")
+    assert e64 <= 1e-6 and e32 <= 1e-3
+
+
+def test_ill_conditioned_tree_switches_to_fp64():
+    import sapien_amd as sa
+
+    rob = S.chain7()
+    urdf = _tiny_tip(rob)
+    m = sa.PinocchioModel(urdf, [0.0, 0.0, -9.81])
+    m.set_joint_order(rob.joint_names)
+    m.set_link_order(rob.link_names)
+    o = OracleModel(urdf)
+    o.set_joint_order(rob.joint_names)
+    o.set_link_order(rob.link_names)
+    prec, cond = sa.dynamics_precision(m)
+    assert prec == "fp64" and cond > 1e6, cond
+    q, qd, _, tau = (_f32(x) for x in S.samples(rob, 256, 1506))
+    tau[:, -1] *= 1e-6  # keep the tip's accelerations in range
+    a = m.compute_forward_dynamics_batched(_t(q), _t(qd), _t(tau)).cpu().numpy()
+    assert rel_err(a, o.aba(q, qd, tau)) <= 1e-6
+    # rollouts and their adjoint run through the same factorization
+    B, T = 4, 12
+    q0, qd0 = q[:B], qd[:B]
+    taus = np.random.default_rng(9).normal(size=(B, T, rob.nv))
+    taus[:, :, -1] *= 1e-6
+    qT, qdT, ck = m.rollout_batched(_t(q0), _t(qd0), _t(taus), DT, 5)
+    rq, rqd = o.rollout(q0, qd0, _f32(taus), DT)
+    assert rel_err(qT.cpu().numpy(), rq) <= 1e-4 and rel_err(qdT.cpu().numpy(), rqd) <= 1e-3
+    rng = np.random.default_rng(10)
+    gq, gqd = rng.normal(size=(B, rob.nv)), rng.normal(size=(B, rob.nv))
+    aq, aqd, atau = (x.cpu().numpy().astype(np.float64)
+                     for x in m.rollout_backward_batched(_t(taus), ck, DT, _t(gq), _t(gqd), 5))
+    for b in range(B):
+        vq, vqd, vt = rng.normal(size=rob.nv), rng.normal(size=rob.nv), rng.normal(size=(T, rob.nv))
+        jq, jqd = o.rollout_jvp(q0[b], qd0[b], _f32(taus[b]), vq, vqd, vt, DT)
+        lhs = gq[b] @ jq + gqd[b] @ jqd
+        rhs = aq[b] @ vq + aqd[b] @ vqd + (atau[b] * vt).sum()
+        scale = np.linalg.norm(np.concatenate([gq[b], gqd[b]])) * np.linalg.norm(np.concatenate([jq, jqd]))
+        assert abs(lhs - rhs) <= 1e-3 * max(scale, 1e-12), (b, lhs, rhs)
