@@ -39,16 +39,20 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", default="auto", help="auto | generic | fast (kernel variant of the fused path)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
+    ap.add_argument("--no-workloads", action="store_true", help="skip the mixed-tree CLIK / rollout / hetero sections")
     ap.add_argument("--probe-traffic", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--probe-shape", type=int, nargs=2, default=None, help=argparse.SUPPRESS)
     return ap.parse_args()
 
 
 COPY_BYTES = 256 << 20  # calibration copy: 256 MiB read + 256 MiB written
 
 
-def probe_traffic():
-    """Child of a rocprofv3 --pmc pass: 10 launches of the fused kernel on one
-    buffer set, then 5 device copies of 256 MiB for the counter calibration."""
+def probe_traffic(shape=None):
+    """Child of a rocprofv3 --pmc pass: 16 launches of the fused kernel (the
+    lanes-per-joint shape the timed run used) over 16 buffer sets, then 5
+    elementwise passes over 256 MiB (read 256 MiB, write 256 MiB) for the
+    counter calibration."""
     import numpy as np
     import torch
 
@@ -61,27 +65,33 @@ def probe_traffic():
     m = sa.PinocchioModel(rob.urdf, [0.0, 0.0, -9.81])
     m.set_joint_order(rob.joint_names)
     m.set_link_order(rob.link_names)
-    if sa.kernel_variant(m) == "lanes_per_joint":
-        tune.autotune(m, batch_sizes=(B,), use_cache=True)
-    q, qd, qdd, _ = (torch.from_numpy(x).to(dev) for x in S.samples(rob, B, 1001, dtype=np.float32))
-    P, J, T = (torch.empty(B, rob.nlinks, 7, device=dev), torch.empty(B, 6, rob.nv, device=dev),
-               torch.empty(B, rob.nv, device=dev))
+    if shape is not None and sa.kernel_variant(m) == "lanes_per_joint":
+        tune.set_shape(m, B, shape[0], shape[1])
+    # 16 rotating buffer sets (132 MB, beyond the 32 MB of L2) so that each
+    # launch's inputs come from outside L2, as in the timed run
+    sets = []
+    for k in range(16):
+        q, qd, qdd, _ = (torch.from_numpy(x).to(dev) for x in S.samples(rob, B, 1001 + k, dtype=np.float32))
+        sets.append((q, qd, qdd, torch.empty(B, rob.nlinks, 7, device=dev), torch.empty(B, 6, rob.nv, device=dev),
+                     torch.empty(B, rob.nv, device=dev)))
     launch = sa.native_fk_jac_rnea(m)
     stream = torch.cuda.current_stream(dev)
-    for _ in range(10):
-        launch(B, q, qd, qdd, P, J, T, stream.cuda_stream)
+    for k in range(16):
+        launch(B, *sets[k], stream.cuda_stream)
     src = torch.ones(COPY_BYTES // 4, device=dev)
     dst = torch.empty_like(src)
     for _ in range(5):
-        dst.copy_(src)
+        torch.neg(src, out=dst)  # an elementwise kernel: reads 256 MiB, writes 256 MiB
     torch.cuda.synchronize()
 
 
-def measure_traffic(timeout=120):
+def measure_traffic(shape=None, timeout=120):
     """HBM bytes per launch of the fused kernel from two rocprofv3 --pmc passes
     (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by the factor
-    that makes the same counter read 256 MiB for the 256 MiB copy kernel.
-    Runs before this process touches the GPU; returns None if rocprofv3 fails."""
+    that makes the same counter read 256 MiB for an elementwise kernel that
+    reads and writes 256 MiB.
+    Each pass is a child process (rocprofv3 -> python3 bench.py
+    --probe-traffic); returns None if rocprofv3 is missing or fails."""
     import csv
     import glob
     import shutil
@@ -96,6 +106,8 @@ def measure_traffic(timeout=120):
         d = tempfile.mkdtemp(prefix="sapamd_pmc_")
         cmd = ["rocprofv3", "--pmc", counter, "-d", d, "-o", "pmc", "--output-format", "csv", "--",
                sys.executable, os.path.abspath(__file__), "--probe-traffic"]
+        if shape is not None:
+            cmd += ["--probe-shape", str(shape[0]), str(shape[1])]
         p = subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, start_new_session=True)
         try:
             p.wait(timeout=timeout)
@@ -112,7 +124,7 @@ def measure_traffic(timeout=120):
                     name, v = row.get("Kernel_Name", ""), float(row.get("Counter_Value", "nan"))
                     if "k_chain_fused" in name or "k_kinematics" in name or "k_tree_fused" in name:
                         kern.append(v)
-                    elif "copy" in name.lower() and "rocclr" not in name.lower():
+                    elif "neg" in name:
                         copy.append(v)
         shutil.rmtree(d, ignore_errors=True)
         if not kern or not copy:
@@ -124,16 +136,110 @@ def measure_traffic(timeout=120):
     return res
 
 
+MIX = (("chain7", 4), ("arm9", 3), ("quad18", 2), ("torso30", 1))
+
+
+def _split(total):
+    """total split 4:3:2:1 over the four trees (rounded, summing to total)."""
+    w = [k for _, k in MIX]
+    n = [round(total * k / sum(w)) for k in w]
+    n[0] += total - sum(n)
+    return n
+
+
+def mixed_workloads(sa, S, np, torch, dev):
+    """The mixed batch of the four trees: FK+J+RNEA+CRBA of 8192 environments
+    in one launch, 65,536 CLIK problems in one persistent launch, and 4096 ABA
+    rollouts x 128 steps (dt = 1/240, checkpoint every 16) forward and backward."""
+    import time as _t
+
+    models, robs = [], []
+    for name, _ in MIX:
+        rob = S.robot(name)
+        m = sa.PinocchioModel(rob.urdf, [0.0, 0.0, -9.81])
+        m.set_joint_order(rob.joint_names)
+        m.set_link_order(rob.link_names)
+        models.append(m)
+        robs.append(rob)
+    res = {}
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = _t.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (_t.perf_counter() - t0) / reps
+
+    # heterogeneous FK + J + RNEA + CRBA, one launch
+    nb = _split(8192)
+    qs, qds, qdds = [], [], []
+    for i, (rob, b) in enumerate(zip(robs, nb)):
+        q, qd, qdd, _ = S.samples(rob, b, 1002 + i, dtype=np.float32)
+        qs.append(torch.from_numpy(q).to(dev))
+        qds.append(torch.from_numpy(qd).to(dev))
+        qdds.append(torch.from_numpy(qdd).to(dev))
+    sec = timed(lambda: sa.hetero_batched(models, qs, qds, qdds), 50)
+    res["hetero_fk_jac_rnea_crba"] = {"envs": nb, "evals_s": 8192 / sec, "ms": sec * 1e3,
+                                      "note": "one launch, outputs allocated per call"}
+
+    # CLIK: 65,536 problems, targets = FK of random configurations, q0 = neutral
+    nc = _split(65536)
+    tg = []
+    for i, (m, rob, b) in enumerate(zip(models, robs, nc)):
+        q = torch.from_numpy(S.samples(rob, b, 1004 + i, dtype=np.float32)[0]).to(dev)
+        tg.append(m.compute_forward_kinematics_batched(q)[:, rob.ee_link].contiguous())
+    links = [rob.ee_link for rob in robs]
+    outs = sa.clik_hetero_batched(models, links, tg)
+    torch.cuda.synchronize()
+    t0 = _t.perf_counter()
+    outs = sa.clik_hetero_batched(models, links, tg)
+    torch.cuda.synchronize()
+    sec = _t.perf_counter() - t0
+    succ = float(sum(o[1].sum().item() for o in outs)) / 65536
+    iters = float(sum(o[3].sum().item() for o in outs)) / 65536
+    res["clik_mixed"] = {"problems": nc, "problems_s": 65536 / sec, "ms": sec * 1e3, "success_rate": succ,
+                         "mean_iterations": iters, "max_iterations": 1000, "kernel": "persistent work queue"}
+
+    # rollouts: 4096 split 4:3:2:1, 128 steps, tau ~ N(0, 5^2), forward and backward
+    nr = _split(4096)
+    T, K, dt = 128, 16, 1.0 / 240.0
+    per = []
+    for i, (m, rob, b) in enumerate(zip(models, robs, nr)):
+        q0, qd0, _, _ = S.samples(rob, b, 1005 + i, dtype=np.float32)
+        tau = np.random.default_rng(1005 + i).normal(scale=5.0, size=(b, T, rob.nv)).astype(np.float32)
+        per.append((m, torch.from_numpy(q0).to(dev), torch.from_numpy(qd0).to(dev), torch.from_numpy(tau).to(dev)))
+    ck = {}
+
+    def fwd():
+        for j, (m, q0, qd0, tau) in enumerate(per):
+            ck[j] = m.rollout_batched(q0, qd0, tau, dt, K)
+
+    def bwd():
+        for j, (m, q0, qd0, tau) in enumerate(per):
+            g = torch.ones_like(q0)
+            m.rollout_backward_batched(tau, ck[j][2], dt, g, g, K)
+
+    sf = timed(fwd, 3)
+    sb = timed(bwd, 3)
+    res["rollouts_mixed"] = {"rollouts": nr, "steps": T, "dt": dt, "checkpoint_every": K,
+                             "forward_env_steps_s": 4096 * T / sf, "forward_ms": sf * 1e3,
+                             "backward_env_steps_s": 4096 * T / sb, "backward_ms": sb * 1e3,
+                             "precision": [sa.dynamics_precision(m)[0] for m in models],
+                             "kernel": "thread per env, one launch per step and tree"}
+    return res
+
+
 def main():
     args = parse()
     if args.probe_traffic:
-        probe_traffic()
+        probe_traffic(args.probe_shape)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X "
               "(the environments shard trivially across GPUs; no multi-GPU run is made).", file=sys.stderr)
         sys.exit(2)
-    traffic = None if args.no_traffic else measure_traffic()
     import numpy as np
     import torch
 
@@ -191,6 +297,7 @@ def main():
     elapsed = min(elapsed, elapsed_plain)
     value = B * args.steps / elapsed
     achieved = BYTES_PER_EVAL * B / (kern_ms * 1e-3) / 1e9
+    traffic = None if args.no_traffic else measure_traffic(tuned[B][:2] if tuned else None)
 
     out = {
         "metric": METRIC,
@@ -218,8 +325,8 @@ def main():
         out["roofline"]["traffic_detail"] = {
             "fetch_bytes": traffic["FETCH_SIZE"]["bytes"], "write_bytes": traffic["WRITE_SIZE"]["bytes"],
             "fetch_scale": traffic["FETCH_SIZE"]["scale"], "write_scale": traffic["WRITE_SIZE"]["scale"],
-            "method": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch, median of 10 launches on one "
-                      "buffer set), scaled so a 256 MiB device copy reads 256 MiB"}
+            "method": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch, median of 16 launches over 16 "
+                      "buffer sets), scaled so an elementwise kernel over 256 MiB reads and writes 256 MiB"}
 
     # gradient throughput: 8192 environments of the 7-dof and 9-dof trees,
     # loss = |p_ee - FK(q*)|^2 + |tau_RNEA(q, qd, qdd) - tau*|^2. Forward-only
@@ -267,6 +374,9 @@ def main():
         res["kernel"] = sa.kernel_variant(gm)
         grads[gname] = res
     out["gradients"] = grads
+
+    if not args.no_workloads:
+        out["workloads"] = mixed_workloads(sa, S, np, torch, dev)
 
     # plain per-call drop-in path on the GPU (FP64, one configuration per call)
     q64, qd64, qdd64, _ = S.samples(rob, 256, 1001)

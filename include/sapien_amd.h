@@ -219,9 +219,10 @@ int sapamd_rollout_backward(const sapamd_model *m, int64_t B, int32_t T, int32_t
 /* ---------------------------------------------------------------- forward-dynamics precision
  * Batched forward dynamics (sapamd_aba_batched, rollouts) runs FP32 ABA unless
  * the model's mass matrix is ill-conditioned: at set_joint_order the library
- * estimates cond(M) (FP64 CRBA on 64 seeded configurations within the joint
- * limits) and, above 1e6, switches to FP64 CRBA + RNEA bias + LTDL
- * factorization (FP32 in and out). The reference computes ABA in FP64 always
+ * estimates cond(M) on the host (FP64 CRBA on 64 seeded configurations within
+ * the joint limits) and, above 1e4 (where FP32 ABA can no longer hold 1e-3
+ * relative), switches to FP64 CRBA + RNEA bias + LTDL factorization (FP32 in
+ * and out). The reference computes ABA in FP64 always
  * (pinocchio_model.cpp:221-226); this keeps its accuracy where FP32 cannot. */
 enum { SAPAMD_DYNAMICS_AUTO = 0, SAPAMD_DYNAMICS_FP32 = 1, SAPAMD_DYNAMICS_FP64 = 2 };
 int sapamd_set_dynamics_precision(sapamd_model *m, int32_t mode);

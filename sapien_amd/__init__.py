@@ -72,7 +72,7 @@ def dynamics_precision(model):
 
 
 def set_dynamics_precision(model, name):
-    """'auto' (FP64 factorization when cond(M) > 1e6), 'fp32' (ABA) or 'fp64'."""
+    """'auto' (FP64 factorization when cond(M) > 1e4), 'fp32' (ABA) or 'fp64'."""
     from .native import check, handle, lib
 
     if name not in _PRECISIONS:

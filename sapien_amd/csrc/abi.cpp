@@ -85,10 +85,13 @@ struct sapamd_model {
   std::vector<double> pose_cache, jw_cache, jl_cache;
   // Largest cond(M) over sampled configurations (FP64, at refresh) and the
   // precision policy of batched forward dynamics: FP32 ABA, or FP64
-  // CRBA + LTDL when the estimate exceeds kCondLimit.
+  // CRBA + LTDL when the estimate exceeds kCondLimit. FP32 ABA loses about
+  // cond(M) * 6e-8 relative (more on floating bases emulated by six 1-dof
+  // joints, where the articulated inertias cancel at the root), so above
+  // 1e4 it cannot hold a 1e-3 relative tolerance.
   double cond_est = 0;
   int dyn_precision = SAPAMD_DYNAMICS_AUTO;
-  static constexpr double kCondLimit = 1e6;
+  static constexpr double kCondLimit = 1e4;
   bool fd64() const {
     return dyn_precision == SAPAMD_DYNAMICS_FP64 || (dyn_precision == SAPAMD_DYNAMICS_AUTO && cond_est > kCondLimit);
   }
@@ -215,7 +218,9 @@ struct sapamd_model {
   // Rebuild device tables and per-call buffers after the orders change.
   void refresh() {
     tables_ready = false;
-    if (device < 0 || !host->supported() || !host->joint_order_set()) return;
+    if (!host->supported() || !host->joint_order_set()) return;
+    cond_est = estimate_condition();
+    if (device < 0) return;
     if (host->njoints() > kMaxGenericJoints)
       throw std::runtime_error("models with more than 64 movable joints are not supported");
     DeviceGuard g(device);
@@ -256,37 +261,104 @@ struct sapamd_model {
       buf_doubles = need;
     }
     tables_ready = true;
-    cond_est = estimate_condition();
   }
 
-  // cond(M) = lambda_max / lambda_min over 64 seeded configurations drawn
-  // uniformly within the joint limits (continuous joints in [-pi, pi]): M from
-  // the FP64 CRBA kernel, eigenvalues by cyclic Jacobi rotations on the host.
-  double estimate_condition() {
-    const int n = nv(), S = 64;
+  // cond(M) = lambda_max / lambda_min over 64 (16 above 32 dofs) seeded configurations drawn
+  // uniformly within the joint limits (continuous joints in [-pi, pi]). Host
+  // FP64 composite-rigid-body mass matrices (no GPU work at set-up), then
+  // eigenvalues by cyclic Jacobi rotations.
+  double estimate_condition() const {
+    const int n = nv(), S = n > 32 ? 16 : 64;
     if (n == 0) return 1.0;
-    std::vector<double> q((size_t)S * n), Mh((size_t)S * n * n);
+    const auto &J = host->joints();
+    const int nj = (int)J.size();
+    std::vector<double> Mh((size_t)S * n * n, 0.0);
     std::mt19937_64 rng(0x5eed);
     std::uniform_real_distribution<double> U(0.0, 1.0);
-    const auto &J = host->joints();
-    for (int b = 0; b < S; ++b)
-      for (size_t j = 0; j < J.size(); ++j) {
-        const int u = host->vuser()[j];
-        if (u < 0) continue;
-        const bool cont = J[j].kind == JK_CONTINUOUS;
-        const double lo = cont ? -M_PI : J[j].lower, hi = cont ? M_PI : J[j].upper;
-        q[(size_t)b * n + u] = lo + (hi - lo) * U(rng);
+    std::vector<double> R(9 * nj), P(3 * nj), col(6 * nj), Ym(nj), Yh(3 * nj), YI(9 * nj);
+    for (int b = 0; b < S; ++b) {
+      for (int j = 0; j < nj; ++j) {
+        const Joint &jt = J[j];
+        const bool cont = jt.kind == JK_CONTINUOUS;
+        const double lo = cont ? -M_PI : jt.lower, hi = cont ? M_PI : jt.upper;
+        const double q = lo + (hi - lo) * U(rng);
+        // local transform: place * (rotation about / translation along axis)
+        double Rl[9], pl[3], K[9], K2[9];
+        skew2(jt.axis, K);
+        mm3(K, K, K2);
+        const double sq = jt.kind == JK_PRISMATIC ? 0.0 : std::sin(q), cq = jt.kind == JK_PRISMATIC ? 1.0 : std::cos(q);
+        double Rq[9];
+        for (int k = 0; k < 9; ++k) Rq[k] = (k % 4 == 0 ? 1.0 : 0.0) + sq * K[k] + (1 - cq) * K2[k];
+        mm3(jt.place.R, Rq, Rl);
+        for (int i = 0; i < 3; ++i) {
+          pl[i] = jt.place.p[i];
+          if (jt.kind == JK_PRISMATIC)
+            for (int k = 0; k < 3; ++k) pl[i] += q * jt.place.R[3 * i + k] * jt.axis[k];
+        }
+        double *Rj = &R[9 * j], *pj = &P[3 * j];
+        if (jt.parent < 0) {
+          std::copy(Rl, Rl + 9, Rj);
+          std::copy(pl, pl + 3, pj);
+        } else {
+          const double *Rp = &R[9 * jt.parent], *pp = &P[3 * jt.parent];
+          mm3(Rp, Rl, Rj);
+          for (int i = 0; i < 3; ++i) pj[i] = pp[i] + Rp[3 * i] * pl[0] + Rp[3 * i + 1] * pl[1] + Rp[3 * i + 2] * pl[2];
+        }
+        double a[3];
+        for (int i = 0; i < 3; ++i) a[i] = Rj[3 * i] * jt.axis[0] + Rj[3 * i + 1] * jt.axis[1] + Rj[3 * i + 2] * jt.axis[2];
+        double *c6 = &col[6 * j];
+        if (jt.kind == JK_PRISMATIC) {
+          c6[0] = a[0]; c6[1] = a[1]; c6[2] = a[2]; c6[3] = c6[4] = c6[5] = 0;
+        } else {
+          c6[0] = pj[1] * a[2] - pj[2] * a[1];
+          c6[1] = pj[2] * a[0] - pj[0] * a[2];
+          c6[2] = pj[0] * a[1] - pj[1] * a[0];
+          c6[3] = a[0]; c6[4] = a[1]; c6[5] = a[2];
+        }
+        // inertia about the world origin: m, h = m c_w, I_O = R Ic R^T + m(|c|^2 1 - c c^T)
+        double m, c[3], Ic[9], cw[3], T1[9], Iw[9];
+        jt.inertia.com_form(m, c, Ic);
+        for (int i = 0; i < 3; ++i) cw[i] = pj[i] + Rj[3 * i] * c[0] + Rj[3 * i + 1] * c[1] + Rj[3 * i + 2] * c[2];
+        mm3(Rj, Ic, T1);
+        for (int i = 0; i < 3; ++i)
+          for (int k = 0; k < 3; ++k) Iw[3 * i + k] = T1[3 * i] * Rj[3 * k] + T1[3 * i + 1] * Rj[3 * k + 1] + T1[3 * i + 2] * Rj[3 * k + 2];
+        const double cc = cw[0] * cw[0] + cw[1] * cw[1] + cw[2] * cw[2];
+        Ym[j] = m;
+        for (int i = 0; i < 3; ++i) {
+          Yh[3 * j + i] = m * cw[i];
+          for (int k = 0; k < 3; ++k) YI[9 * j + 3 * i + k] = Iw[3 * i + k] + m * ((i == k ? cc : 0.0) - cw[i] * cw[k]);
+        }
       }
-    double *dq = nullptr, *dM = nullptr;
-    HIP_OK(hipMalloc(&dq, q.size() * sizeof(double)));
-    HIP_OK(hipMalloc(&dM, Mh.size() * sizeof(double)));
-    hipError_t e = hipMemcpy(dq, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = launch_crba<double>(t64(), host->njoints(), S, dq, dM, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    if (e == hipSuccess) e = hipMemcpy(Mh.data(), dM, Mh.size() * sizeof(double), hipMemcpyDeviceToHost);
-    (void)hipFree(dq);
-    (void)hipFree(dM);
-    HIP_OK(e);
+      for (int j = nj - 1; j >= 0; --j) {  // composite inertias (children follow parents)
+        const int p = J[j].parent;
+        if (p < 0) continue;
+        Ym[p] += Ym[j];
+        for (int i = 0; i < 3; ++i) Yh[3 * p + i] += Yh[3 * j + i];
+        for (int i = 0; i < 9; ++i) YI[9 * p + i] += YI[9 * j + i];
+      }
+      double *Mb = &Mh[(size_t)b * n * n];
+      for (int j = 0; j < nj; ++j) {
+        const double *l = &col[6 * j], *w = l + 3, *h = &Yh[3 * j], *I = &YI[9 * j];
+        // f = Yc_j col_j: f_l = m l + w x h, f_a = h x l + I_O w
+        double f[6];
+        f[0] = Ym[j] * l[0] + w[1] * h[2] - w[2] * h[1];
+        f[1] = Ym[j] * l[1] + w[2] * h[0] - w[0] * h[2];
+        f[2] = Ym[j] * l[2] + w[0] * h[1] - w[1] * h[0];
+        for (int i = 0; i < 3; ++i) f[3 + i] = I[3 * i] * w[0] + I[3 * i + 1] * w[1] + I[3 * i + 2] * w[2];
+        f[3] += h[1] * l[2] - h[2] * l[1];
+        f[4] += h[2] * l[0] - h[0] * l[2];
+        f[5] += h[0] * l[1] - h[1] * l[0];
+        const int uj = host->vuser()[j];
+        for (int i = j; i >= 0; i = J[i].parent) {
+          const double *ci = &col[6 * i];
+          double v = 0;
+          for (int k = 0; k < 6; ++k) v += ci[k] * f[k];
+          const int ui = host->vuser()[i];
+          Mb[ui * n + uj] = v;
+          Mb[uj * n + ui] = v;
+        }
+      }
+    }
     double worst = 1.0;
     std::vector<double> A((size_t)n * n);
     for (int b = 0; b < S; ++b) {

@@ -122,3 +122,37 @@ def test_pose_type_matches_sapien_pose_kats():
     import pickle
     c = pickle.loads(pickle.dumps(a))
     assert np.allclose(c.p, a.p) and np.allclose(c.q, a.q)
+
+
+def test_condition_estimate_matches_oracle_mass_matrices():
+    """The host cond(M) estimate that picks FP32 ABA or FP64 factorization:
+    within the range the oracle's CRBA gives over random configurations, and
+    far above the 1e4 switch for a chain with a 1e-5 kg tip."""
+    import re
+
+    import sapien_amd as sa
+    from oracle.urdf_model import OracleModel
+
+    for name in ("chain7", "quad18"):
+        rob = S.robot(name)
+        m = sa.PinocchioModel(rob.urdf, [0.0, 0.0, -9.81])
+        m.set_joint_order(rob.joint_names)
+        prec, cond = sa.dynamics_precision(m)
+        o = OracleModel(rob.urdf)
+        o.set_joint_order(rob.joint_names)
+        ev = np.linalg.eigvalsh(o.crba(S.samples(rob, 512, 3)[0]))
+        c = ev[:, -1] / ev[:, 0]
+        assert np.median(c) <= cond <= 10 * c.max(), (name, cond, c.max())
+        assert prec == ("fp64" if name == "quad18" else "fp32")  # switch above cond 1e4
+    rob = S.chain7()
+    i = rob.urdf.rfind("<inertial>")
+    tail = re.sub(r'<mass value="[^"]*"/>', '<mass value="1e-05"/>', rob.urdf[i:], count=1)
+    tail = re.sub(r'(ixx|iyy|izz)="[^"]*"', r'\1="1e-10"', tail, count=3)
+    m = sa.PinocchioModel(rob.urdf[:i] + tail, [0.0, 0.0, -9.81])
+    m.set_joint_order(rob.joint_names)
+    prec, cond = sa.dynamics_precision(m)
+    assert prec == "fp64" and cond > 1e6
+    sa.set_dynamics_precision(m, "fp32")
+    assert sa.dynamics_precision(m)[0] == "fp32"
+    with pytest.raises(ValueError):
+        sa.set_dynamics_precision(m, "fp16")
