@@ -4,11 +4,14 @@ B = 16384 environments of the seeded 7-dof chain, FP32, one MI355X.
 One step = one launch of the fused kernel over one batch of 16384 environments,
 inputs already resident in HBM. 64 distinct input/output buffer sets (~530 MB)
 are rotated so the working set exceeds the 256 MB Infinity Cache and the
-measured bandwidth is HBM bandwidth. Prints ONE JSON line.
+measured bandwidth is HBM bandwidth. Steps run either eagerly (one C-ABI call
+per step) or as replays of a HIP graph holding the 64 launches; `value` is the
+faster of the two. Prints ONE JSON line.
 
 Beside the headline number it reports:
-  roofline      algorithmic bytes per launch / mean kernel time (HIP events on
-                the launch stream, per launch), against 8 TB/s HBM3E;
+  roofline      algorithmic bytes per launch / mean launch duration (HIP events
+                on the launch stream over the timed graph replays), against
+                8 TB/s HBM3E; traffic from rocprofv3 FETCH_SIZE + WRITE_SIZE;
   cpu_baseline  the FP64 C++ oracle (same algorithms) timed on this host's cores
                 on a bounded sample;
   plain_gpu     the drop-in per-call path (one configuration per call, FP64 on
@@ -276,7 +279,8 @@ def main():
     for i in range(args.warmup):
         step(i)
     torch.cuda.synchronize()
-    # per-launch kernel durations with events on the launch stream
+    # (1) eager: one C-ABI launch per step from Python, with and without
+    # per-launch events in the stream
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -285,20 +289,59 @@ def main():
         step(i)
         ev[i][1].record(stream)
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
-    # wall-clock throughput without per-launch events in the stream
-    torch.cuda.synchronize()
+    eager_ms_events = float(np.mean([a.elapsed_time(b) for a, b in ev]))
     t0 = time.perf_counter()
     for i in range(args.steps):
         step(i)
     torch.cuda.synchronize()
-    elapsed_plain = time.perf_counter() - t0
-    elapsed = min(elapsed, elapsed_plain)
+    elapsed_eager = time.perf_counter() - t0
+
+    # (2) graph: the 64 launches (one per buffer set) captured once in a HIP
+    # graph; K steps = K // 64 replays + K % 64 eager launches, bracketed by
+    # events on the launch stream. Host launch cost leaves the loop.
+    ref0 = [t.clone() for t in (sets[0]["P"], sets[0]["J"], sets[0]["T"])]
+    graph = torch.cuda.CUDAGraph()
+    cap = torch.cuda.Stream(dev)
+    cap.wait_stream(stream)
+    with torch.cuda.graph(graph, stream=cap):
+        cs = torch.cuda.current_stream(dev).cuda_stream
+        for k in range(NSETS):
+            sk = sets[k]
+            launch(B, sk["q"], sk["qd"], sk["qdd"], sk["P"], sk["J"], sk["T"], cs)
+    stream.wait_stream(cap)
+    for t in (sets[0]["P"], sets[0]["J"], sets[0]["T"]):
+        t.fill_(float("nan"))
+    graph.replay()
+    torch.cuda.synchronize()
+    graph_ok = all(torch.equal(a, b) for a, b in zip(ref0, (sets[0]["P"], sets[0]["J"], sets[0]["T"])))
+    full, rem = divmod(args.steps, NSETS)
+    for _ in range(2):
+        graph.replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(full):
+        graph.replay()
+    for i in range(rem):
+        step(i)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    elapsed_graph = time.perf_counter() - t0
+    graph_ms = e0.elapsed_time(e1) / args.steps
+
+    elapsed = min(elapsed_eager, elapsed_graph)
     value = B * args.steps / elapsed
+    # average launch duration on the launch stream over the timed region
+    kern_ms = graph_ms if graph_ok else eager_ms_events
     achieved = BYTES_PER_EVAL * B / (kern_ms * 1e-3) / 1e9
     traffic = None if args.no_traffic else measure_traffic(tuned[B][:2] if tuned else None)
 
+    launch_modes = {
+        "graph": {"ms_per_step": elapsed_graph / args.steps * 1e3, "events_ms_per_launch": graph_ms,
+                  "replay_matches_eager": graph_ok},
+        "eager": {"ms_per_step": elapsed_eager / args.steps * 1e3, "events_ms_per_launch": eager_ms_events},
+    }
     out = {
         "metric": METRIC,
         "value": value,
@@ -320,6 +363,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (traffic["FETCH_SIZE"]["bytes"] + traffic["WRITE_SIZE"]["bytes"]) if traffic else None,
                      "kernel_ms": kern_ms, "bytes_per_launch": BYTES_PER_EVAL * B},
+        "launch_modes": launch_modes,
     }
     if traffic:
         out["roofline"]["traffic_detail"] = {

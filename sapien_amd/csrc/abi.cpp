@@ -167,6 +167,7 @@ struct sapamd_model {
     hdr->root_link = -1;
     for (int j = 0; j < nj; ++j) {
       jr[j].link = -1;
+      jr[j].lident = 0;
       jr[j].diagI = (jr[j].Ic[1] == T(0) && jr[j].Ic[2] == T(0) && jr[j].Ic[4] == T(0)) ? 1 : 0;
     }
     for (int l = 0; l < nl; ++l) {
@@ -179,8 +180,16 @@ struct sapamd_model {
       } else {
         ++per_joint[f.joint];
         jr[f.joint].link = l;
-        for (int k = 0; k < 9; ++k) jr[f.joint].lR[k] = T(f.place.R[k]);
-        for (int k = 0; k < 3; ++k) jr[f.joint].lp[k] = T(f.place.p[k]);
+        bool id = true;
+        for (int k = 0; k < 9; ++k) {
+          jr[f.joint].lR[k] = T(f.place.R[k]);
+          if (f.place.R[k] != (k % 4 == 0 ? 1.0 : 0.0)) id = false;
+        }
+        for (int k = 0; k < 3; ++k) {
+          jr[f.joint].lp[k] = T(f.place.p[k]);
+          if (f.place.p[k] != 0.0) id = false;
+        }
+        jr[f.joint].lident = id ? 1 : 0;
       }
     }
     // pointer-jumping ancestors and depth-first subtree sizes

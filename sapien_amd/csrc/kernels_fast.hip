@@ -111,12 +111,17 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
     if (!valid[t]) X[t] = identity_xf();
     if (t > 0) X[t] = compose(X[t - 1], X[t]);
   }
+  // The inclusive scan is the lane's last joint; only the joints before it
+  // need the exclusive prefix (none when C = 1).
   if constexpr (G > 1) {
     Xf incl = X[C - 1];
     scan_xf<G>(g, incl);
-    const Xf E = excl_xf<G>(g, incl);
+    if constexpr (C > 1) {
+      const Xf E = excl_xf<G>(g, incl);
 #pragma unroll
-    for (int t = 0; t < C; ++t) X[t] = compose(E, X[t]);
+      for (int t = 0; t < C - 1; ++t) X[t] = compose(E, X[t]);
+    }
+    X[C - 1] = incl;
   }
   S6<float> col[C];
 #pragma unroll
@@ -130,9 +135,12 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
     if constexpr (G > 1) {
       S6<float> incl = v[C - 1];
       scan_sum<G>(g, incl);
-      const S6<float> E = excl_sum<G>(g, incl);
+      if constexpr (C > 1) {
+        const S6<float> E = excl_sum<G>(g, incl);
 #pragma unroll
-      for (int t = 0; t < C; ++t) v[t] = v[t] + E;
+        for (int t = 0; t < C - 1; ++t) v[t] = v[t] + E;
+      }
+      v[C - 1] = incl;
     }
 #pragma unroll
     for (int t = 0; t < C; ++t)
@@ -141,13 +149,15 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
     if constexpr (G > 1) {
       S6<float> incl = a[C - 1];
       scan_sum<G>(g, incl);
-      Ea = excl_sum<G>(g, incl);
+      if constexpr (C > 1) Ea = excl_sum<G>(g, incl);
+      a[C - 1] = incl;
     }
-    Ea.l = Ea.l - V3<float>{M->gravity[0], M->gravity[1], M->gravity[2]};
+    const V3<float> grav = {M->gravity[0], M->gravity[1], M->gravity[2]};
     S6<float> f[C];
 #pragma unroll
     for (int t = 0; t < C; ++t) {
-      const S6<float> at = a[t] + Ea;
+      S6<float> at = (G > 1 && t == C - 1) ? a[t] : a[t] + Ea;
+      at.l = at.l - grav;
       const WInertia<float> Y = world_inertia_fast(*J[t], X[t].R, X[t].p);
       f[t] = valid[t] ? inertia_apply(Y, at) + crossf(v[t], inertia_apply(Y, v[t])) : zero6<float>();
     }
@@ -156,9 +166,12 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
     if constexpr (G > 1) {
       S6<float> incl = f[0];
       scan_suffix<G>(g, incl);
-      const S6<float> E = excl_suffix<G>(g, incl);
+      if constexpr (C > 1) {
+        const S6<float> E = excl_suffix<G>(g, incl);
 #pragma unroll
-      for (int t = 0; t < C; ++t) f[t] = f[t] + E;
+        for (int t = 1; t < C; ++t) f[t] = f[t] + E;
+      }
+      f[0] = incl;
     }
 #pragma unroll
     for (int t = 0; t < C; ++t)
@@ -181,6 +194,10 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
 #pragma unroll
     for (int t = 0; t < C; ++t) {
       if (!valid[t] || J[t]->link < 0) continue;
+      if (J[t]->lident) {  // link frame = joint frame
+        store_pose(poses + (e * L + J[t]->link) * 7, X[t].R, X[t].p);
+        continue;
+      }
       float Rw[9];
       mm(X[t].R, J[t]->lR, Rw);
       store_pose(poses + (e * L + J[t]->link) * 7, Rw, mv(X[t].R, load3(J[t]->lp)) + X[t].p);

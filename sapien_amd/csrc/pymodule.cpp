@@ -546,6 +546,7 @@ class PyModel {
     std::vector<int32_t *> Sx, It;
     bool any_q0 = false, any_mask = false;
     py::list out;
+    std::vector<std::array<py::object, 4>> res;
     void *s = nullptr;
     for (size_t i = 0; i < n; ++i) {
       PyModel &pm = models[i].cast<PyModel &>();
@@ -587,11 +588,13 @@ class PyModel {
       Sx.push_back(static_cast<int32_t *>(views.back()->ptr));
       views.push_back(take(It_, dlabi::kInt, {b.B}, b.device));
       It.push_back(static_cast<int32_t *>(views.back()->ptr));
-      out.append(py::make_tuple(Qt, St.attr("bool")(), Et, It_));
+      res.push_back({Qt, St, Et, It_});
     }
     check(sapamd_clik_hetero_batched(ms.data(), (int32_t)n, Bs.data(), lk.data(), tg.data(),
                                      any_q0 ? q0.data() : nullptr, any_mask ? mk.data() : nullptr, eps,
                                      max_iterations, dt, damp, Q.data(), Sx.data(), E.data(), It.data(), s));
+    // the int32 -> bool conversion is stream-ordered after the launch
+    for (auto &r : res) out.append(py::make_tuple(r[0], r[1].attr("bool")(), r[2], r[3]));
     return out;
   }
 

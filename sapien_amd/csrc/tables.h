@@ -36,7 +36,7 @@ template <class T> struct alignas(16) JointRec {
   int32_t nchild;
   int32_t link;   // the one user link attached to this joint, -1 = none (see links_le1)
   int32_t diagI;  // 1 when Ic is diagonal
-  int32_t pad;
+  int32_t lident; // 1 when the attached link's frame is the joint frame (lR = I, lp = 0)
   int32_t anc[5]; // 2^k-th ancestor (k = 0..4), -1 past the root
   int32_t subtree;  // joints in the subtree (the subtree is [j, j + subtree) in depth-first order)
   int32_t pad2[2];
