@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void k_chain_vjp(const ModelTable<float> *__re
 #pragma unroll
   for (int t = 0; t < C; ++t) {
     float s, c;
-    sincosf(qj[t], &s, &c);
+    joint_sincos(qj[t], &s, &c);
     joint_world(*J[t], qj[t], s, c, static_cast<const float *>(nullptr), V3<float>{0, 0, 0}, true, X[t].R, X[t].p);
     if (!valid[t]) X[t] = identity_xf();
     if (t > 0) X[t] = compose(X[t - 1], X[t]);

@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
 #pragma unroll
   for (int t = 0; t < C; ++t) {
     float s, c;
-    sincosf(qj[t], &s, &c);
+    joint_sincos(qj[t], &s, &c);
     joint_world(*J[t], qj[t], s, c, static_cast<const float *>(nullptr), V3<float>{0, 0, 0}, true, X[t].R, X[t].p);
     if (!valid[t]) X[t] = identity_xf();
     if (t > 0) X[t] = compose(X[t - 1], X[t]);
@@ -127,108 +127,113 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
 #pragma unroll
   for (int t = 0; t < C; ++t) col[t] = valid[t] ? jcol(*J[t], X[t].R, X[t].p) : zero6<float>();
 
-  // ---- RNEA
-  if (dyn) {
-    S6<float> v[C], a[C];
-#pragma unroll
-    for (int t = 0; t < C; ++t) v[t] = (t > 0 ? v[t - 1] : zero6<float>()) + qdj[t] * col[t];
-    if constexpr (G > 1) {
-      S6<float> incl = v[C - 1];
-      scan_sum<G>(g, incl);
-      if constexpr (C > 1) {
-        const S6<float> E = excl_sum<G>(g, incl);
-#pragma unroll
-        for (int t = 0; t < C - 1; ++t) v[t] = v[t] + E;
-      }
-      v[C - 1] = incl;
-    }
-#pragma unroll
-    for (int t = 0; t < C; ++t)
-      a[t] = (t > 0 ? a[t - 1] : zero6<float>()) + qddj[t] * col[t] + qdj[t] * crossm(v[t], col[t]);
-    S6<float> Ea = zero6<float>();
-    if constexpr (G > 1) {
-      S6<float> incl = a[C - 1];
-      scan_sum<G>(g, incl);
-      if constexpr (C > 1) Ea = excl_sum<G>(g, incl);
-      a[C - 1] = incl;
-    }
-    const V3<float> grav = {M->gravity[0], M->gravity[1], M->gravity[2]};
-    S6<float> f[C];
-#pragma unroll
-    for (int t = 0; t < C; ++t) {
-      S6<float> at = (G > 1 && t == C - 1) ? a[t] : a[t] + Ea;
-      at.l = at.l - grav;
-      const WInertia<float> Y = world_inertia_fast(*J[t], X[t].R, X[t].p);
-      f[t] = valid[t] ? inertia_apply(Y, at) + crossf(v[t], inertia_apply(Y, v[t])) : zero6<float>();
-    }
-#pragma unroll
-    for (int t = C - 2; t >= 0; --t) f[t] = f[t] + f[t + 1];  // suffix sums inside the lane
-    if constexpr (G > 1) {
-      S6<float> incl = f[0];
-      scan_suffix<G>(g, incl);
-      if constexpr (C > 1) {
-        const S6<float> E = excl_suffix<G>(g, incl);
-#pragma unroll
-        for (int t = 1; t < C; ++t) f[t] = f[t] + E;
-      }
-      f[0] = incl;
-    }
-#pragma unroll
-    for (int t = 0; t < C; ++t)
-      if (live && valid[t]) tau[e * nv + J[t]->vuser] = dot6(col[t], f[t]);
-  }
-  if ((flags & KF_JFULL) && live) {
+  // ---- Jacobian columns and link poses, emitted before the dynamics so that
+  // their stores drain while RNEA computes
+  if (flags & KF_JFULL) {
 #pragma unroll
     for (int t = 0; t < C; ++t) {
       if (!valid[t]) continue;
-      float *Jb = Jout + e * 6 * nv + J[t]->vuser;
-      Jb[0 * nv] = col[t].l.x; Jb[1 * nv] = col[t].l.y; Jb[2 * nv] = col[t].l.z;
-      Jb[3 * nv] = col[t].a.x; Jb[4 * nv] = col[t].a.y; Jb[5 * nv] = col[t].a.z;
+      if (live) {
+        float *Jb = Jout + e * 6 * nv + J[t]->vuser;
+        Jb[0 * nv] = col[t].l.x; Jb[1 * nv] = col[t].l.y; Jb[2 * nv] = col[t].l.z;
+        Jb[3 * nv] = col[t].a.x; Jb[4 * nv] = col[t].a.y; Jb[5 * nv] = col[t].a.z;
+      }
     }
   }
-  // ---- link poses
-  if (!(flags & KF_POSES)) return;
-  if (M->links_le1) {
+  if ((flags & KF_POSES) && M->links_le1) {
     // each joint's lane writes the link attached to it; lane 0 writes the root link
-    if (!live) return;
 #pragma unroll
     for (int t = 0; t < C; ++t) {
       if (!valid[t] || J[t]->link < 0) continue;
-      if (J[t]->lident) {  // link frame = joint frame
-        store_pose(poses + (e * L + J[t]->link) * 7, X[t].R, X[t].p);
-        continue;
+      float Rw[9];
+      V3<float> pw = X[t].p;
+      const float *R = X[t].R;
+      if (!J[t]->lident) {  // link frame != joint frame
+        mm(X[t].R, J[t]->lR, Rw);
+        pw = mv(X[t].R, load3(J[t]->lp)) + X[t].p;
+        R = Rw;
+      }
+      if (live) store_pose(poses + (e * L + J[t]->link) * 7, R, pw);
+    }
+    if (g == 0 && M->root_link >= 0 && live) store_pose(poses + (e * L + M->root_link) * 7, M->root_R, load3(M->root_p));
+  } else if (flags & KF_POSES) {
+    // general link lists: link slot l = l0 + g fetches its joint's pose from the owner lane
+    for (int l0 = 0; l0 < L; l0 += G) {
+      const int l = l0 + g;
+      const LinkRec<float> &LL = links_of(Mg)[l < L ? l : L - 1];  // link records stay in global memory
+      const int jl = LL.joint < 0 ? 0 : LL.joint;
+      const int src = base4 + 4 * (jl / C), slot = jl % C;
+      Xf W;
+#pragma unroll
+      for (int s = 0; s < C; ++s) {
+        Xf y;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) y.R[k] = bperm(src, X[s].R[k]);
+        y.p = {bperm(src, X[s].p.x), bperm(src, X[s].p.y), bperm(src, X[s].p.z)};
+        W = s == 0 ? y : sel_xf(slot == s, y, W);
       }
       float Rw[9];
-      mm(X[t].R, J[t]->lR, Rw);
-      store_pose(poses + (e * L + J[t]->link) * 7, Rw, mv(X[t].R, load3(J[t]->lp)) + X[t].p);
+      mm(W.R, LL.R, Rw);
+      V3<float> pw = mv(W.R, load3(LL.p)) + W.p;
+      if (LL.joint < 0) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Rw[k] = LL.R[k];
+        pw = load3(LL.p);
+      }
+      if (live && l < L) store_pose(poses + (e * L + l) * 7, Rw, pw);
     }
-    if (g == 0 && M->root_link >= 0) store_pose(poses + (e * L + M->root_link) * 7, M->root_R, load3(M->root_p));
-    return;
   }
-  // general link lists: link slot l = l0 + g fetches its joint's pose from the owner lane
-  for (int l0 = 0; l0 < L; l0 += G) {
-    const int l = l0 + g;
-    const LinkRec<float> &LL = links_of(Mg)[l < L ? l : L - 1];  // link records stay in global memory
-    const int jl = LL.joint < 0 ? 0 : LL.joint;
-    const int src = base4 + 4 * (jl / C), slot = jl % C;
-    Xf W;
+
+  // ---- RNEA
+  if (!dyn) return;
+  S6<float> v[C], a[C];
 #pragma unroll
-    for (int s = 0; s < C; ++s) {
-      Xf y;
+  for (int t = 0; t < C; ++t) v[t] = (t > 0 ? v[t - 1] : zero6<float>()) + qdj[t] * col[t];
+  if constexpr (G > 1) {
+    S6<float> incl = v[C - 1];
+    scan_sum<G>(g, incl);
+    if constexpr (C > 1) {
+      const S6<float> E = excl_sum<G>(g, incl);
 #pragma unroll
-      for (int k = 0; k < 9; ++k) y.R[k] = bperm(src, X[s].R[k]);
-      y.p = {bperm(src, X[s].p.x), bperm(src, X[s].p.y), bperm(src, X[s].p.z)};
-      W = s == 0 ? y : sel_xf(slot == s, y, W);
+      for (int t = 0; t < C - 1; ++t) v[t] = v[t] + E;
     }
-    float Rw[9];
-    mm(W.R, LL.R, Rw);
-    V3<float> pw = mv(W.R, load3(LL.p)) + W.p;
-    if (LL.joint < 0) {
+    v[C - 1] = incl;
+  }
 #pragma unroll
-      for (int k = 0; k < 9; ++k) Rw[k] = LL.R[k];
-      pw = load3(LL.p);
+  for (int t = 0; t < C; ++t)
+    a[t] = (t > 0 ? a[t - 1] : zero6<float>()) + qddj[t] * col[t] + qdj[t] * crossm(v[t], col[t]);
+  S6<float> Ea = zero6<float>();
+  if constexpr (G > 1) {
+    S6<float> incl = a[C - 1];
+    scan_sum<G>(g, incl);
+    if constexpr (C > 1) Ea = excl_sum<G>(g, incl);
+    a[C - 1] = incl;
+  }
+  const V3<float> grav = {M->gravity[0], M->gravity[1], M->gravity[2]};
+  S6<float> f[C];
+#pragma unroll
+  for (int t = 0; t < C; ++t) {
+    S6<float> at = (G > 1 && t == C - 1) ? a[t] : a[t] + Ea;
+    at.l = at.l - grav;
+    const WInertia<float> Y = world_inertia_fast(*J[t], X[t].R, X[t].p);
+    f[t] = valid[t] ? inertia_apply(Y, at) + crossf(v[t], inertia_apply(Y, v[t])) : zero6<float>();
+  }
+#pragma unroll
+  for (int t = C - 2; t >= 0; --t) f[t] = f[t] + f[t + 1];  // suffix sums inside the lane
+  if constexpr (G > 1) {
+    S6<float> incl = f[0];
+    scan_suffix<G>(g, incl);
+    if constexpr (C > 1) {
+      const S6<float> E = excl_suffix<G>(g, incl);
+#pragma unroll
+      for (int t = 1; t < C; ++t) f[t] = f[t] + E;
     }
-    if (live && l < L) store_pose(poses + (e * L + l) * 7, Rw, pw);
+    f[0] = incl;
+  }
+#pragma unroll
+  for (int t = 0; t < C; ++t) {
+    if (!valid[t]) continue;
+    if (live) tau[e * nv + J[t]->vuser] = dot6(col[t], f[t]);
   }
 }
 

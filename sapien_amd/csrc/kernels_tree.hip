@@ -142,7 +142,7 @@ __device__ __forceinline__ void tree_body(const ModelTable<float> *M, const Mode
   if (!dyn) qdj = qddj = 0.f;
 
   float s, c;
-  sincosf(qj, &s, &c);
+  joint_sincos(qj, &s, &c);
   Xf X;
   joint_world(J, qj, s, c, static_cast<const float *>(nullptr), V3<float>{0, 0, 0}, true, X.R, X.p);
   if (!valid) X = identity_xf();
@@ -259,7 +259,7 @@ __device__ __forceinline__ void tree_vjp_body(const ModelTable<float> *M, int64_
   if (!dyn) qdj = qddj = gj = 0.f;
 
   float s, c;
-  sincosf(qj, &s, &c);
+  joint_sincos(qj, &s, &c);
   Xf X;
   joint_world(J, qj, s, c, static_cast<const float *>(nullptr), V3<float>{0, 0, 0}, true, X.R, X.p);
   if (!valid) X = identity_xf();

@@ -415,3 +415,20 @@ def test_fused_launch_graph_replay_matches_eager():
             assert all(torch.equal(a, b) for a, b in zip(s[3:], r)), rep
         m.compute_fk_jacobian_rnea_batched(sets[0][0], sets[0][1], sets[0][2])  # eager work between replays
         torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("scale", [16.0, 200.0])
+def test_large_joint_angles(scale):
+    """Joint angles far outside the sampled range (both sides of the 16 rad
+    switch between the hardware sine and the full-range routine) keep the pose,
+    Jacobian and torque tolerances, in the chain and tree kernels."""
+    for tree in ("chain7", "arm9"):
+        rob = S.robot(tree)
+        m, o = _pair(rob)
+        q, qd, qdd, _ = S.samples(rob, 2048, 1900)
+        # revolute joints only (prismatic fingers keep their range), at
+        # FP32-representable angles so that the oracle sees the kernel's inputs
+        rev = np.asarray(rob.upper) > 1.0
+        big = np.random.default_rng(5).uniform(-scale, scale, size=q.shape).astype(np.float32).astype(np.float64)
+        q = np.where(rev[None, :], big, q)
+        _check_kin(m, o, q, qd, qdd)
