@@ -41,6 +41,7 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
                                                      const float *__restrict__ q, const float *__restrict__ qd,
                                                      const float *__restrict__ qdd, float *__restrict__ poses,
                                                      float *__restrict__ Jout, float *__restrict__ tau) {
+  constexpr bool NT = G >= 8;  // streaming output stores (see st())
   // model header + joint records staged in LDS once per block
   __shared__ float4 s_tab[kTableWords4];
   const int g = threadIdx.x & (G - 1);
@@ -135,8 +136,8 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
       if (!valid[t]) continue;
       if (live) {
         float *Jb = Jout + e * 6 * nv + J[t]->vuser;
-        Jb[0 * nv] = col[t].l.x; Jb[1 * nv] = col[t].l.y; Jb[2 * nv] = col[t].l.z;
-        Jb[3 * nv] = col[t].a.x; Jb[4 * nv] = col[t].a.y; Jb[5 * nv] = col[t].a.z;
+        st<NT>(col[t].l.x, Jb + 0 * nv); st<NT>(col[t].l.y, Jb + 1 * nv); st<NT>(col[t].l.z, Jb + 2 * nv);
+        st<NT>(col[t].a.x, Jb + 3 * nv); st<NT>(col[t].a.y, Jb + 4 * nv); st<NT>(col[t].a.z, Jb + 5 * nv);
       }
     }
   }
@@ -153,9 +154,10 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
         pw = mv(X[t].R, load3(J[t]->lp)) + X[t].p;
         R = Rw;
       }
-      if (live) store_pose(poses + (e * L + J[t]->link) * 7, R, pw);
+      if (live) store_pose<NT>(poses + (e * L + J[t]->link) * 7, R, pw);
     }
-    if (g == 0 && M->root_link >= 0 && live) store_pose(poses + (e * L + M->root_link) * 7, M->root_R, load3(M->root_p));
+    if (g == 0 && M->root_link >= 0 && live)
+      store_pose<NT>(poses + (e * L + M->root_link) * 7, M->root_R, load3(M->root_p));
   } else if (flags & KF_POSES) {
     // general link lists: link slot l = l0 + g fetches its joint's pose from the owner lane
     for (int l0 = 0; l0 < L; l0 += G) {
@@ -180,7 +182,7 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
         for (int k = 0; k < 9; ++k) Rw[k] = LL.R[k];
         pw = load3(LL.p);
       }
-      if (live && l < L) store_pose(poses + (e * L + l) * 7, Rw, pw);
+      if (live && l < L) store_pose<NT>(poses + (e * L + l) * 7, Rw, pw);
     }
   }
 
@@ -233,7 +235,7 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
 #pragma unroll
   for (int t = 0; t < C; ++t) {
     if (!valid[t]) continue;
-    if (live) tau[e * nv + J[t]->vuser] = dot6(col[t], f[t]);
+    if (live) st<NT>(dot6(col[t], f[t]), tau + e * nv + J[t]->vuser);
   }
 }
 

@@ -149,6 +149,44 @@ __device__ __forceinline__ void tree_body(const ModelTable<float> *M, const Mode
   T.prefix(X);
   const S6<float> col = valid ? jcol(J, X.R, X.p) : zero6<float>();
   const int u = J.vuser;
+  constexpr bool NT = G >= 8;  // streaming output stores (see st())
+
+  // Jacobian columns and link poses first: their stores drain while the
+  // dynamics compute
+  if ((flags & KF_JFULL) && live && valid) {
+    float *Jb = io.J + e * 6 * nv + u;
+    st<NT>(col.l.x, Jb + 0 * nv); st<NT>(col.l.y, Jb + 1 * nv); st<NT>(col.l.z, Jb + 2 * nv);
+    st<NT>(col.a.x, Jb + 3 * nv); st<NT>(col.a.y, Jb + 4 * nv); st<NT>(col.a.z, Jb + 5 * nv);
+  }
+  if ((flags & KF_POSES) && M->links_le1) {
+    if (live && valid && J.link >= 0) {
+      if (J.lident) {  // link frame = joint frame
+        store_pose<NT>(io.poses + (e * L + J.link) * 7, X.R, X.p);
+      } else {
+        float Rw[9];
+        mm(X.R, J.lR, Rw);
+        store_pose<NT>(io.poses + (e * L + J.link) * 7, Rw, mv(X.R, load3(J.lp)) + X.p);
+      }
+    }
+    if (live && g == 0 && M->root_link >= 0)
+      store_pose<NT>(io.poses + (e * L + M->root_link) * 7, M->root_R, load3(M->root_p));
+  } else if (flags & KF_POSES) {
+    for (int l0 = 0; l0 < L; l0 += G) {  // every lane takes part in the permutes
+      const int l = l0 + g;
+      const LinkRec<float> &LL = links_of(Mg)[l < L ? l : L - 1];
+      const int jl = LL.joint < 0 ? 0 : LL.joint;
+      const Xf W = bperm_xf(T.base4 + 4 * jl, X);
+      float Rw[9];
+      mm(W.R, LL.R, Rw);
+      V3<float> pw = mv(W.R, load3(LL.p)) + W.p;
+      if (LL.joint < 0) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Rw[k] = LL.R[k];
+        pw = load3(LL.p);
+      }
+      if (live && l < L) store_pose<NT>(io.poses + (e * L + l) * 7, Rw, pw);
+    }
+  }
 
   if (dyn) {
     S6<float> v = qdj * col;
@@ -159,12 +197,7 @@ __device__ __forceinline__ void tree_body(const ModelTable<float> *M, const Mode
     const WInertia<float> Y = world_inertia_fast(J, X.R, X.p);
     S6<float> f = valid ? inertia_apply(Y, a) + crossf(v, inertia_apply(Y, v)) : zero6<float>();
     T.subtree(f);
-    if (live && valid) io.tau[e * nv + u] = dot6(col, f);
-  }
-  if ((flags & KF_JFULL) && live && valid) {
-    float *Jb = io.J + e * 6 * nv + u;
-    Jb[0 * nv] = col.l.x; Jb[1 * nv] = col.l.y; Jb[2 * nv] = col.l.z;
-    Jb[3 * nv] = col.a.x; Jb[4 * nv] = col.a.y; Jb[5 * nv] = col.a.z;
+    if (live && valid) st<NT>(dot6(col, f), io.tau + e * nv + u);
   }
   if (flags & KF_CRBA) {
     // composite inertia of the subtree (additive form about the world origin)
@@ -187,34 +220,8 @@ __device__ __forceinline__ void tree_body(const ModelTable<float> *M, const Mode
       const bool i_anc_j = i <= g && g < i + Ji.subtree;
       const bool j_anc_i = g < i && i < g + T.size;
       const float mij = i_anc_j ? dot6(ci, Fj) : j_anc_i ? dot6(col, Fi) : 0.f;
-      if (live && valid) io.M[(e * nv + u) * nv + Ji.vuser] = mij;
+      if (live && valid) st<NT>(mij, io.M + (e * nv + u) * nv + Ji.vuser);
     }
-  }
-  if (!(flags & KF_POSES)) return;
-  if (M->links_le1) {
-    if (!live) return;
-    if (valid && J.link >= 0) {
-      float Rw[9];
-      mm(X.R, J.lR, Rw);
-      store_pose(io.poses + (e * L + J.link) * 7, Rw, mv(X.R, load3(J.lp)) + X.p);
-    }
-    if (g == 0 && M->root_link >= 0) store_pose(io.poses + (e * L + M->root_link) * 7, M->root_R, load3(M->root_p));
-    return;
-  }
-  for (int l0 = 0; l0 < L; l0 += G) {
-    const int l = l0 + g;
-    const LinkRec<float> &LL = links_of(Mg)[l < L ? l : L - 1];
-    const int jl = LL.joint < 0 ? 0 : LL.joint;
-    const Xf W = bperm_xf(T.base4 + 4 * jl, X);
-    float Rw[9];
-    mm(W.R, LL.R, Rw);
-    V3<float> pw = mv(W.R, load3(LL.p)) + W.p;
-    if (LL.joint < 0) {
-#pragma unroll
-      for (int k = 0; k < 9; ++k) Rw[k] = LL.R[k];
-      pw = load3(LL.p);
-    }
-    if (live && l < L) store_pose(io.poses + (e * L + l) * 7, Rw, pw);
   }
 }
 

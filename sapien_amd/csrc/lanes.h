@@ -133,11 +133,20 @@ __device__ __forceinline__ void quat_branchless(const float *R, float *q) {
   q[3] = k == 0 ? c : k == 1 ? e : k == 2 ? f : big;
 }
 
-__device__ __forceinline__ void store_pose(float *P, const float *R, V3<float> p) {
+// Output store; NT = non-temporal (streamed past the caches). With 8 or more
+// lanes per environment the lanes of a wave cover whole runs of each output
+// row, and streaming them is faster; with fewer lanes the runs are scattered
+// and cached stores merge better.
+template <bool NT> __device__ __forceinline__ void st(float v, float *p) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+template <bool NT = false> __device__ __forceinline__ void store_pose(float *P, const float *R, V3<float> p) {
   float qq[4];
   quat_branchless(R, qq);
-  P[0] = p.x; P[1] = p.y; P[2] = p.z;
-  P[3] = qq[0]; P[4] = qq[1]; P[5] = qq[2]; P[6] = qq[3];
+  st<NT>(p.x, P + 0); st<NT>(p.y, P + 1); st<NT>(p.z, P + 2);
+  st<NT>(qq[0], P + 3); st<NT>(qq[1], P + 4); st<NT>(qq[2], P + 5); st<NT>(qq[3], P + 6);
 }
 
 // World inertia of a body whose rotational inertia is diagonal in its frame:
