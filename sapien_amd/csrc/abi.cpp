@@ -63,6 +63,28 @@ void skew2(const double a[3], double K[9]) {
   K[3] = a[2]; K[4] = 0; K[5] = -a[0];
   K[6] = -a[1]; K[7] = a[0]; K[8] = 0;
 }
+// Quaternion (w x y z) of a rotation matrix, Shoemake's method with Eigen's
+// branch rule (w from the trace when it is positive, else the largest
+// diagonal entry first), as Eigen::Quaterniond(R) in pinocchio_model.cpp:159.
+void quat_of_matrix(const double *R, double *qq) {
+  double t = R[0] + R[4] + R[8];
+  if (t > 0) {
+    double r = std::sqrt(t + 1), h = 0.5 / r;
+    qq[0] = 0.5 * r; qq[1] = (R[7] - R[5]) * h; qq[2] = (R[2] - R[6]) * h; qq[3] = (R[3] - R[1]) * h;
+  } else {
+    int i = 0;
+    if (R[4] > R[0]) i = 1;
+    if (R[8] > R[3 * i + i]) i = 2;
+    int j = (i + 1) % 3, k2 = (j + 1) % 3;
+    double r = std::sqrt(R[3 * i + i] - R[3 * j + j] - R[3 * k2 + k2] + 1), h = 0.5 / r;
+    double xyz[3];
+    xyz[i] = 0.5 * r;
+    xyz[j] = (R[3 * j + i] + R[3 * i + j]) * h;
+    xyz[k2] = (R[3 * k2 + i] + R[3 * i + k2]) * h;
+    qq[0] = (R[3 * k2 + j] - R[3 * j + k2]) * h;
+    qq[1] = xyz[0]; qq[2] = xyz[1]; qq[3] = xyz[2];
+  }
+}
 void mm3(const double *A, const double *B, double *C) {
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
@@ -177,6 +199,9 @@ struct sapamd_model {
         hdr->root_link = l;
         for (int k = 0; k < 9; ++k) hdr->root_R[k] = T(f.place.R[k]);
         for (int k = 0; k < 3; ++k) hdr->root_p[k] = T(f.place.p[k]);
+        double rq[4];
+        quat_of_matrix(f.place.R, rq);
+        for (int k = 0; k < 4; ++k) hdr->root_q[k] = T(rq[k]);
       } else {
         ++per_joint[f.joint];
         jr[f.joint].link = l;
@@ -415,23 +440,7 @@ struct sapamd_model {
       const Frame &f = host->frames()[host->link_frames()[k]];
       const double *R = f.place.R;
       for (int i = 0; i < 3; ++i) pose_cache[7 * k + i] = f.place.p[i];
-      double t = R[0] + R[4] + R[8], *qq = &pose_cache[7 * k + 3];
-      if (t > 0) {
-        double r = std::sqrt(t + 1), h = 0.5 / r;
-        qq[0] = 0.5 * r; qq[1] = (R[7] - R[5]) * h; qq[2] = (R[2] - R[6]) * h; qq[3] = (R[3] - R[1]) * h;
-      } else {
-        int i = 0;
-        if (R[4] > R[0]) i = 1;
-        if (R[8] > R[3 * i + i]) i = 2;
-        int j = (i + 1) % 3, k2 = (j + 1) % 3;
-        double r = std::sqrt(R[3 * i + i] - R[3 * j + j] - R[3 * k2 + k2] + 1), h = 0.5 / r;
-        double xyz[3];
-        xyz[i] = 0.5 * r;
-        xyz[j] = (R[3 * j + i] + R[3 * i + j]) * h;
-        xyz[k2] = (R[3 * k2 + i] + R[3 * i + k2]) * h;
-        qq[0] = (R[3 * k2 + j] - R[3 * j + k2]) * h;
-        qq[1] = xyz[0]; qq[2] = xyz[1]; qq[3] = xyz[2];
-      }
+      quat_of_matrix(R, &pose_cache[7 * k + 3]);
     }
     jw_cache.assign(6 * n * l, 0.0);
     jl_cache.assign(6 * n * l, 0.0);

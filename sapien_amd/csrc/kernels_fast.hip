@@ -127,6 +127,10 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
   S6<float> col[C];
 #pragma unroll
   for (int t = 0; t < C; ++t) col[t] = valid[t] ? jcol(*J[t], X[t].R, X[t].p) : zero6<float>();
+  // this environment's output rows (64-bit offsets formed once)
+  float *const prow = poses + e * (int64_t)(L * 7);
+  float *const jrow = Jout + e * (int64_t)(6 * nv);
+  float *const trow = tau + e * (int64_t)nv;
 
   // ---- Jacobian columns and link poses, emitted before the dynamics so that
   // their stores drain while RNEA computes
@@ -135,7 +139,7 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
     for (int t = 0; t < C; ++t) {
       if (!valid[t]) continue;
       if (live) {
-        float *Jb = Jout + e * 6 * nv + J[t]->vuser;
+        float *Jb = jrow + J[t]->vuser;
         st<NT>(col[t].l.x, Jb + 0 * nv); st<NT>(col[t].l.y, Jb + 1 * nv); st<NT>(col[t].l.z, Jb + 2 * nv);
         st<NT>(col[t].a.x, Jb + 3 * nv); st<NT>(col[t].a.y, Jb + 4 * nv); st<NT>(col[t].a.z, Jb + 5 * nv);
       }
@@ -154,10 +158,10 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
         pw = mv(X[t].R, load3(J[t]->lp)) + X[t].p;
         R = Rw;
       }
-      if (live) store_pose<NT>(poses + (e * L + J[t]->link) * 7, R, pw);
+      if (live) store_pose<NT>(prow + J[t]->link * 7, R, pw);
     }
     if (g == 0 && M->root_link >= 0 && live)
-      store_pose<NT>(poses + (e * L + M->root_link) * 7, M->root_R, load3(M->root_p));
+      store_pose_q<NT>(prow + M->root_link * 7, M->root_p, M->root_q);
   } else if (flags & KF_POSES) {
     // general link lists: link slot l = l0 + g fetches its joint's pose from the owner lane
     for (int l0 = 0; l0 < L; l0 += G) {
@@ -182,7 +186,7 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
         for (int k = 0; k < 9; ++k) Rw[k] = LL.R[k];
         pw = load3(LL.p);
       }
-      if (live && l < L) store_pose<NT>(poses + (e * L + l) * 7, Rw, pw);
+      if (live && l < L) store_pose<NT>(prow + l * 7, Rw, pw);
     }
   }
 
@@ -235,7 +239,7 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
 #pragma unroll
   for (int t = 0; t < C; ++t) {
     if (!valid[t]) continue;
-    if (live) st<NT>(dot6(col[t], f[t]), tau + e * nv + J[t]->vuser);
+    if (live) st<NT>(dot6(col[t], f[t]), trow + J[t]->vuser);
   }
 }
 

@@ -169,7 +169,7 @@ __device__ __forceinline__ void tree_body(const ModelTable<float> *M, const Mode
       }
     }
     if (live && g == 0 && M->root_link >= 0)
-      store_pose<NT>(io.poses + (e * L + M->root_link) * 7, M->root_R, load3(M->root_p));
+      store_pose_q<NT>(io.poses + (e * L + M->root_link) * 7, M->root_p, M->root_q);
   } else if (flags & KF_POSES) {
     for (int l0 = 0; l0 < L; l0 += G) {  // every lane takes part in the permutes
       const int l = l0 + g;

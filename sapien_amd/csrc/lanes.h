@@ -122,8 +122,11 @@ __device__ __forceinline__ void quat_branchless(const float *R, float *q) {
   float di = i ? R[4] : R[0];
   int k = tr > 0.f ? 0 : 1 + (R[8] > di ? 2 : i);
   float t = k == 0 ? tr : k == 1 ? R[0] - R[4] - R[8] : k == 2 ? R[4] - R[8] - R[0] : R[8] - R[0] - R[4];
-  float r = sqrtf(t + 1.f);
-  float h = 0.5f / r;
+  // t + 1 >= 1 on the chosen branch: the hardware reciprocal square root
+  // (about 1 ulp) replaces the correctly rounded sqrt and division
+  const float rs = __builtin_amdgcn_rsqf(t + 1.f);
+  float r = (t + 1.f) * rs;
+  float h = 0.5f * rs;
   float a = (R[7] - R[5]) * h, b = (R[2] - R[6]) * h, c = (R[3] - R[1]) * h;  // w-type
   float d = (R[1] + R[3]) * h, e = (R[2] + R[6]) * h, f = (R[5] + R[7]) * h;  // off-diagonal sums
   float big = 0.5f * r;
@@ -140,6 +143,12 @@ __device__ __forceinline__ void quat_branchless(const float *R, float *q) {
 template <bool NT> __device__ __forceinline__ void st(float v, float *p) {
   if constexpr (NT) __builtin_nontemporal_store(v, p);
   else *p = v;
+}
+
+// A constant pose (position, quaternion already computed).
+template <bool NT = false> __device__ __forceinline__ void store_pose_q(float *P, const float *p, const float *q) {
+  st<NT>(p[0], P + 0); st<NT>(p[1], P + 1); st<NT>(p[2], P + 2);
+  st<NT>(q[0], P + 3); st<NT>(q[1], P + 4); st<NT>(q[2], P + 5); st<NT>(q[3], P + 6);
 }
 
 template <bool NT = false> __device__ __forceinline__ void store_pose(float *P, const float *R, V3<float> p) {

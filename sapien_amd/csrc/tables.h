@@ -58,6 +58,7 @@ template <class T> struct ModelTable {
   int32_t root_link;    // the user link on the universe (-1 = none) when links_le1
   T gravity[4];
   T root_R[9], root_p[3];  // that link's placement
+  T root_q[4];             // root_R as a quaternion (w x y z, Eigen's sign rule)
   // JointRec<T> joints[nj]; LinkRec<T> links[nlinks]; follow at kHeaderBytes
 };
 constexpr size_t kHeaderBytes = 256;
