@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--kernel", default="auto", help="auto | generic | fast (kernel variant of the fused path)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
     ap.add_argument("--no-workloads", action="store_true", help="skip the mixed-tree CLIK / rollout / hetero sections")
+    ap.add_argument("--shape", type=int, nargs=2, default=None, metavar=("G", "C"),
+                    help="lanes-per-joint shape instead of autotuning (profiling runs, where tracing skews the tuner)")
     ap.add_argument("--probe-traffic", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--probe-shape", type=int, nargs=2, default=None, help=argparse.SUPPRESS)
     return ap.parse_args()
@@ -259,7 +261,11 @@ def main():
     n, L = rob.nv, rob.nlinks
     from sapien_amd import tune
 
-    tuned = tune.autotune(m, batch_sizes=(B,), use_cache=False) if sa.kernel_variant(m) == "lanes_per_joint" else {}
+    if args.shape and sa.kernel_variant(m) == "lanes_per_joint":
+        tune.set_shape(m, B, args.shape[0], args.shape[1])
+        tuned = {B: (args.shape[0], args.shape[1], None)}
+    else:
+        tuned = tune.autotune(m, batch_sizes=(B,), use_cache=False) if sa.kernel_variant(m) == "lanes_per_joint" else {}
 
     # 64 rotating buffer sets, seeded like the workload (seed 1001 + set index)
     sets = []
@@ -358,7 +364,8 @@ def main():
         "config": {"workload": "chain7 FK (8 link poses) + WORLD Jacobian (6x7) + RNEA, B=16384, FP32",
                    "envs": B, "dof": n, "links": L, "kernel": sa.kernel_variant(m),
                    "shape": ({"lanes_per_env": tuned[B][0], "joints_per_lane": tuned[B][1],
-                              "envs_per_wavefront": 64 // tuned[B][0], "autotuned": True} if tuned else None)},
+                              "envs_per_wavefront": 64 // tuned[B][0], "autotuned": not args.shape}
+                             if tuned else None)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (traffic["FETCH_SIZE"]["bytes"] + traffic["WRITE_SIZE"]["bytes"]) if traffic else None,
