@@ -129,6 +129,24 @@ int sapamd_get_chain_shape(const sapamd_model *m, int64_t B, int32_t *G, int32_t
  * the throughput-critical entry point. Any of poses/J/tau may be NULL to skip it. */
 int sapamd_fk_jac_rnea_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd,
                                const float *qdd, float *poses, float *J, float *tau, void *stream);
+/* The same over padded buffers, without copies: q/qd/qdd/tau rows every *_ld
+ * floats (0 = nv), link poses every pose_link_stride floats (0 = 7) within
+ * rows of pose_env_stride floats (0 = L * 7); J stays (B, 6, nv). This reads a
+ * GPU simulator's joint state [n_art, max_dof] in place and writes p | q wxyz
+ * into the first 7 floats per link of a [n_art, max_links, 13] body-state
+ * buffer (src/physx/physx_system.cpp:1048-1101 lay these out for SAPIEN's GPU
+ * PhysX system). */
+int sapamd_fk_jac_rnea_strided(const sapamd_model *m, int64_t B, const float *q, int64_t q_ld, const float *qd,
+                               int64_t qd_ld, const float *qdd, int64_t qdd_ld, float *poses,
+                               int64_t pose_env_stride, int64_t pose_link_stride, float *J, float *tau,
+                               int64_t tau_ld, void *stream);
+/* Passive joint forces, as PhysxArticulation::computePassiveForce
+ * (src/physx/articulation.cpp:400-418): gravity compensation g(q) and/or
+ * Coriolis and centrifugal forces C(q, qd) qd, i.e. RNEA(q, qd, 0) with the
+ * chosen terms (gravity = 0 drops g; coriolis_and_centrifugal = 0 reads no qd). */
+int sapamd_passive_force_batched(const sapamd_model *m, int64_t B, const float *q, int64_t q_ld, const float *qd,
+                                 int64_t qd_ld, int32_t gravity, int32_t coriolis_and_centrifugal, float *tau,
+                                 int64_t tau_ld, void *stream);
 /* Batched compute_forward_kinematics + get_link_pose for all links: poses (B, L, 7). */
 int sapamd_fk_batched(const sapamd_model *m, int64_t B, const float *q, float *poses, void *stream);
 /* Batched compute_full_jacobian: (B, 6, nv). */

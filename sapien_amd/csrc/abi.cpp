@@ -834,29 +834,88 @@ int sapamd_get_chain_shape(const sapamd_model *m, int64_t B, int32_t *G, int32_t
   });
 }
 
+namespace {
+// Launch of the fused FK / Jacobian / RNEA kernel family for one model; flags
+// carry the outputs and RNEA variants, o the pointers and strides.
+void fused_launch(const sapamd_model *m, int64_t B, int flags, const float *q, const float *qd, const float *qdd,
+                  const KinOut<float> &o, void *stream) {
+  if (m->effective_variant() == SAPAMD_KERNEL_LANES_PER_JOINT) {
+    if (m->chain_kernel()) {
+      int G, C;
+      m->shape_for(B, &G, &C);
+      HIP_OK(launch_chain_fused(m->t32(), m->host->njoints(), B, flags, q, qd, qdd, o, S(stream), G, C));
+    } else {
+      HIP_OK(launch_tree_fused(m->t32(), m->host->njoints(), B, flags, q, qd, qdd, o, S(stream)));
+    }
+  } else {
+    // the generic kernel reads qd / qdd whenever RNEA runs: alias the unused ones to q
+    HIP_OK(launch_kinematics<float>(m->t32(), m->host->njoints(), B, flags, q,
+                                    (flags & KF_QD0) || !qd ? q : qd, (flags & KF_QDD0) || !qdd ? q : qdd, o,
+                                    S(stream)));
+  }
+}
+void check_ld(int64_t ld, int64_t min, const char *what) {
+  if (ld != 0 && ld < min) throw std::invalid_argument(std::string(what) + " row stride smaller than a row");
+}
+}  // namespace
+
 int sapamd_fk_jac_rnea_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd,
                                const float *qdd, float *poses, float *J, float *tau, void *stream) {
+  return sapamd_fk_jac_rnea_strided(m, B, q, 0, qd, 0, qdd, 0, poses, 0, 0, J, tau, 0, stream);
+}
+
+int sapamd_fk_jac_rnea_strided(const sapamd_model *m, int64_t B, const float *q, int64_t q_ld, const float *qd,
+                               int64_t qd_ld, const float *qdd, int64_t qdd_ld, float *poses,
+                               int64_t pose_env_stride, int64_t pose_link_stride, float *J, float *tau,
+                               int64_t tau_ld, void *stream) {
   return guarded([&] {
     check_batched(m, B);
     if (B == 0) return;
     need(q, "q");
+    const int nv = m->nv(), L = m->L();
+    check_ld(q_ld, nv, "q");
+    check_ld(qd_ld, nv, "qd");
+    check_ld(qdd_ld, nv, "qdd");
+    check_ld(tau_ld, nv, "tau");
+    check_ld(pose_link_stride, 7, "pose link");
+    check_ld(pose_env_stride, (int64_t)(L > 0 ? L - 1 : 0) * (pose_link_stride ? pose_link_stride : 7) + 7, "pose");
     int flags = 0;
     KinOut<float> o;
-    if (poses) { flags |= KF_POSES; o.poses = poses; if (m->L() == 0) throw std::runtime_error("link index out of bound"); }
+    o.ld_q = q_ld;
+    o.ld_qd = qd_ld;
+    o.ld_qdd = qdd_ld;
+    o.ld_tau = tau_ld;
+    o.pose_env = pose_env_stride;
+    o.pose_link = pose_link_stride;
+    if (poses) { flags |= KF_POSES; o.poses = poses; if (L == 0) throw std::runtime_error("link index out of bound"); }
     if (J) { flags |= KF_JFULL; o.jfull = J; }
     if (tau) { need(qd, "qd"); need(qdd, "qdd"); flags |= KF_RNEA; o.tau = tau; }
-    if (m->effective_variant() == SAPAMD_KERNEL_LANES_PER_JOINT) {
-      if (m->chain_kernel()) {
-        int G, C;
-        m->shape_for(B, &G, &C);
-        HIP_OK(launch_chain_fused(m->t32(), m->host->njoints(), B, flags, q, qd, qdd, o, S(stream), G, C));
-      } else {
-        HIP_OK(launch_tree_fused(m->t32(), m->host->njoints(), B, flags, q, qd, qdd, poses, J, tau, nullptr,
-                                 S(stream)));
-      }
-    } else {
-      HIP_OK(launch_kinematics<float>(m->t32(), m->host->njoints(), B, flags, q, qd, qdd, o, S(stream)));
-    }
+    fused_launch(m, B, flags, q, qd, qdd, o, stream);
+  });
+}
+
+int sapamd_passive_force_batched(const sapamd_model *m, int64_t B, const float *q, int64_t q_ld, const float *qd,
+                                 int64_t qd_ld, int32_t gravity, int32_t coriolis_and_centrifugal, float *tau,
+                                 int64_t tau_ld, void *stream) {
+  return guarded([&] {
+    check_batched(m, B);
+    if (B == 0) return;
+    need(q, "q");
+    need(tau, "tau");
+    const int nv = m->nv();
+    check_ld(q_ld, nv, "q");
+    check_ld(qd_ld, nv, "qd");
+    check_ld(tau_ld, nv, "tau");
+    if (coriolis_and_centrifugal) need(qd, "qd");
+    int flags = KF_RNEA | KF_QDD0;
+    if (!gravity) flags |= KF_NOGRAV;
+    if (!coriolis_and_centrifugal) flags |= KF_QD0;
+    KinOut<float> o;
+    o.tau = tau;
+    o.ld_q = q_ld;
+    o.ld_qd = qd_ld;
+    o.ld_tau = tau_ld;
+    fused_launch(m, B, flags, q, coriolis_and_centrifugal ? qd : nullptr, nullptr, o, stream);
   });
 }
 
@@ -896,10 +955,11 @@ int sapamd_crba_batched(const sapamd_model *m, int64_t B, const float *q, float 
     if (B == 0) return;
     need(q, "q");
     need(M, "M");
-    if (m->effective_variant() == SAPAMD_KERNEL_LANES_PER_JOINT && tree_group(m->host->njoints()))
-      HIP_OK(launch_tree_fused(m->t32(), m->host->njoints(), B, KF_CRBA, q, nullptr, nullptr, nullptr, nullptr,
-                               nullptr, M, S(stream)));
-    else
+    if (m->effective_variant() == SAPAMD_KERNEL_LANES_PER_JOINT && tree_group(m->host->njoints())) {
+      KinOut<float> o;
+      o.M = M;
+      HIP_OK(launch_tree_fused(m->t32(), m->host->njoints(), B, KF_CRBA, q, nullptr, nullptr, o, S(stream)));
+    } else
       HIP_OK(launch_crba<float>(m->t32(), m->host->njoints(), B, q, M, S(stream)));
   });
 }

@@ -36,11 +36,17 @@ using namespace lanes;
 constexpr int kMaxChainJoints = 16;
 constexpr int kTableWords4 = (int)((kHeaderBytes + sizeof(JointRec<float>) * kMaxChainJoints) / 16);
 
+// Row strides of the batched arrays, resolved on the host (elements).
+struct Layout {
+  int64_t q, qd, qdd, tau, pose_env, pose_link;
+};
+
 template <int G, int C>
 __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__restrict__ Mg, int64_t B, int flags,
                                                      const float *__restrict__ q, const float *__restrict__ qd,
                                                      const float *__restrict__ qdd, float *__restrict__ poses,
-                                                     float *__restrict__ Jout, float *__restrict__ tau) {
+                                                     float *__restrict__ Jout, float *__restrict__ tau,
+                                                     Layout lay) {
   constexpr bool NT = G >= 8;  // streaming output stores (see st())
   // model header + joint records staged in LDS once per block
   __shared__ float4 s_tab[kTableWords4];
@@ -61,10 +67,10 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
 #pragma unroll
   for (int t = 0; t < C; ++t) {
     const int j = g * C + t;
-    const int64_t qi = e * nv + (j < nv ? j : nv - 1);
-    qu[t] = q[qi];
-    qdu[t] = qd[qi];
-    qddu[t] = qdd[qi];
+    const int jc = j < nv ? j : nv - 1;
+    qu[t] = q[e * lay.q + jc];
+    qdu[t] = qd[e * lay.qd + jc];
+    qddu[t] = qdd[e * lay.qdd + jc];
   }
   __syncthreads();
   const ModelTable<float> *M = reinterpret_cast<const ModelTable<float> *>(s_tab);
@@ -99,6 +105,8 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
       }
     }
     if (!valid[t] || !dyn) qdj[t] = qddj[t] = 0.f;
+    if (flags & KF_QD0) qdj[t] = 0.f;
+    if (flags & KF_QDD0) qddj[t] = 0.f;
     if (!valid[t]) qj[t] = 0.f;
   }
 
@@ -128,9 +136,10 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
 #pragma unroll
   for (int t = 0; t < C; ++t) col[t] = valid[t] ? jcol(*J[t], X[t].R, X[t].p) : zero6<float>();
   // this environment's output rows (64-bit offsets formed once)
-  float *const prow = poses + e * (int64_t)(L * 7);
+  float *const prow = poses + e * (lay.pose_env ? lay.pose_env : (int64_t)L * 7);
   float *const jrow = Jout + e * (int64_t)(6 * nv);
-  float *const trow = tau + e * (int64_t)nv;
+  float *const trow = tau + e * lay.tau;
+  const int pl = (int)lay.pose_link;
 
   // ---- Jacobian columns and link poses, emitted before the dynamics so that
   // their stores drain while RNEA computes
@@ -158,10 +167,10 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
         pw = mv(X[t].R, load3(J[t]->lp)) + X[t].p;
         R = Rw;
       }
-      if (live) store_pose<NT>(prow + J[t]->link * 7, R, pw);
+      if (live) store_pose<NT>(prow + J[t]->link * pl, R, pw);
     }
     if (g == 0 && M->root_link >= 0 && live)
-      store_pose_q<NT>(prow + M->root_link * 7, M->root_p, M->root_q);
+      store_pose_q<NT>(prow + M->root_link * pl, M->root_p, M->root_q);
   } else if (flags & KF_POSES) {
     // general link lists: link slot l = l0 + g fetches its joint's pose from the owner lane
     for (int l0 = 0; l0 < L; l0 += G) {
@@ -186,7 +195,7 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
         for (int k = 0; k < 9; ++k) Rw[k] = LL.R[k];
         pw = load3(LL.p);
       }
-      if (live && l < L) store_pose<NT>(prow + l * 7, Rw, pw);
+      if (live && l < L) store_pose<NT>(prow + l * pl, Rw, pw);
     }
   }
 
@@ -215,7 +224,8 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
     if constexpr (C > 1) Ea = excl_sum<G>(g, incl);
     a[C - 1] = incl;
   }
-  const V3<float> grav = {M->gravity[0], M->gravity[1], M->gravity[2]};
+  const V3<float> grav = (flags & KF_NOGRAV) ? V3<float>{0.f, 0.f, 0.f}
+                                            : V3<float>{M->gravity[0], M->gravity[1], M->gravity[2]};
   S6<float> f[C];
 #pragma unroll
   for (int t = 0; t < C; ++t) {
@@ -245,11 +255,11 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
 
 template <int G, int C>
 void launch_gc(const ModelTable<float> *tab, int64_t B, int flags, const float *q, const float *qd, const float *qdd,
-               KinOut<float> out, hipStream_t s) {
+               KinOut<float> out, hipStream_t s, const Layout &lay) {
   const int64_t envs_per_block = 256 / G;
   dim3 grid((unsigned)((B + envs_per_block - 1) / envs_per_block));
   hipLaunchKernelGGL((k_chain_fused<G, C>), grid, dim3(256), 0, s, tab, B, flags, q, qd, qdd, out.poses, out.jfull,
-                     out.tau);
+                     out.tau, lay);
 }
 
 }  // namespace
@@ -277,9 +287,18 @@ hipError_t launch_chain_fused(const ModelTable<float> *tab, int nj, int64_t B, i
                               const float *qd, const float *qdd, KinOut<float> out, hipStream_t s, int G, int C) {
   if (B <= 0) return hipSuccess;
   if (nj < 1 || G * C < nj) return hipErrorNotSupported;
-  if (!(flags & KF_RNEA) || !qd || !qdd) qd = qdd = q;  // read but unused
+  if (!(flags & KF_RNEA) || (flags & KF_QD0) || !qd) qd = q;  // read but unused
+  if (!(flags & KF_RNEA) || (flags & KF_QDD0) || !qdd) qdd = q;
+  const int nv = nj;  // chains: one dof per joint
+  Layout lay;
+  lay.q = out.ld_q ? out.ld_q : nv;
+  lay.qd = qd == q ? lay.q : (out.ld_qd ? out.ld_qd : nv);
+  lay.qdd = qdd == q ? lay.q : (out.ld_qdd ? out.ld_qdd : nv);
+  lay.tau = out.ld_tau ? out.ld_tau : nv;
+  lay.pose_link = out.pose_link ? out.pose_link : 7;
+  lay.pose_env = out.pose_env;  // 0: L * 7, resolved in the kernel from the table
 #define SAPAMD_SHAPE(g_, c_) \
-  if (G == g_ && C == c_) { launch_gc<g_, c_>(tab, B, flags, q, qd, qdd, out, s); return hipGetLastError(); }
+  if (G == g_ && C == c_) { launch_gc<g_, c_>(tab, B, flags, q, qd, qdd, out, s, lay); return hipGetLastError(); }
   SAPAMD_SHAPE(16, 1) SAPAMD_SHAPE(8, 1) SAPAMD_SHAPE(8, 2) SAPAMD_SHAPE(4, 2) SAPAMD_SHAPE(4, 4)
   SAPAMD_SHAPE(2, 4) SAPAMD_SHAPE(2, 8) SAPAMD_SHAPE(1, 8) SAPAMD_SHAPE(1, 16)
 #undef SAPAMD_SHAPE

@@ -44,8 +44,9 @@ __global__ __launch_bounds__(64) void k_kinematics(const ModelTable<T> *__restri
   V3<T> op[MJ];
   S6<T> col[MJ], vel[MJ], acc[MJ], F[MJ];
   const bool dyn = flags & KF_RNEA;
-  const S6<T> a0 = gravity_acc(M);
-  const T *qb = q + b * nv;
+  const S6<T> a0 = (flags & KF_NOGRAV) ? zero6<T>() : gravity_acc(M);
+  const T *qb = q + b * (out.ld_q ? out.ld_q : nv);
+  const T *qdb = qd + b * (out.ld_qd ? out.ld_qd : nv), *qddb = qdd + b * (out.ld_qdd ? out.ld_qdd : nv);
   for (int j = 0; j < nj; ++j) {
     const JointRec<T> &J = JR[j];
     const int u = J.vuser, p = J.parent;
@@ -54,7 +55,7 @@ __global__ __launch_bounds__(64) void k_kinematics(const ModelTable<T> *__restri
     joint_world(J, qj, s, c, p >= 0 ? oR[p] : nullptr, p >= 0 ? op[p] : V3<T>{0, 0, 0}, p < 0, oR[j], op[j]);
     col[j] = jcol(J, oR[j], op[j]);
     if (dyn) {
-      T qdj = qd[b * nv + u], qddj = qdd[b * nv + u];
+      T qdj = (flags & KF_QD0) ? T(0) : qdb[u], qddj = (flags & KF_QDD0) ? T(0) : qddb[u];
       S6<T> vp = p >= 0 ? vel[p] : zero6<T>();
       S6<T> ap = p >= 0 ? acc[p] : a0;
       vel[j] = vp + qdj * col[j];
@@ -65,7 +66,7 @@ __global__ __launch_bounds__(64) void k_kinematics(const ModelTable<T> *__restri
   }
   if (dyn) {
     for (int j = nj - 1; j >= 0; --j) {
-      out.tau[b * nv + JR[j].vuser] = dot6(col[j], F[j]);
+      out.tau[b * (out.ld_tau ? out.ld_tau : nv) + JR[j].vuser] = dot6(col[j], F[j]);
       int p = JR[j].parent;
       if (p >= 0) F[p] = F[p] + F[j];
     }
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(64) void k_kinematics(const ModelTable<T> *__restri
         pw = mv(oR[jl], load3(Lk.p)) + op[jl];
       }
       if (flags & KF_POSES) {
-        T *P = out.poses + (b * L + l) * 7;
+        T *P = out.poses + b * (out.pose_env ? out.pose_env : (int64_t)L * 7) + l * (out.pose_link ? out.pose_link : 7);
         P[0] = pw.x; P[1] = pw.y; P[2] = pw.z;
         quat_of(R, P + 3);
       }

@@ -113,6 +113,8 @@ template <int G> struct Tree {
 struct TreeIO {
   const float *q, *qd, *qdd;
   float *poses, *J, *tau, *M;
+  // row strides in elements (0 = contiguous: nv, L * 7 and 7)
+  int64_t ld_q = 0, ld_qd = 0, ld_qdd = 0, ld_tau = 0, pose_env = 0, pose_link = 0;
 };
 
 // One environment per G-lane group; M is the LDS copy of the model table.
@@ -134,12 +136,14 @@ __device__ __forceinline__ void tree_body(const ModelTable<float> *M, const Mode
   T.size = valid ? J.subtree : 0;
 
   // dofs: lane g loads user dof g, then the joint -> dof permutation via bpermute
-  const int64_t qi = e * nv + (g < nv ? g : nv - 1);
-  float qu = io.q[qi], qdu = io.qd[qi], qddu = io.qdd[qi];
+  const int gc = g < nv ? g : nv - 1;
+  float qu = io.q[e * (io.ld_q ? io.ld_q : nv) + gc], qdu = io.qd[e * (io.ld_qd ? io.ld_qd : nv) + gc],
+        qddu = io.qdd[e * (io.ld_qdd ? io.ld_qdd : nv) + gc];
   const int src = T.base4 + 4 * J.vuser;
   float qj = bperm(src, qu), qdj = bperm(src, qdu), qddj = bperm(src, qddu);
   if (!valid) qj = qdj = qddj = 0.f;
-  if (!dyn) qdj = qddj = 0.f;
+  if (!dyn || (flags & KF_QD0)) qdj = 0.f;
+  if (!dyn || (flags & KF_QDD0)) qddj = 0.f;
 
   float s, c;
   joint_sincos(qj, &s, &c);
@@ -150,6 +154,8 @@ __device__ __forceinline__ void tree_body(const ModelTable<float> *M, const Mode
   const S6<float> col = valid ? jcol(J, X.R, X.p) : zero6<float>();
   const int u = J.vuser;
   constexpr bool NT = G >= 8;  // streaming output stores (see st())
+  float *const prow = io.poses + e * (io.pose_env ? io.pose_env : (int64_t)L * 7);
+  const int pl = io.pose_link ? (int)io.pose_link : 7;
 
   // Jacobian columns and link poses first: their stores drain while the
   // dynamics compute
@@ -161,15 +167,15 @@ __device__ __forceinline__ void tree_body(const ModelTable<float> *M, const Mode
   if ((flags & KF_POSES) && M->links_le1) {
     if (live && valid && J.link >= 0) {
       if (J.lident) {  // link frame = joint frame
-        store_pose<NT>(io.poses + (e * L + J.link) * 7, X.R, X.p);
+        store_pose<NT>(prow + J.link * pl, X.R, X.p);
       } else {
         float Rw[9];
         mm(X.R, J.lR, Rw);
-        store_pose<NT>(io.poses + (e * L + J.link) * 7, Rw, mv(X.R, load3(J.lp)) + X.p);
+        store_pose<NT>(prow + J.link * pl, Rw, mv(X.R, load3(J.lp)) + X.p);
       }
     }
     if (live && g == 0 && M->root_link >= 0)
-      store_pose_q<NT>(io.poses + (e * L + M->root_link) * 7, M->root_p, M->root_q);
+      store_pose_q<NT>(prow + M->root_link * pl, M->root_p, M->root_q);
   } else if (flags & KF_POSES) {
     for (int l0 = 0; l0 < L; l0 += G) {  // every lane takes part in the permutes
       const int l = l0 + g;
@@ -184,7 +190,7 @@ __device__ __forceinline__ void tree_body(const ModelTable<float> *M, const Mode
         for (int k = 0; k < 9; ++k) Rw[k] = LL.R[k];
         pw = load3(LL.p);
       }
-      if (live && l < L) store_pose<NT>(io.poses + (e * L + l) * 7, Rw, pw);
+      if (live && l < L) store_pose<NT>(prow + l * pl, Rw, pw);
     }
   }
 
@@ -193,11 +199,11 @@ __device__ __forceinline__ void tree_body(const ModelTable<float> *M, const Mode
     T.prefix(v);
     S6<float> a = qddj * col + qdj * crossm(v, col);
     T.prefix(a);
-    a.l = a.l - V3<float>{M->gravity[0], M->gravity[1], M->gravity[2]};
+    if (!(flags & KF_NOGRAV)) a.l = a.l - V3<float>{M->gravity[0], M->gravity[1], M->gravity[2]};
     const WInertia<float> Y = world_inertia_fast(J, X.R, X.p);
     S6<float> f = valid ? inertia_apply(Y, a) + crossf(v, inertia_apply(Y, v)) : zero6<float>();
     T.subtree(f);
-    if (live && valid) st<NT>(dot6(col, f), io.tau + e * nv + u);
+    if (live && valid) st<NT>(dot6(col, f), io.tau + e * (io.ld_tau ? io.ld_tau : nv) + u);
   }
   if (flags & KF_CRBA) {
     // composite inertia of the subtree (additive form about the world origin)
@@ -400,12 +406,18 @@ __global__ __launch_bounds__(256) void k_hetero(HeteroArgs args, int flags) {
 int tree_group(int nj) { return nj < 1 ? 0 : nj <= 8 ? 8 : nj <= 16 ? 16 : nj <= 32 ? 32 : 0; }
 
 hipError_t launch_tree_fused(const ModelTable<float> *tab, int nj, int64_t B, int flags, const float *q,
-                             const float *qd, const float *qdd, float *poses, float *J, float *tau, float *M,
-                             hipStream_t s) {
+                             const float *qd, const float *qdd, const KinOut<float> &out, hipStream_t s) {
   if (B <= 0) return hipSuccess;
   const int G = tree_group(nj);
   if (!G) return hipErrorNotSupported;
-  TreeIO io{q, (flags & KF_RNEA) && qd ? qd : q, (flags & KF_RNEA) && qdd ? qdd : q, poses, J, tau, M};
+  const bool use_qd = (flags & KF_RNEA) && !(flags & KF_QD0) && qd, use_qdd = (flags & KF_RNEA) && !(flags & KF_QDD0) && qdd;
+  TreeIO io{q, use_qd ? qd : q, use_qdd ? qdd : q, out.poses, out.jfull, out.tau, out.M};
+  io.ld_q = out.ld_q;
+  io.ld_qd = use_qd ? out.ld_qd : out.ld_q;
+  io.ld_qdd = use_qdd ? out.ld_qdd : out.ld_q;
+  io.ld_tau = out.ld_tau;
+  io.pose_env = out.pose_env;
+  io.pose_link = out.pose_link;
   const int64_t per_block = 256 / G;
   dim3 grid((unsigned)((B + per_block - 1) / per_block));
   if (G == 8) hipLaunchKernelGGL(k_tree_fused<8>, grid, dim3(256), 0, s, tab, B, flags, io);

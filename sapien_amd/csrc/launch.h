@@ -11,7 +11,24 @@
 
 namespace sapamd {
 
-enum KinFlags : int { KF_POSES = 1, KF_JFULL = 2, KF_JLINKS = 4, KF_RNEA = 8, KF_CRBA = 64 };
+enum KinFlags : int {
+  KF_POSES = 1, KF_JFULL = 2, KF_JLINKS = 4, KF_RNEA = 8, KF_CRBA = 64,
+  // RNEA variants (passive forces): no gravity term, qd = 0, qdd = 0 (the
+  // corresponding input pointer is then not read)
+  KF_NOGRAV = 256, KF_QD0 = 512, KF_QDD0 = 1024
+};
+
+// Outputs and the layout of the batched arrays. Strides are in elements; 0
+// means contiguous: q/qd/qdd/tau rows of nv, poses (B, L, 7). Non-zero strides
+// let a caller pass padded buffers without copies (a simulator's
+// [n_art, max_dof] joint state, a [n_art, max_links, 13] body-state buffer whose
+// first 7 floats per link are p xyz | q wxyz).
+template <class T> struct KinOut {
+  T *poses = nullptr, *jfull = nullptr, *jw_links = nullptr, *jl_links = nullptr, *tau = nullptr;
+  T *M = nullptr;  // (B, nv, nv), tree kernels only
+  int64_t ld_q = 0, ld_qd = 0, ld_qdd = 0, ld_tau = 0;
+  int64_t pose_env = 0, pose_link = 0;
+};
 
 // One tree type of a heterogeneous batch: its table and buffers (device
 // pointers, user order), its lanes per environment and its first block.
@@ -26,8 +43,7 @@ struct HeteroSegment {
 // RNEA + CRBA in one pass (flags select outputs).
 int tree_group(int nj);
 hipError_t launch_tree_fused(const ModelTable<float> *tab, int nj, int64_t B, int flags, const float *q,
-                             const float *qd, const float *qdd, float *poses, float *J, float *tau, float *M,
-                             hipStream_t s);
+                             const float *qd, const float *qdd, const KinOut<float> &out, hipStream_t s);
 template <class T> struct VjpArgs;
 // Tree version of the fused forward + reverse pass (pose gradients need one
 // user link per joint).
@@ -41,10 +57,6 @@ struct HeteroArgs {
   int32_t nseg;
 };
 hipError_t launch_hetero(const HeteroArgs &args, int64_t total_blocks, int flags, hipStream_t s);
-
-template <class T> struct KinOut {
-  T *poses = nullptr, *jfull = nullptr, *jw_links = nullptr, *jl_links = nullptr, *tau = nullptr;
-};
 
 constexpr int kMaxGenericJoints = 64;
 

@@ -126,6 +126,7 @@ Pose pose_from_object(const py::object &o) {
 struct DLIn {
   dlabi::ManagedTensor *mt = nullptr;
   void *ptr = nullptr;
+  std::vector<int64_t> strides;  // elements, filled by take_strided
   ~DLIn() {
     if (mt && mt->deleter) mt->deleter(mt);
   }
@@ -154,6 +155,39 @@ std::unique_ptr<DLIn> take(const py::object &t, uint8_t code, const std::vector<
       throw py::value_error("tensor must be contiguous");
     stride *= d.shape[i];
   }
+  in->ptr = static_cast<char *>(d.data) + d.byte_offset;
+  return in;
+}
+
+// A float32 GPU tensor whose last dimension is contiguous and whose other
+// dimensions may be strided (padded buffers, slices of them): shape entries
+// < 0 are free; returns the element strides.
+std::unique_ptr<DLIn> take_strided(const py::object &t, const std::vector<int64_t> &shape, int device) {
+  py::object cap = t.attr("__dlpack__")();
+  PyObject *c = cap.ptr();
+  if (!PyCapsule_IsValid(c, "dltensor")) throw py::type_error("__dlpack__ did not return a dltensor capsule");
+  auto *mt = static_cast<dlabi::ManagedTensor *>(PyCapsule_GetPointer(c, "dltensor"));
+  PyCapsule_SetName(c, "used_dltensor");
+  auto in = std::make_unique<DLIn>();
+  in->mt = mt;
+  const dlabi::Tensor &d = mt->dl_tensor;
+  if (d.device.device_type != dlabi::kROCM)
+    throw py::value_error("batched methods need a GPU tensor (DLPack device kDLROCM)");
+  if (d.device.device_id != device) throw py::value_error("tensor is on a different GPU than the model");
+  if (d.dtype.code != dlabi::kFloat || d.dtype.bits != 32 || d.dtype.lanes != 1)
+    throw py::value_error("strided buffers must be float32");
+  if (d.ndim != (int32_t)shape.size()) throw py::value_error("unexpected tensor rank");
+  in->strides.assign(d.ndim, 0);
+  int64_t dense = 1;
+  for (int i = d.ndim - 1; i >= 0; --i) {
+    if (shape[i] >= 0 && d.shape[i] != shape[i]) throw py::value_error("unexpected tensor shape");
+    in->strides[i] = d.strides ? d.strides[i] : dense;
+    dense *= d.shape[i];
+  }
+  if (d.ndim > 0 && d.shape[d.ndim - 1] > 1 && in->strides[d.ndim - 1] != 1)
+    throw py::value_error("the last dimension of a strided buffer must be contiguous");
+  for (int i = 0; i < d.ndim; ++i)
+    if (in->strides[i] < 0) throw py::value_error("negative strides are not supported");
   in->ptr = static_cast<char *>(d.data) + d.byte_offset;
   return in;
 }
@@ -350,6 +384,70 @@ class PyModel {
                               f(vQ), static_cast<int32_t *>(vS->ptr), f(vE), static_cast<int32_t *>(vI->ptr),
                               b.stream(stream)));
     return py::make_tuple(Q, S.attr("bool")(), E, I);
+  }
+
+  // ---------------------------------------------------------------- padded buffers, passive forces
+  // Writes into caller buffers (no allocation, no copies): poses (B, >= L, >= 7)
+  // with the last dimension contiguous, e.g. the first 7 columns of a
+  // [n_art, max_links, 13] body-state buffer; joint arrays (B, nv) with any row
+  // stride, e.g. [:, :nv] of a [n_art, max_dof] buffer.
+  void fk_jac_rnea_into(py::object q, py::object qd, py::object qdd, py::object poses, py::object J, py::object tau,
+                        py::object stream) {
+    sapamd_info i = info();
+    auto vq = take_strided(q, {-1, i.nv}, i.device);
+    const int64_t B = q.attr("shape").cast<std::vector<int64_t>>()[0];
+    std::unique_ptr<DLIn> vqd, vqdd, vP, vJ, vT;
+    if (!tau.is_none()) {
+      vT = take_strided(tau, {B, i.nv}, i.device);
+      vqd = take_strided(qd, {B, i.nv}, i.device);
+      vqdd = take_strided(qdd, {B, i.nv}, i.device);
+    }
+    if (!poses.is_none()) {
+      vP = take_strided(poses, {B, -1, -1}, i.device);
+      auto shp = poses.attr("shape").cast<std::vector<int64_t>>();
+      if (shp[1] < i.nlinks || shp[2] < 7) throw py::value_error("poses must have shape (B, >= nlinks, >= 7)");
+    }
+    if (!J.is_none()) vJ = take(J, dlabi::kFloat, {B, 6, i.nv}, i.device);
+    void *s = stream.is_none() ? current_stream(i.device) : reinterpret_cast<void *>(stream.cast<intptr_t>());
+    auto fp = [](const std::unique_ptr<DLIn> &v) { return v ? static_cast<float *>(v->ptr) : nullptr; };
+    check(sapamd_fk_jac_rnea_strided(m_, B, fp(vq), vq->strides[0], fp(vqd), vqd ? vqd->strides[0] : 0, fp(vqdd),
+                                     vqdd ? vqdd->strides[0] : 0, fp(vP), vP ? vP->strides[0] : 0,
+                                     vP ? vP->strides[1] : 0, fp(vJ), fp(vT), vT ? vT->strides[0] : 0, s));
+  }
+  py::object passive_force_batched(py::object q, py::object qd, bool gravity, bool coriolis, py::object out,
+                                   py::object stream) {
+    sapamd_info i = info();
+    py::module_ torch = py::module_::import("torch");
+    py::object dev = torch.attr("device")("cuda", i.device);
+    auto as_gpu = [&](py::object x) {
+      if (!py::isinstance(x, torch.attr("Tensor"))) x = torch.attr("as_tensor")(x);
+      x = x.attr("detach")().attr("to")(py::arg("device") = dev, py::arg("dtype") = torch.attr("float32"));
+      if (x.attr("dim")().cast<int>() == 1) x = x.attr("unsqueeze")(0);
+      return x;
+    };
+    q = as_gpu(q);
+    auto vq = take_strided(q, {-1, i.nv}, i.device);
+    const int64_t B = q.attr("shape").cast<std::vector<int64_t>>()[0];
+    std::unique_ptr<DLIn> vqd;
+    if (coriolis) {
+      if (qd.is_none()) throw py::value_error("qvel is needed for Coriolis and centrifugal forces");
+      qd = as_gpu(qd);
+      vqd = take_strided(qd, {B, i.nv}, i.device);
+    }
+    if (out.is_none())
+      out = torch.attr("empty")(py::make_tuple(B, i.nv), py::arg("device") = dev, py::arg("dtype") = torch.attr("float32"));
+    auto vo = take_strided(out, {B, i.nv}, i.device);
+    void *s = stream.is_none() ? current_stream(i.device) : reinterpret_cast<void *>(stream.cast<intptr_t>());
+    check(sapamd_passive_force_batched(m_, B, static_cast<float *>(vq->ptr), vq->strides[0],
+                                       vqd ? static_cast<float *>(vqd->ptr) : nullptr, vqd ? vqd->strides[0] : 0,
+                                       gravity ? 1 : 0, coriolis ? 1 : 0, static_cast<float *>(vo->ptr),
+                                       vo->strides[0], s));
+    return out;
+  }
+  static void *current_stream(int device) {
+    py::module_ torch = py::module_::import("torch");
+    py::object cs = torch.attr("cuda").attr("current_stream")(torch.attr("device")("cuda", device));
+    return reinterpret_cast<void *>(cs.attr("cuda_stream").cast<intptr_t>());
   }
 
   // ---------------------------------------------------------------- adjoints (DLPack, FP32)
@@ -822,6 +920,17 @@ PYBIND11_MODULE(_pinocchio, m) {
            py::arg("stream") = py::none(), "Adjoint of rollout_batched: (dL/dq0, dL/dqd0, dL/dqf)")
       .def("rollout_step_", &PyModel::rollout_step_, py::arg("qpos"), py::arg("qvel"), py::arg("qf"), py::arg("dt"),
            py::arg("stream") = py::none(), "One in-place rollout step (graph-capturable)")
+      .def("compute_fk_jacobian_rnea_into", &PyModel::fk_jac_rnea_into, py::arg("qpos"), py::arg("qvel") = py::none(),
+           py::arg("qacc") = py::none(), py::arg("poses") = py::none(), py::arg("jacobian") = py::none(),
+           py::arg("qf") = py::none(), py::arg("stream") = py::none(),
+           "Fused FK / WORLD Jacobian / RNEA into caller buffers, without copies: joint arrays (B, nv) with any "
+           "row stride (e.g. [:, :nv] of a padded [n_art, max_dof] state), poses (B, >= L, >= 7) with the last "
+           "dimension contiguous (e.g. a [n_art, max_links, 13] body-state buffer: p | q wxyz in the first 7).")
+      .def("compute_passive_force_batched", &PyModel::passive_force_batched, py::arg("qpos"),
+           py::arg("qvel") = py::none(), py::arg("gravity") = true, py::arg("coriolis_and_centrifugal") = true,
+           py::arg("out") = py::none(), py::arg("stream") = py::none(),
+           "Passive joint forces g(q) + C(q, qd) qd (either term optional), as "
+           "PhysxArticulation.compute_passive_force: RNEA(q, qd, 0). Inputs may be padded (row-strided).")
       .def("pose_torque_loss_batched", &PyModel::pose_torque_loss_batched, py::arg("qpos"),
            py::arg("qvel") = py::none(), py::arg("qacc") = py::none(), py::arg("tau_target") = py::none(),
            py::arg("ee_link") = 0, py::arg("p_target") = py::none(), py::arg("stream") = py::none(),

@@ -432,3 +432,69 @@ def test_large_joint_angles(scale):
         big = np.random.default_rng(5).uniform(-scale, scale, size=q.shape).astype(np.float32).astype(np.float64)
         q = np.where(rev[None, :], big, q)
         _check_kin(m, o, q, qd, qdd)
+
+
+@pytest.mark.parametrize("tree", TREES)
+def test_passive_force_batched(tree):
+    """compute_passive_force_batched = PhysxArticulation.compute_passive_force
+    semantics (src/physx/articulation.cpp:400-418; the reference's test equates
+    it with RNEA(q, 0, 0) at rest, unittest/test_physx/test_articulation.py:90-97):
+    g(q) + C(q, qd) qd, each term optional; inputs may be padded rows."""
+    import sapien_amd as sa
+
+    rob = S.robot(tree)
+    m, o = _pair(rob)
+    B, n = 1000, rob.nv
+    q, qd, _, _ = S.samples(rob, B, 1950)
+    z = np.zeros_like(q)
+    g_ref, full_ref = o.rnea(q, z, z), o.rnea(q, qd, z)
+    # padded [B, max_dof] state buffers, the model reads the first nv columns
+    qbuf = torch.full((B, n + 5), 7.0, device=DEV)
+    qdbuf = torch.full((B, n + 3), 7.0, device=DEV)
+    qbuf[:, :n] = _t(q)
+    qdbuf[:, :n] = _t(qd)
+    for v in ("auto", "thread_per_env"):
+        sa.set_kernel_variant(m, v)
+        both = m.compute_passive_force_batched(qbuf[:, :n], qdbuf[:, :n]).cpu().numpy()
+        grav = m.compute_passive_force_batched(_t(q), coriolis_and_centrifugal=False).cpu().numpy()
+        cori = m.compute_passive_force_batched(_t(q), _t(qd), gravity=False).cpu().numpy()
+        none = m.compute_passive_force_batched(_t(q), gravity=False, coriolis_and_centrifugal=False).cpu().numpy()
+        assert rel_err(both, full_ref) <= REL_TOL, v
+        assert rel_err(grav, g_ref) <= REL_TOL, v
+        assert rel_err(cori, full_ref - g_ref) <= REL_TOL, v
+        assert np.array_equal(none, np.zeros_like(none))
+        # at rest the passive force is the inverse dynamics of zero motion
+        rest = m.compute_passive_force_batched(_t(q), torch.zeros(B, n, device=DEV))
+        assert torch.equal(rest, m.compute_inverse_dynamics_batched(_t(q), torch.zeros(B, n, device=DEV),
+                                                                    torch.zeros(B, n, device=DEV)))
+
+
+@pytest.mark.parametrize("tree", TREES)
+def test_padded_simulator_buffers(tree):
+    """The fused kernel reads a padded [n_art, max_dof] joint state in place and
+    writes p | q wxyz into the first 7 floats of each link of a
+    [n_art, max_links, 13] body-state buffer (the layout of
+    src/physx/physx_system.cpp:1048-1101), leaving every other float alone."""
+    import sapien_amd as sa
+
+    rob = S.robot(tree)
+    m, o = _pair(rob)
+    B, n, L = 700, rob.nv, rob.nlinks
+    q, qd, qdd, _ = S.samples(rob, B, 1951)
+    max_dof, max_links = n + 4, L + 3
+    state = torch.full((4, B, max_dof), -3.0, device=DEV)  # qpos, qvel, qacc, qf blocks
+    state[0, :, :n], state[1, :, :n], state[2, :, :n] = _t(q), _t(qd), _t(qdd)
+    body = torch.full((B, max_links, 13), -5.0, device=DEV)
+    J = torch.empty(B, 6, n, device=DEV)
+    for v in ("auto", "thread_per_env"):
+        sa.set_kernel_variant(m, v)
+        body.fill_(-5.0)
+        state[3].fill_(-3.0)
+        m.compute_fk_jacobian_rnea_into(state[0, :, :n], state[1, :, :n], state[2, :, :n], poses=body, jacobian=J,
+                                        qf=state[3, :, :n])
+        P, Jr, T = m.compute_fk_jacobian_rnea_batched(_t(q), _t(qd), _t(qdd))
+        assert torch.equal(body[:, :L, :7], P) and torch.equal(J, Jr) and torch.equal(state[3, :, :n], T), v
+        assert bool((body[:, :, 7:] == -5.0).all()) and bool((body[:, L:, :] == -5.0).all())
+        assert bool((state[3, :, n:] == -3.0).all())
+        pos, ang = pose_errors(body[:, :L, :7].cpu().numpy(), o.link_poses(q))
+        assert pos <= POSE_TOL_M and ang <= POSE_TOL_RAD
