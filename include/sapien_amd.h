@@ -182,6 +182,18 @@ int sapamd_hetero_batched(const sapamd_model *const *models, int32_t nmodels, co
                           const float *const *q, const float *const *qd, const float *const *qdd,
                           float *const *poses, float *const *J, float *const *tau, float *const *M, void *stream);
 
+/* Multi-start CLIK (the mplib-style Planner.IK(target, init_qpos, n_init_qpos)
+ * use of compute_inverse_kinematics, docs/source/tutorial/motion_planning/
+ * inverse_kinematics.rst:7-19): B targets (B, 7), nstart starts each; q0 is
+ * (nstart * B, nv) start-major (start k of target t at row k * B + t), the
+ * outputs have the same layout. A start is skipped (success 0, iterations -1,
+ * error 1e30) once a lower start of its target converged; solved (B) receives
+ * the lowest converged start per target (INT32_MAX when none did). */
+int sapamd_clik_multistart_batched(const sapamd_model *m, int64_t B, int32_t nstart, int32_t link,
+                                   const float *target, const float *q0, const int32_t *mask, double eps,
+                                   int32_t max_iterations, double dt, double damp, float *q_out, int32_t *success,
+                                   float *err, int32_t *iterations, int32_t *solved, void *stream);
+
 /* CLIK problems of several models in ONE persistent launch (at most 8
  * models): model i solves B[i] problems for its link links[i], with targets[i]
  * (B[i], 7), optional q0[i] (B[i], nv_i) and masks[i] (nv_i), writing

@@ -1,6 +1,7 @@
 // C ABI implementation (declared in include/sapien_amd.h).
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -1094,6 +1095,43 @@ int sapamd_aba_batched(const sapamd_model *m, int64_t B, const float *q, const f
   });
 }
 
+namespace {
+// Batched CLIK over nstart x B problems (start-major); solved != nullptr turns
+// on the multi-start skip.
+void clik_launch(const sapamd_model *m, int64_t B, int64_t nstart, int32_t link, const float *target,
+                 const float *q0, const int32_t *mask, double eps, int32_t max_iterations, double dt, double damp,
+                 float *q_out, int32_t *success, float *err, int32_t *iterations, int32_t *solved, void *stream) {
+  ClikParams P;
+  P.link = link;
+  P.chain = m->d_chain + m->chain_off[link];
+  P.nchain = m->chain_len[link];
+  P.eps = eps;
+  P.dt = dt;
+  P.damp = damp;
+  P.max_iter = max_iterations;
+  P.mask = mask;
+  P.mask_ld = 0;
+  P.counter = m->d_counter;
+  P.ntargets = solved ? B : 0;
+  P.solved = solved;
+  const int64_t total = B * nstart;
+  int dev_cus = 256;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, m->device) == hipSuccess) dev_cus = prop.multiProcessorCount;
+  const int64_t waves = (total + 63) / 64;
+  // Persistent launch with a work queue once there are more problems than
+  // resident lanes (and always for multi-start, so that starts are taken in
+  // order): finished lanes pull the next problem. The queue counter belongs to
+  // the model, so overlapping batched CLIK calls on one model race.
+  P.persistent_waves = (waves > 4 * dev_cus || solved) ? (int)std::min<int64_t>(waves, 4 * dev_cus) : 0;
+  if (P.persistent_waves) HIP_OK(launch_fill_i32(m->d_counter, 0, 1, S(stream)));
+  else P.counter = nullptr;
+  if (solved) HIP_OK(launch_fill_i32(solved, INT32_MAX, B, S(stream)));
+  HIP_OK(launch_clik<float>(m->t32(), m->host->njoints(), total, P, target, q0, q_out, success, err, iterations,
+                            S(stream)));
+}
+}  // namespace
+
 int sapamd_clik_batched(const sapamd_model *m, int64_t B, int32_t link, const float *target, const float *q0,
                         const int32_t *mask, double eps, int32_t max_iterations, double dt, double damp,
                         float *q_out, int32_t *success, float *err, int32_t *iterations, void *stream) {
@@ -1105,29 +1143,25 @@ int sapamd_clik_batched(const sapamd_model *m, int64_t B, int32_t link, const fl
     need(q_out, "q_out");
     need(success, "success");
     need(err, "err");
-    ClikParams P;
-    P.link = link;
-    P.chain = m->d_chain + m->chain_off[link];
-    P.nchain = m->chain_len[link];
-    P.eps = eps;
-    P.dt = dt;
-    P.damp = damp;
-    P.max_iter = max_iterations;
-    P.mask = mask;
-    P.mask_ld = 0;
-    P.counter = m->d_counter;
-    int dev_cus = 256;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, m->device) == hipSuccess) dev_cus = prop.multiProcessorCount;
-    int64_t waves = (B + 63) / 64;
-    // Persistent launch with a work queue once there are more problems than
-    // resident lanes: finished lanes pull the next problem. The queue counter
-    // belongs to the model, so overlapping batched CLIK calls on one model race.
-    P.persistent_waves = waves > 4 * dev_cus ? 4 * dev_cus : 0;
-    if (P.persistent_waves) HIP_OK(launch_fill_i32(m->d_counter, 0, 1, S(stream)));
-    else P.counter = nullptr;
-    HIP_OK(launch_clik<float>(m->t32(), m->host->njoints(), B, P, target, q0, q_out, success, err, iterations,
-                              S(stream)));
+    clik_launch(m, B, 1, link, target, q0, mask, eps, max_iterations, dt, damp, q_out, success, err, iterations,
+                nullptr, stream);
+  });
+}
+
+int sapamd_clik_multistart_batched(const sapamd_model *m, int64_t B, int32_t nstart, int32_t link,
+                                   const float *target, const float *q0, const int32_t *mask, double eps,
+                                   int32_t max_iterations, double dt, double damp, float *q_out, int32_t *success,
+                                   float *err, int32_t *iterations, int32_t *solved, void *stream) {
+  return guarded([&] {
+    check_batched(m, B);
+    m->check_link(link);
+    if (nstart < 1) throw std::invalid_argument("at least one start per target");
+    if (B == 0) return;
+    for (const void *p : {(const void *)target, (const void *)q0, (const void *)q_out, (const void *)success,
+                          (const void *)err, (const void *)solved})
+      need(p, "multi-start buffer");
+    clik_launch(m, B, nstart, link, target, q0, mask, eps, max_iterations, dt, damp, q_out, success, err, iterations,
+                solved, stream);
   });
 }
 

@@ -477,11 +477,22 @@ __device__ void clik_problem(const ModelTable<T> *__restrict__ M, const ClikPara
   const int nv = M->nv, nc = P.nchain;
   const JointRec<T> *JR = joints_of(M);
   const LinkRec<T> &Lk = links_of(M)[P.link];
+  const int64_t tgt = P.ntargets ? prob % P.ntargets : prob;
+  const int32_t start = P.ntargets ? (int32_t)(prob / P.ntargets) : 0;
+  if (P.solved && __hip_atomic_load(P.solved + tgt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < start) {
+    // a lower start already converged for this target: skip
+    T *qo = q_out + prob * nv;
+    for (int u = 0; u < nv; ++u) qo[u] = q0 ? q0[prob * nv + u] : T(0);
+    success[prob] = 0;
+    for (int r = 0; r < 6; ++r) err_out[prob * 6 + r] = T(1e30);
+    if (iters) iters[prob] = -1;
+    return;
+  }
   {
     // desired pose of the link's joint frame: oMdes = target * link2joint^-1
     T Rt[9], Rd[9];
     V3<T> pt;
-    pose_to_mat(target + prob * 7, Rt, pt);
+    pose_to_mat(target + tgt * 7, Rt, pt);
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j)
         Rd[3 * i + j] = Rt[3 * i] * Lk.R[3 * j] + Rt[3 * i + 1] * Lk.R[3 * j + 1] + Rt[3 * i + 2] * Lk.R[3 * j + 2];
@@ -581,6 +592,7 @@ __device__ void clik_problem(const ModelTable<T> *__restrict__ M, const ClikPara
     success[prob] = ok ? 1 : 0;
     for (int r = 0; r < 6; ++r) err_out[prob * 6 + r] = bestErr[r];
     if (iters) iters[prob] = it;
+    if (ok && P.solved) __hip_atomic_fetch_min(P.solved + tgt, start, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 

@@ -86,6 +86,12 @@ struct ClikParams {
   int64_t mask_ld;      // 0 = shared mask, nv = per problem
   int32_t *counter;     // work-queue counter, zeroed before the launch
   int persistent_waves; // 0 = one lane per problem
+  // Multi-start: problem p solves target p % ntargets from start p / ntargets
+  // (start-major order); solved[t] holds the lowest start index that converged
+  // for target t (INT32_MAX before), and a problem whose target already
+  // converged from a lower start is skipped (success 0, iterations -1).
+  int64_t ntargets = 0;  // 0 = one target per problem
+  int32_t *solved = nullptr;
 };
 // CLIK (pinocchio_model.cpp:228-300) for B problems sharing one target link.
 // target (B, 7) p | wxyz; q0 (B, nv) or nullptr (neutral); outputs q (B, nv),

@@ -450,6 +450,35 @@ class PyModel {
     return reinterpret_cast<void *>(cs.attr("cuda_stream").cast<intptr_t>());
   }
 
+  // Multi-start CLIK, raw: q0 (nstart * B, nv) start-major; returns per start
+  // (q, success, error, iterations) and the lowest converged start per target.
+  py::tuple clik_multistart_raw(int64_t link, py::object target, py::object q0, int nstart, py::object mask,
+                                double eps, int max_iterations, double dt, double damp, py::object stream) {
+    Batch b(*this, target, 7);
+    const int64_t total = b.B * nstart;
+    py::object Q0 = b.torch.attr("as_tensor")(q0).attr("to")(py::arg("device") = b.dev,
+                                                                py::arg("dtype") = b.torch.attr("float32"));
+    Q0 = Q0.attr("contiguous")();
+    auto vq0 = take(Q0, dlabi::kFloat, {total, b.n}, b.device);
+    py::object Q = b.empty({total, b.n}), E = b.empty({total, 6});
+    py::object Sx = b.empty_i32({total}), I = b.empty_i32({total}), solved = b.empty_i32({b.B});
+    py::object maskT = py::none();
+    std::unique_ptr<DLIn> vmask;
+    if (!mask.is_none()) {
+      maskT = b.input_i32(mask, b.n);
+      vmask = take(maskT, dlabi::kInt, {b.n}, b.device);
+    }
+    auto vt = b.view(b.q), vQ = b.view(Q), vE = b.view(E);
+    auto vS = take(Sx, dlabi::kInt, {total}, b.device), vI = take(I, dlabi::kInt, {total}, b.device),
+         vs = take(solved, dlabi::kInt, {b.B}, b.device);
+    check(sapamd_clik_multistart_batched(m_, b.B, nstart, (int32_t)clamp_index(link), f(vt), f(vq0),
+                                         vmask ? static_cast<int32_t *>(vmask->ptr) : nullptr, eps, max_iterations,
+                                         dt, damp, f(vQ), static_cast<int32_t *>(vS->ptr), f(vE),
+                                         static_cast<int32_t *>(vI->ptr), static_cast<int32_t *>(vs->ptr),
+                                         b.stream(stream)));
+    return py::make_tuple(Q, Sx, E, I, solved);
+  }
+
   // ---------------------------------------------------------------- adjoints (DLPack, FP32)
   py::tuple kin_vjp_batched(py::object q, py::object qd, py::object qdd, py::object gposes, py::object gtau,
                             py::object stream) {
@@ -931,6 +960,10 @@ PYBIND11_MODULE(_pinocchio, m) {
            py::arg("out") = py::none(), py::arg("stream") = py::none(),
            "Passive joint forces g(q) + C(q, qd) qd (either term optional), as "
            "PhysxArticulation.compute_passive_force: RNEA(q, qd, 0). Inputs may be padded (row-strided).")
+      .def("_clik_multistart_raw", &PyModel::clik_multistart_raw, py::arg("link_index"), py::arg("poses"),
+           py::arg("initial_qpos"), py::arg("n_start"), py::arg("active_qmask") = py::none(), py::arg("eps") = 1e-4,
+           py::arg("max_iterations") = 1000, py::arg("dt") = 0.1, py::arg("damp") = 1e-6,
+           py::arg("stream") = py::none())
       .def("pose_torque_loss_batched", &PyModel::pose_torque_loss_batched, py::arg("qpos"),
            py::arg("qvel") = py::none(), py::arg("qacc") = py::none(), py::arg("tau_target") = py::none(),
            py::arg("ee_link") = 0, py::arg("p_target") = py::none(), py::arg("stream") = py::none(),

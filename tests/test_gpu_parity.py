@@ -498,3 +498,41 @@ def test_padded_simulator_buffers(tree):
         assert bool((state[3, :, n:] == -3.0).all())
         pos, ang = pose_errors(body[:, :L, :7].cpu().numpy(), o.link_poses(q))
         assert pos <= POSE_TOL_M and ang <= POSE_TOL_RAD
+
+
+def test_multistart_ik():
+    """Multi-start batched IK (mplib's Planner.IK(target, init_qpos,
+    n_init_qpos)): the answer is the lowest start that converges, identical to
+    running the starts one after the other with the single-start batched
+    CLIK; targets no start reaches keep the smallest-error start."""
+    import sapien_amd as sa
+
+    rob = S.arm9()
+    m, o = _pair(rob)
+    link, B, n_init = rob.ee_link, 256, 6
+    qs = S.samples(rob, B, 1960)[0]
+    t = o.link_poses(qs)[:, link]
+    t[::17, :3] += 5.0  # a few unreachable targets
+    tt = _t(t)
+    Q, ok, err, start = m.compute_inverse_kinematics_multistart_batched(link, tt, n_init_qpos=n_init,
+                                                                        max_iterations=300, seed=3)
+    # the same starts, one launch per start
+    gen = torch.Generator(device=DEV)
+    gen.manual_seed(3)
+    lo, hi = (torch.as_tensor(x, dtype=torch.float32, device=DEV) for x in m.joint_limits())
+    starts = torch.empty(1 + n_init, B, rob.nv, device=DEV)
+    starts[0] = 0.0
+    starts[1:] = lo + (hi - lo) * torch.rand(n_init, B, rob.nv, generator=gen, device=DEV)
+    runs = [m.compute_inverse_kinematics_batched(link, tt, starts[k], max_iterations=300) for k in range(1 + n_init)]
+    oks = torch.stack([r[1] for r in runs])
+    first = torch.where(oks.any(0), oks.float().argmax(0), torch.stack([r[2].norm(dim=1) for r in runs]).argmin(0))
+    assert torch.equal(start, first)
+    for b in range(B):
+        k = int(start[b])
+        assert torch.equal(Q[b], runs[k][0][b]) and bool(ok[b]) == bool(runs[k][1][b])
+        assert torch.equal(err[b], runs[k][2][b])
+    unreachable = np.zeros(B, bool)
+    unreachable[::17] = True
+    assert not ok.cpu().numpy()[unreachable].any()
+    assert ok.cpu().numpy()[~unreachable].mean() > 0.9
+    assert oks[0].float().mean() <= ok.float().mean()  # extra starts never hurt
