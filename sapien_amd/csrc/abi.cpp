@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -91,6 +92,26 @@ void mm3(const double *A, const double *B, double *C) {
     for (int j = 0; j < 3; ++j) C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
 }
 
+// One stream per device for every model's per-call (one-configuration) work,
+// created on first use and kept for the process. Per-call methods synchronize
+// before returning, so models can share it. A private segment is reserved per
+// hardware queue, for good, from an agent-wide scratch budget shared by all
+// queues (hsa_ext_amd.h, HSA_AMD_AGENT_INFO_SCRATCH_LIMIT_MAX/_CURRENT): one
+// stream for all models keeps this library's large-private-segment FP64
+// kernels on one queue instead of spreading their reservations over every
+// queue the streams of many models would map to.
+hipStream_t shared_stream(int device) {
+  static std::mutex mu;
+  static std::vector<hipStream_t> streams;
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)streams.size() <= device) streams.resize(device + 1, nullptr);
+  if (!streams[device]) {
+    DeviceGuard g(device);
+    HIP_OK(hipStreamCreate(&streams[device]));
+  }
+  return streams[device];
+}
+
 }  // namespace
 
 struct sapamd_model {
@@ -100,7 +121,7 @@ struct sapamd_model {
   int32_t *d_chain = nullptr;   // every link's support chain, concatenated
   std::vector<int> chain_off, chain_len;
   int32_t *d_counter = nullptr;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // the device's shared per-call stream (not owned)
   double *h_buf = nullptr, *d_buf = nullptr;
   size_t buf_doubles = 0;
   bool tables_ready = false;
@@ -127,7 +148,6 @@ struct sapamd_model {
     if (d_counter) (void)hipFree(d_counter);
     if (d_buf) (void)hipFree(d_buf);
     if (h_buf) (void)hipHostFree(h_buf);
-    if (stream) (void)hipStreamDestroy(stream);
   }
 
   int nv() const { return host->nv(); }
@@ -520,7 +540,7 @@ int sapamd_model_create(const char *urdf, const double gravity[3], sapamd_model 
       m->device = -1;
     } else {
       HIP_OK(hipGetDevice(&m->device));
-      HIP_OK(hipStreamCreate(&m->stream));
+      m->stream = shared_stream(m->device);
       HIP_OK(hipMalloc(&m->d_counter, sizeof(int32_t) * 4));
     }
     m->reset_caches();

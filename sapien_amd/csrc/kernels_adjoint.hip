@@ -149,7 +149,9 @@ __global__ __launch_bounds__(64) void k_kin_vjp(const ModelTable<T> *__restrict_
   if (lossmode && A.loss) A.loss[b] = loss;
 }
 
-inline int pick_mj(int nj) { return nj <= 8 ? 8 : nj <= 16 ? 16 : nj <= 32 ? 32 : 64; }
+inline int pick_mj(int nj) {
+  return nj <= 8 ? 8 : nj <= 16 ? 16 : nj <= 24 ? 24 : nj <= 32 ? 32 : nj <= 48 ? 48 : 64;
+}
 
 }  // namespace
 
@@ -161,7 +163,9 @@ hipError_t launch_kin_vjp(const ModelTable<T> *tab, int nj, int64_t B, const T *
   switch (pick_mj(nj)) {
     case 8: hipLaunchKernelGGL((k_kin_vjp<T, 8>), grid, dim3(64), 0, s, tab, B, q, qd, qdd, a); break;
     case 16: hipLaunchKernelGGL((k_kin_vjp<T, 16>), grid, dim3(64), 0, s, tab, B, q, qd, qdd, a); break;
+    case 24: hipLaunchKernelGGL((k_kin_vjp<T, 24>), grid, dim3(64), 0, s, tab, B, q, qd, qdd, a); break;
     case 32: hipLaunchKernelGGL((k_kin_vjp<T, 32>), grid, dim3(64), 0, s, tab, B, q, qd, qdd, a); break;
+    case 48: hipLaunchKernelGGL((k_kin_vjp<T, 48>), grid, dim3(64), 0, s, tab, B, q, qd, qdd, a); break;
     default: hipLaunchKernelGGL((k_kin_vjp<T, 64>), grid, dim3(64), 0, s, tab, B, q, qd, qdd, a); break;
   }
   return hipGetLastError();

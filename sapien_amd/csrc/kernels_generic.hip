@@ -25,13 +25,24 @@ template <class T> __device__ __forceinline__ S6<T> gravity_acc(const ModelTable
   return {{-M->gravity[0], -M->gravity[1], -M->gravity[2]}, {T(0), T(0), T(0)}};
 }
 
+// Per-joint arrays of the thread-per-environment kernels. LDS = false: private
+// arrays (scratch), one environment per lane. LDS = true: the arrays live in
+// the block's LDS, for one-environment launches (the per-call path, one
+// thread per block): a private segment is reserved per hardware queue for good
+// from an agent-wide budget (hsa_ext_amd.h, SCRATCH_LIMIT_MAX), and FP64 arrays
+// of 17-64 joints would reserve tens of GB of it.
+#define SAPAMD_JARR(TYPE, NAME)            \
+  __shared__ TYPE NAME##_s[LDS ? MJ : 1]; \
+  TYPE NAME##_l[LDS ? 1 : MJ];            \
+  TYPE *const NAME = LDS ? NAME##_s : NAME##_l;
+
 // Body (local) frame image of a world motion vector under link pose (R, p):
 // Ad(M)^-1 m = [R^T (l - p x a); R^T a].
 template <class T> __device__ __forceinline__ S6<T> to_local(const T *R, V3<T> p, S6<T> m) {
   return {mtv(R, m.l - cross(p, m.a)), mtv(R, m.a)};
 }
 
-template <class T, int MJ>
+template <class T, int MJ, bool LDS>
 __global__ __launch_bounds__(64) void k_kinematics(const ModelTable<T> *__restrict__ M, int64_t B, int flags,
                                                    const T *__restrict__ q, const T *__restrict__ qd,
                                                    const T *__restrict__ qdd, KinOut<T> out) {
@@ -40,9 +51,13 @@ __global__ __launch_bounds__(64) void k_kinematics(const ModelTable<T> *__restri
   const int nj = M->nj, nv = M->nv, L = M->nlinks;
   const JointRec<T> *JR = joints_of(M);
   const LinkRec<T> *LR = links_of(M);
-  T oR[MJ][9];
-  V3<T> op[MJ];
-  S6<T> col[MJ], vel[MJ], acc[MJ], F[MJ];
+  using R9 = T[9];
+  SAPAMD_JARR(R9, oR)
+  SAPAMD_JARR(V3<T>, op)
+  SAPAMD_JARR(S6<T>, col)
+  SAPAMD_JARR(S6<T>, vel)
+  SAPAMD_JARR(S6<T>, acc)
+  SAPAMD_JARR(S6<T>, F)
   const bool dyn = flags & KF_RNEA;
   const S6<T> a0 = (flags & KF_NOGRAV) ? zero6<T>() : gravity_acc(M);
   const T *qb = q + b * (out.ld_q ? out.ld_q : nv);
@@ -114,7 +129,7 @@ __global__ __launch_bounds__(64) void k_kinematics(const ModelTable<T> *__restri
   }
 }
 
-template <class T, int MJ>
+template <class T, int MJ, bool LDS>
 __global__ __launch_bounds__(64) void k_link_local_jacobian(const ModelTable<T> *__restrict__ M, int64_t B, int link,
                                                             const T *__restrict__ q, T *__restrict__ Jout) {
   int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -122,8 +137,9 @@ __global__ __launch_bounds__(64) void k_link_local_jacobian(const ModelTable<T> 
   const int nj = M->nj, nv = M->nv;
   const JointRec<T> *JR = joints_of(M);
   const LinkRec<T> &Lk = links_of(M)[link];
-  T oR[MJ][9];
-  V3<T> op[MJ];
+  using R9 = T[9];
+  SAPAMD_JARR(R9, oR)
+  SAPAMD_JARR(V3<T>, op)
   const T *qb = q + b * nv;
   for (int j = 0; j < nj; ++j) {
     const JointRec<T> &J = JR[j];
@@ -147,17 +163,18 @@ __global__ __launch_bounds__(64) void k_link_local_jacobian(const ModelTable<T> 
   }
 }
 
-template <class T, int MJ>
+template <class T, int MJ, bool LDS>
 __global__ __launch_bounds__(64) void k_crba(const ModelTable<T> *__restrict__ M, int64_t B, const T *__restrict__ q,
                                              T *__restrict__ Mout) {
   int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (b >= B) return;
   const int nj = M->nj, nv = M->nv;
   const JointRec<T> *JR = joints_of(M);
-  T oR[MJ][9];
-  V3<T> op[MJ];
-  S6<T> col[MJ];
-  AInertia<T> Yc[MJ];
+  using R9 = T[9];
+  SAPAMD_JARR(R9, oR)
+  SAPAMD_JARR(V3<T>, op)
+  SAPAMD_JARR(S6<T>, col)
+  SAPAMD_JARR(AInertia<T>, Yc)
   const T *qb = q + b * nv;
   for (int j = 0; j < nj; ++j) {
     const JointRec<T> &J = JR[j];
@@ -229,7 +246,7 @@ template <class T> __device__ __forceinline__ void vec_of(S6<T> x, T *v) {
   v[0] = x.l.x; v[1] = x.l.y; v[2] = x.l.z; v[3] = x.a.x; v[4] = x.a.y; v[5] = x.a.z;
 }
 
-template <class T, int MJ>
+template <class T, int MJ, bool LDS>
 __global__ __launch_bounds__(64) void k_aba(const ModelTable<T> *__restrict__ M, int64_t B, const T *__restrict__ q,
                                             const T *__restrict__ qd, const T *__restrict__ tau,
                                             T *__restrict__ qdd) {
@@ -237,11 +254,17 @@ __global__ __launch_bounds__(64) void k_aba(const ModelTable<T> *__restrict__ M,
   if (b >= B) return;
   const int nj = M->nj, nv = M->nv;
   const JointRec<T> *JR = joints_of(M);
-  T oR[MJ][9];
-  V3<T> op[MJ];
-  S6<T> col[MJ], vel[MJ], cb[MJ], pA[MJ], U[MJ];
-  Mat6<T> IA[MJ];
-  T D[MJ], uu[MJ];
+  using R9 = T[9];
+  SAPAMD_JARR(R9, oR)
+  SAPAMD_JARR(V3<T>, op)
+  SAPAMD_JARR(S6<T>, col)
+  SAPAMD_JARR(S6<T>, vel)
+  SAPAMD_JARR(S6<T>, cb)
+  SAPAMD_JARR(S6<T>, pA)
+  SAPAMD_JARR(S6<T>, U)
+  SAPAMD_JARR(Mat6<T>, IA)
+  SAPAMD_JARR(T, D)
+  SAPAMD_JARR(T, uu)
   const T *qb = q + b * nv;
   for (int j = 0; j < nj; ++j) {
     const JointRec<T> &J = JR[j];
@@ -275,7 +298,7 @@ __global__ __launch_bounds__(64) void k_aba(const ModelTable<T> *__restrict__ M,
     }
   }
   const S6<T> a0 = gravity_acc(M);
-  S6<T> acc[MJ];
+  SAPAMD_JARR(S6<T>, acc)
   for (int j = 0; j < nj; ++j) {
     int p = JR[j].parent;
     acc[j] = (p >= 0 ? acc[p] : a0) + cb[j];
@@ -307,7 +330,7 @@ template <class T> __device__ Mat6<T> coriolis_body(const Mat6<T> &I, S6<T> v) {
   return out;
 }
 
-template <class T, int MJ>
+template <class T, int MJ, bool LDS>
 __global__ __launch_bounds__(64) void k_coriolis(const ModelTable<T> *__restrict__ M, int64_t B,
                                                  const T *__restrict__ q, const T *__restrict__ qd,
                                                  T *__restrict__ Cout) {
@@ -315,10 +338,14 @@ __global__ __launch_bounds__(64) void k_coriolis(const ModelTable<T> *__restrict
   if (b >= B) return;
   const int nj = M->nj, nv = M->nv;
   const JointRec<T> *JR = joints_of(M);
-  T oR[MJ][9];
-  V3<T> op[MJ];
-  S6<T> col[MJ], dcol[MJ], vel[MJ];
-  Mat6<T> Ic[MJ], Bc[MJ];
+  using R9 = T[9];
+  SAPAMD_JARR(R9, oR)
+  SAPAMD_JARR(V3<T>, op)
+  SAPAMD_JARR(S6<T>, col)
+  SAPAMD_JARR(S6<T>, dcol)
+  SAPAMD_JARR(S6<T>, vel)
+  SAPAMD_JARR(Mat6<T>, Ic)
+  SAPAMD_JARR(Mat6<T>, Bc)
   const T *qb = q + b * nv;
   for (int j = 0; j < nj; ++j) {
     const JointRec<T> &J = JR[j];
@@ -627,14 +654,43 @@ __global__ void k_fill_i32(int32_t *dst, int32_t value, int64_t n) {
   if (i < n) dst[i] = value;
 }
 
-inline int pick_mj(int nj) { return nj <= 8 ? 8 : nj <= 16 ? 16 : nj <= 32 ? 32 : 64; }
+// Per-joint arrays are sized by the joint-count class MJ; the private segment
+// (scratch) of a thread-per-environment kernel grows with MJ, so the classes
+// are kept tight.
+inline int pick_mj(int nj) {
+  return nj <= 8 ? 8 : nj <= 16 ? 16 : nj <= 24 ? 24 : nj <= 32 ? 32 : nj <= 48 ? 48 : 64;
+}
 
 #define SAPAMD_DISPATCH_MJ(nj, KERNEL, ...)                                                      \
   switch (pick_mj(nj)) {                                                                         \
     case 8: hipLaunchKernelGGL((KERNEL<T, 8>), __VA_ARGS__); break;                             \
     case 16: hipLaunchKernelGGL((KERNEL<T, 16>), __VA_ARGS__); break;                           \
+    case 24: hipLaunchKernelGGL((KERNEL<T, 24>), __VA_ARGS__); break;                           \
     case 32: hipLaunchKernelGGL((KERNEL<T, 32>), __VA_ARGS__); break;                           \
+    case 48: hipLaunchKernelGGL((KERNEL<T, 48>), __VA_ARGS__); break;                           \
     default: hipLaunchKernelGGL((KERNEL<T, 64>), __VA_ARGS__); break;                           \
+  }
+// Kernels with per-joint arrays: one environment -> LDS arrays, one thread.
+#define SAPAMD_DISPATCH_MJ_ARR(nj, B, KERNEL, s, ...)                                                     \
+  if ((B) == 1) {                                                                                          \
+    switch (pick_mj(nj)) {                                                                                 \
+      case 8: hipLaunchKernelGGL((KERNEL<T, 8, true>), dim3(1), dim3(1), 0, s, __VA_ARGS__); break;       \
+      case 16: hipLaunchKernelGGL((KERNEL<T, 16, true>), dim3(1), dim3(1), 0, s, __VA_ARGS__); break;     \
+      case 24: hipLaunchKernelGGL((KERNEL<T, 24, true>), dim3(1), dim3(1), 0, s, __VA_ARGS__); break;     \
+      case 32: hipLaunchKernelGGL((KERNEL<T, 32, true>), dim3(1), dim3(1), 0, s, __VA_ARGS__); break;     \
+      case 48: hipLaunchKernelGGL((KERNEL<T, 48, true>), dim3(1), dim3(1), 0, s, __VA_ARGS__); break;     \
+      default: hipLaunchKernelGGL((KERNEL<T, 64, true>), dim3(1), dim3(1), 0, s, __VA_ARGS__); break;     \
+    }                                                                                                      \
+  } else {                                                                                                 \
+    const dim3 g_ = grid_for(B);                                                                           \
+    switch (pick_mj(nj)) {                                                                                 \
+      case 8: hipLaunchKernelGGL((KERNEL<T, 8, false>), g_, dim3(64), 0, s, __VA_ARGS__); break;          \
+      case 16: hipLaunchKernelGGL((KERNEL<T, 16, false>), g_, dim3(64), 0, s, __VA_ARGS__); break;        \
+      case 24: hipLaunchKernelGGL((KERNEL<T, 24, false>), g_, dim3(64), 0, s, __VA_ARGS__); break;        \
+      case 32: hipLaunchKernelGGL((KERNEL<T, 32, false>), g_, dim3(64), 0, s, __VA_ARGS__); break;        \
+      case 48: hipLaunchKernelGGL((KERNEL<T, 48, false>), g_, dim3(64), 0, s, __VA_ARGS__); break;        \
+      default: hipLaunchKernelGGL((KERNEL<T, 64, false>), g_, dim3(64), 0, s, __VA_ARGS__); break;        \
+    }                                                                                                      \
   }
 
 inline dim3 grid_for(int64_t B) { return dim3((unsigned)((B + 63) / 64)); }
@@ -651,34 +707,34 @@ template <class T>
 hipError_t launch_kinematics(const ModelTable<T> *tab, int nj, int64_t B, int flags, const T *q, const T *qd,
                              const T *qdd, KinOut<T> out, hipStream_t s) {
   if (B <= 0) return hipSuccess;
-  SAPAMD_DISPATCH_MJ(nj, k_kinematics, grid_for(B), dim3(64), 0, s, tab, B, flags, q, qd, qdd, out);
+  SAPAMD_DISPATCH_MJ_ARR(nj, B, k_kinematics, s, tab, B, flags, q, qd, qdd, out);
   return hipGetLastError();
 }
 template <class T>
 hipError_t launch_link_local_jacobian(const ModelTable<T> *tab, int nj, int64_t B, int link, const T *q, T *J,
                                       hipStream_t s) {
   if (B <= 0) return hipSuccess;
-  SAPAMD_DISPATCH_MJ(nj, k_link_local_jacobian, grid_for(B), dim3(64), 0, s, tab, B, link, q, J);
+  SAPAMD_DISPATCH_MJ_ARR(nj, B, k_link_local_jacobian, s, tab, B, link, q, J);
   return hipGetLastError();
 }
 template <class T>
 hipError_t launch_crba(const ModelTable<T> *tab, int nj, int64_t B, const T *q, T *M, hipStream_t s) {
   if (B <= 0) return hipSuccess;
-  SAPAMD_DISPATCH_MJ(nj, k_crba, grid_for(B), dim3(64), 0, s, tab, B, q, M);
+  SAPAMD_DISPATCH_MJ_ARR(nj, B, k_crba, s, tab, B, q, M);
   return hipGetLastError();
 }
 template <class T>
 hipError_t launch_coriolis(const ModelTable<T> *tab, int nj, int64_t B, const T *q, const T *qd, T *C,
                            hipStream_t s) {
   if (B <= 0) return hipSuccess;
-  SAPAMD_DISPATCH_MJ(nj, k_coriolis, grid_for(B), dim3(64), 0, s, tab, B, q, qd, C);
+  SAPAMD_DISPATCH_MJ_ARR(nj, B, k_coriolis, s, tab, B, q, qd, C);
   return hipGetLastError();
 }
 template <class T>
 hipError_t launch_aba(const ModelTable<T> *tab, int nj, int64_t B, const T *q, const T *qd, const T *tau, T *qdd,
                       hipStream_t s) {
   if (B <= 0) return hipSuccess;
-  SAPAMD_DISPATCH_MJ(nj, k_aba, grid_for(B), dim3(64), 0, s, tab, B, q, qd, tau, qdd);
+  SAPAMD_DISPATCH_MJ_ARR(nj, B, k_aba, s, tab, B, q, qd, tau, qdd);
   return hipGetLastError();
 }
 template <class T>

@@ -337,7 +337,7 @@ __global__ __launch_bounds__(64) void k_fd_ldlt(const ModelTable<double> *__rest
   for (int i = 0; i < nv; ++i) qdd[b * nv + i] = out[i];
 }
 
-inline int pick(int nj) { return nj <= 8 ? 8 : nj <= 16 ? 16 : nj <= 32 ? 32 : 64; }
+inline int pick(int nj) { return nj <= 8 ? 8 : nj <= 16 ? 16 : nj <= 24 ? 24 : nj <= 32 ? 32 : nj <= 48 ? 48 : 64; }
 
 __global__ void k_restore(int64_t B, int nv, const float *__restrict__ src, int64_t stride, float *__restrict__ q,
                           float *__restrict__ qd) {
@@ -363,8 +363,12 @@ hipError_t launch_restore_state(int64_t B, int nv, const float *src, int64_t str
     case 17: hipLaunchKernelGGL((KERNEL<T, 8, true>), grid, dim3(64), 0, s, __VA_ARGS__); break;       \
     case 32: hipLaunchKernelGGL((KERNEL<T, 16, false>), grid, dim3(64), 0, s, __VA_ARGS__); break;     \
     case 33: hipLaunchKernelGGL((KERNEL<T, 16, true>), grid, dim3(64), 0, s, __VA_ARGS__); break;      \
+    case 48: hipLaunchKernelGGL((KERNEL<T, 24, false>), grid, dim3(64), 0, s, __VA_ARGS__); break;     \
+    case 49: hipLaunchKernelGGL((KERNEL<T, 24, true>), grid, dim3(64), 0, s, __VA_ARGS__); break;      \
     case 64: hipLaunchKernelGGL((KERNEL<T, 32, false>), grid, dim3(64), 0, s, __VA_ARGS__); break;     \
     case 65: hipLaunchKernelGGL((KERNEL<T, 32, true>), grid, dim3(64), 0, s, __VA_ARGS__); break;      \
+    case 96: hipLaunchKernelGGL((KERNEL<T, 48, false>), grid, dim3(64), 0, s, __VA_ARGS__); break;     \
+    case 97: hipLaunchKernelGGL((KERNEL<T, 48, true>), grid, dim3(64), 0, s, __VA_ARGS__); break;      \
     case 128: hipLaunchKernelGGL((KERNEL<T, 64, false>), grid, dim3(64), 0, s, __VA_ARGS__); break;    \
     default: hipLaunchKernelGGL((KERNEL<T, 64, true>), grid, dim3(64), 0, s, __VA_ARGS__); break;      \
   }
@@ -397,6 +401,8 @@ hipError_t launch_fd_ldlt(const ModelTable<double> *tab64, int nj, int64_t B, co
   switch (pick(nj)) {
     case 8: hipLaunchKernelGGL((k_fd_ldlt<8>), grid, dim3(64), 0, s, tab64, B, q, qd, tau, qdd); break;
     case 16: hipLaunchKernelGGL((k_fd_ldlt<16>), grid, dim3(64), 0, s, tab64, B, q, qd, tau, qdd); break;
+    case 24: hipLaunchKernelGGL((k_fd_ldlt<24>), grid, dim3(64), 0, s, tab64, B, q, qd, tau, qdd); break;
+    case 48: hipLaunchKernelGGL((k_fd_ldlt<48>), grid, dim3(64), 0, s, tab64, B, q, qd, tau, qdd); break;
     case 32: hipLaunchKernelGGL((k_fd_ldlt<32>), grid, dim3(64), 0, s, tab64, B, q, qd, tau, qdd); break;
     default: hipLaunchKernelGGL((k_fd_ldlt<64>), grid, dim3(64), 0, s, tab64, B, q, qd, tau, qdd); break;
   }

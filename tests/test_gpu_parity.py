@@ -536,3 +536,64 @@ def test_multistart_ik():
     assert not ok.cpu().numpy()[unreachable].any()
     assert ok.cpu().numpy()[~unreachable].mean() > 0.9
     assert oks[0].float().mean() <= ok.float().mean()  # extra starts never hurt
+
+
+def _robot_file(fname):
+    import xml.etree.ElementTree as ET
+
+    text = open(os.path.join(os.path.dirname(__file__), "golden", "robots", fname)).read()
+    root = ET.fromstring(text)
+    jn = [j.get("name") for j in root.findall("joint") if j.get("type") != "fixed"]
+    ln = [l.get("name") for l in root.findall("link")]
+    return text, jn, ln
+
+
+ROBOT_FILES = sorted(f for f in os.listdir(os.path.join(os.path.dirname(__file__), "golden", "robots"))
+                     if f.endswith(".urdf"))
+
+
+@pytest.mark.parametrize("fname", ROBOT_FILES)
+def test_real_robot_parity(fname):
+    """SAPIEN's robot assets (fixed-joint chains, rpy inertial origins,
+    continuous wheels, prismatic joints, massless links): the drop-in FP64
+    per-call methods and the batched FP32 kernels against the oracle."""
+    import sapien_amd as sa
+
+    text, jn, ln = _robot_file(fname)
+    m = sa.PinocchioModel(text, [0.0, 0.0, -9.81])
+    m.set_joint_order(jn)
+    m.set_link_order(ln)
+    o = OracleModel(text)
+    o.set_joint_order(jn)
+    o.set_link_order(ln)
+    n, B = len(jn), 512
+    rng = np.random.default_rng(31)
+    lo, hi = m.joint_limits()
+    q = (lo + (hi - lo) * rng.uniform(size=(B, n))).astype(np.float32).astype(np.float64)
+    qd = rng.normal(size=(B, n)).astype(np.float32).astype(np.float64)
+    qdd = rng.normal(size=(B, n)).astype(np.float32).astype(np.float64)
+    massive = np.abs(o.crba(q[:1])).max() > 0
+    # forward dynamics only where M is positive definite (lwr and the gripper
+    # have joints that move no mass: M is singular and ABA divides by zero)
+    spd = massive and np.linalg.eigvalsh(o.crba(q[:16])).min() > 1e-12
+    # drop-in FP64, one configuration per call
+    for b in range(3):
+        m.compute_forward_kinematics(q[b])
+        P = np.array([np.concatenate([m.get_link_pose(i).p, m.get_link_pose(i).q]) for i in range(len(ln))])
+        pos, ang = pose_errors(P[None], o.link_poses(q[b:b + 1]))
+        assert pos <= 1e-6 and ang <= 1e-6
+        assert np.allclose(m.compute_inverse_dynamics(q[b], qd[b], qdd[b]), o.rnea(q[b:b + 1], qd[b:b + 1],
+                                                                                    qdd[b:b + 1])[0], atol=1e-9)
+        m.compute_full_jacobian(q[b])
+        for i in range(len(ln)):
+            assert np.allclose(m.get_link_jacobian(i), o.link_jacobian(q[b], i)[0], atol=1e-9)
+    # batched FP32, both kernel families
+    for v in ("auto", "thread_per_env"):
+        sa.set_kernel_variant(m, v)
+        _check_kin(m, o, q, qd, qdd)
+        if massive:
+            assert rel_err(m.compute_generalized_mass_matrix_batched(_t(q)).cpu().numpy(), o.crba(q)) <= REL_TOL
+        if spd:
+            tau = o.rnea(q, qd, qdd)
+            A = m.compute_forward_dynamics_batched(_t(q), _t(qd), _t(tau)).cpu().numpy()
+            assert rel_err(A, qdd) <= 1e-3, (v, sa.dynamics_precision(m))
