@@ -1110,6 +1110,9 @@ int sapamd_aba_batched(const sapamd_model *m, int64_t B, const float *q, const f
     need(qdd, "qdd");
     if (m->fd64())
       HIP_OK(launch_fd_ldlt(m->t64(), m->host->njoints(), B, q, qd, tau, qdd, S(stream)));
+    else if (m->effective_variant() == SAPAMD_KERNEL_LANES_PER_JOINT && tree_dyn_group(m->host->njoints()))
+      HIP_OK(launch_tree_step(m->t32(), m->host->njoints(), B, 0, 0.f, const_cast<float *>(q),
+                              const_cast<float *>(qd), tau, m->nv(), qdd, nullptr, 0, S(stream)));
     else
       HIP_OK(launch_aba<float>(m->t32(), m->host->njoints(), B, q, qd, tau, qdd, S(stream)));
   });
@@ -1269,6 +1272,34 @@ int sapamd_pose_torque_loss_batched(const sapamd_model *m, int64_t B, const floa
   });
 }
 
+namespace {
+// Forward dynamics of one step: the lanes-per-joint FP32 factorization for
+// trees of <= 32 joints (lanes-per-joint variant), the thread-per-environment
+// kernels otherwise (FP32 ABA, or FP64 CRBA + LTDL for ill-conditioned models).
+bool lanes_dynamics(const sapamd_model *m) {
+  return !m->fd64() && m->effective_variant() == SAPAMD_KERNEL_LANES_PER_JOINT &&
+         tree_dyn_group(m->host->njoints()) != 0;
+}
+void step_launch(const sapamd_model *m, int64_t B, float dt, float *q, float *qd, const float *tau,
+                 int64_t tau_stride, float *save, int64_t save_stride, void *stream) {
+  if (lanes_dynamics(m))
+    HIP_OK(launch_tree_step(m->t32(), m->host->njoints(), B, 1, dt, q, qd, tau, tau_stride, nullptr, save,
+                            save_stride, S(stream)));
+  else
+    HIP_OK(launch_rollout_step<float>(m->t32(), m->fd64() ? m->t64() : nullptr, m->host->njoints(), B, dt, q, qd,
+                                      tau, tau_stride, save, save_stride, S(stream)));
+}
+void step_vjp_launch(const sapamd_model *m, int64_t B, float dt, const float *state, int64_t state_stride,
+                     const float *tau, int64_t tau_stride, float *aq, float *aqd, float *gtau, void *stream) {
+  if (lanes_dynamics(m))
+    HIP_OK(launch_tree_step_vjp(m->t32(), m->host->njoints(), B, dt, state, state_stride, tau, tau_stride, aq, aqd,
+                                gtau, S(stream)));
+  else
+    HIP_OK(launch_rollout_step_vjp<float>(m->t32(), m->fd64() ? m->t64() : nullptr, m->host->njoints(), B, dt, state,
+                                          state_stride, tau, tau_stride, aq, aqd, gtau, S(stream)));
+}
+}  // namespace
+
 int sapamd_rollout_step(const sapamd_model *m, int64_t B, float dt, float *q, float *qd, const float *tau_t,
                         int64_t tau_stride, void *stream) {
   return guarded([&] {
@@ -1277,8 +1308,7 @@ int sapamd_rollout_step(const sapamd_model *m, int64_t B, float dt, float *q, fl
     need(q, "q");
     need(qd, "qd");
     need(tau_t, "tau_t");
-    HIP_OK(launch_rollout_step<float>(m->t32(), m->fd64() ? m->t64() : nullptr, m->host->njoints(), B, dt, q, qd,
-                                      tau_t, tau_stride, nullptr, 0, S(stream)));
+    step_launch(m, B, dt, q, qd, tau_t, tau_stride, nullptr, 0, stream);
   });
 }
 
@@ -1291,13 +1321,11 @@ int sapamd_rollout_forward(const sapamd_model *m, int64_t B, int32_t T, int32_t 
     need(q, "q");
     need(qd, "qd");
     need(tau, "tau");
-    const int nv = m->nv(), nj = m->host->njoints();
-    const ModelTable<double> *t64 = m->fd64() ? m->t64() : nullptr;
+    const int nv = m->nv();
     const int64_t S_ = (T + K - 1) / K;
     for (int t = 0; t < T; ++t) {
       float *save = (ckpt && t % K == 0) ? ckpt + (int64_t)(t / K) * 2 * nv : nullptr;
-      HIP_OK(launch_rollout_step<float>(m->t32(), t64, nj, B, dt, q, qd, tau + (int64_t)t * nv, (int64_t)T * nv,
-                                        save, S_ * 2 * nv, S(stream)));
+      step_launch(m, B, dt, q, qd, tau + (int64_t)t * nv, (int64_t)T * nv, save, S_ * 2 * nv, stream);
     }
   });
 }
@@ -1312,20 +1340,17 @@ int sapamd_rollout_backward(const sapamd_model *m, int64_t B, int32_t T, int32_t
     for (const void *p : {(const void *)tau, (const void *)ckpt, (const void *)seg, (const void *)q_tmp,
                           (const void *)qd_tmp, (const void *)aq, (const void *)aqd, (const void *)gtau})
       need(p, "rollout buffer");
-    const int nv = m->nv(), nj = m->host->njoints();
-    const ModelTable<double> *t64 = m->fd64() ? m->t64() : nullptr;
+    const int nv = m->nv();
     const int64_t S_ = (T + K - 1) / K;
     for (int64_t s = S_ - 1; s >= 0; --s) {
       const int t0 = (int)(s * K), t1 = std::min(T, t0 + K);
       HIP_OK(launch_restore_state(B, nv, ckpt + s * 2 * nv, S_ * 2 * nv, q_tmp, qd_tmp, S(stream)));
       for (int t = t0; t < t1; ++t)  // recompute the segment, keeping every pre-step state
-        HIP_OK(launch_rollout_step<float>(m->t32(), t64, nj, B, dt, q_tmp, qd_tmp, tau + (int64_t)t * nv,
-                                          (int64_t)T * nv, seg + (int64_t)(t - t0) * 2 * nv, (int64_t)K * 2 * nv,
-                                          S(stream)));
+        step_launch(m, B, dt, q_tmp, qd_tmp, tau + (int64_t)t * nv, (int64_t)T * nv,
+                    seg + (int64_t)(t - t0) * 2 * nv, (int64_t)K * 2 * nv, stream);
       for (int t = t1 - 1; t >= t0; --t)
-        HIP_OK(launch_rollout_step_vjp<float>(m->t32(), t64, nj, B, dt, seg + (int64_t)(t - t0) * 2 * nv,
-                                              (int64_t)K * 2 * nv, tau + (int64_t)t * nv, (int64_t)T * nv, aq, aqd,
-                                              gtau + (int64_t)t * nv, S(stream)));
+        step_vjp_launch(m, B, dt, seg + (int64_t)(t - t0) * 2 * nv, (int64_t)K * 2 * nv, tau + (int64_t)t * nv,
+                        (int64_t)T * nv, aq, aqd, gtau + (int64_t)t * nv, stream);
     }
   });
 }

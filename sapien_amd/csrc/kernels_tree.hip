@@ -18,6 +18,7 @@
 #include "lanes.h"
 #include "launch.h"
 #include "spatial.h"
+#include "tree.h"
 
 namespace sapamd {
 
@@ -27,88 +28,6 @@ using namespace lanes;
 
 constexpr int kMaxTreeJoints = 32;
 constexpr int kTreeTableWords4 = (int)((kHeaderBytes + sizeof(JointRec<float>) * kMaxTreeJoints) / 16);
-
-__device__ __forceinline__ S6<float> bperm6(int src4, S6<float> x) {
-  return {{bperm(src4, x.l.x), bperm(src4, x.l.y), bperm(src4, x.l.z)},
-          {bperm(src4, x.a.x), bperm(src4, x.a.y), bperm(src4, x.a.z)}};
-}
-__device__ __forceinline__ Xf bperm_xf(int src4, const Xf &x) {
-  Xf o;
-#pragma unroll
-  for (int k = 0; k < 9; ++k) o.R[k] = bperm(src4, x.R[k]);
-  o.p = {bperm(src4, x.p.x), bperm(src4, x.p.y), bperm(src4, x.p.z)};
-  return o;
-}
-
-// log2(G) rounds of pointer jumping toward the root.
-template <int G> struct Tree {
-  static constexpr int K = G == 8 ? 3 : G == 16 ? 4 : 5;
-  int base4;
-  int anc[K];
-  int size;
-  int g;
-  __device__ __forceinline__ void prefix(Xf &x) const {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int a = anc[k];
-      const Xf y = bperm_xf(base4 + 4 * (a < 0 ? 0 : a), x);
-      x = a >= 0 ? compose(y, x) : x;
-    }
-  }
-  __device__ __forceinline__ void prefix(S6<float> &x) const {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int a = anc[k];
-      const S6<float> y = bperm6(base4 + 4 * (a < 0 ? 0 : a), x);
-      x = sel(a >= 0, x + y, x);
-    }
-  }
-  // subtree sum of x over [g, g + size) (x = 0 on lanes without a joint)
-  template <int N> __device__ __forceinline__ void subtree(float (&x)[N]) const {
-    float blk[K][N];
-    float cur[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) cur[i] = x[i];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-#pragma unroll
-      for (int i = 0; i < N; ++i) blk[k][i] = cur[i];
-      const int src = g + (1 << k);
-      const int s4 = base4 + 4 * (src < G ? src : 0);
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        const float y = bperm(s4, cur[i]);
-        cur[i] = src < G ? cur[i] + y : cur[i];
-      }
-    }
-    // blocks of size 2^K cover the whole group: size <= G
-    float acc[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) acc[i] = 0.f;
-    // every lane runs the query (ds_bpermute must not sit in divergent code);
-    // size == G (a single-root tree filling the group) has no bits below K and
-    // takes the full block instead
-    int pos = g;
-#pragma unroll
-    for (int k = K - 1; k >= 0; --k) {
-      const bool take = (size >> k) & 1;
-      const int s4 = base4 + 4 * (pos < G ? pos : 0);
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        const float y = bperm(s4, blk[k][i]);
-        acc[i] = take ? acc[i] + y : acc[i];
-      }
-      pos += take ? (1 << k) : 0;
-    }
-#pragma unroll
-    for (int i = 0; i < N; ++i) x[i] = size >= G ? cur[i] : acc[i];
-  }
-  __device__ __forceinline__ void subtree(S6<float> &f) const {
-    float x[6] = {f.l.x, f.l.y, f.l.z, f.a.x, f.a.y, f.a.z};
-    subtree<6>(x);
-    f = {{x[0], x[1], x[2]}, {x[3], x[4], x[5]}};
-  }
-};
 
 struct TreeIO {
   const float *q, *qd, *qdd;
@@ -229,17 +148,6 @@ __device__ __forceinline__ void tree_body(const ModelTable<float> *M, const Mode
       if (live && valid) st<NT>(mij, io.M + (e * nv + u) * nv + Ji.vuser);
     }
   }
-}
-
-// Sum over the G lanes of the group, result in lane 0.
-template <int G> __device__ __forceinline__ float group_sum(int g, int base4, float x) {
-#pragma unroll
-  for (int d = 1; d < G; d *= 2) {
-    const int src = g + d;
-    const float y = bperm(base4 + 4 * (src < G ? src : 0), x);
-    x = src < G ? x + y : x;
-  }
-  return x;
 }
 
 // Forward recomputation + reverse pass (formulas in kernels_adjoint.hip).

@@ -44,18 +44,25 @@ def _problem(rob, B, T, seed):
 # quad18's mass matrix reaches cond ~1e6 (light feet under a heavy floating
 # base): its FP64 rollout moves by 1e-4 relative under a 1e-6 relative change
 # of q0, so FP32 states carry ~1e-3 after 37 steps.
-FWD_TOL = {"chain7": 1e-4, "arm9": 1e-4, "quad18": 2e-3}
+FWD_TOL = {"chain7": 1e-4, "arm9": 1e-4, "quad18": 2e-3, "torso30": 1e-3}
 
 
-@pytest.mark.parametrize("tree", ["chain7", "arm9", "quad18"])
-def test_rollout_forward_matches_oracle(tree):
+VARIANTS = ["auto", "thread_per_env"]  # lanes-per-joint factorization / thread-per-environment ABA
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("tree", ["chain7", "arm9", "quad18", "torso30"])
+def test_rollout_forward_matches_oracle(tree, variant):
+    import sapien_amd as sa
+
     rob = S.robot(tree)
     m, o = _pair(rob)
+    sa.set_kernel_variant(m, variant)
     q0, qd0, tau = _problem(rob, 32, 37, 1500)
     qT, qdT, ckpt = m.rollout_batched(_t(q0), _t(qd0), _t(tau), DT, 8)
     rq, rqd = o.rollout(q0, qd0, tau, DT)
     eq, eqd = rel_err(qT.cpu().numpy(), rq), rel_err(qdT.cpu().numpy(), rqd)
-    print(f"{tree}: rel err q {eq:.2e} qd {eqd:.2e}")
+    print(f"{tree} {variant}: rel err q {eq:.2e} qd {eqd:.2e}")
     assert eq <= FWD_TOL[tree] and eqd <= FWD_TOL[tree]
     # checkpoints: the pre-step state of steps 0, 8, ..., 32
     assert tuple(ckpt.shape) == (32, 5, 2 * rob.nv)
@@ -64,10 +71,14 @@ def test_rollout_forward_matches_oracle(tree):
     assert rel_err(ckpt[:, 1].cpu().numpy(), np.concatenate([q8, qd8], 1)) <= FWD_TOL[tree]
 
 
-@pytest.mark.parametrize("tree", ["chain7", "arm9"])
-def test_rollout_adjoint_dot_product(tree):
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("tree", ["chain7", "arm9", "torso30"])
+def test_rollout_adjoint_dot_product(tree, variant):
+    import sapien_amd as sa
+
     rob = S.robot(tree)
     m, o = _pair(rob)
+    sa.set_kernel_variant(m, variant)
     B, T = 6, 37
     q0, qd0, tau = _problem(rob, B, T, 1501)
     rng = np.random.default_rng(7)
