@@ -216,23 +216,47 @@ def mixed_workloads(sa, S, np, torch, dev):
         tau = np.random.default_rng(1005 + i).normal(scale=5.0, size=(b, T, rob.nv)).astype(np.float32)
         per.append((m, torch.from_numpy(q0).to(dev), torch.from_numpy(qd0).to(dev), torch.from_numpy(tau).to(dev)))
     ck = {}
+    # one default-priority stream per tree: the trees' step launches overlap
+    streams = [torch.cuda.Stream(device=dev) for _ in per]
+    main = torch.cuda.current_stream(dev)
 
-    def fwd():
-        for j, (m, q0, qd0, tau) in enumerate(per):
-            ck[j] = m.rollout_batched(q0, qd0, tau, dt, K)
+    def fwd_one(j):
+        m, q0, qd0, tau = per[j]
+        ck[j] = m.rollout_batched(q0, qd0, tau, dt, K)
 
-    def bwd():
-        for j, (m, q0, qd0, tau) in enumerate(per):
-            g = torch.ones_like(q0)
-            m.rollout_backward_batched(tau, ck[j][2], dt, g, g, K)
+    def bwd_one(j):
+        m, q0, qd0, tau = per[j]
+        g = torch.ones_like(q0)
+        m.rollout_backward_batched(tau, ck[j][2], dt, g, g, K)
 
-    sf = timed(fwd, 3)
-    sb = timed(bwd, 3)
+    def seq(fn):
+        return lambda: [fn(j) for j in range(len(per))]
+
+    def conc(fn):
+        def run():
+            for j, st in enumerate(streams):
+                st.wait_stream(main)
+                with torch.cuda.stream(st):
+                    fn(j)
+            for st in streams:
+                main.wait_stream(st)
+        return run
+
+    sf, sb = timed(seq(fwd_one), 3), timed(seq(bwd_one), 3)
+    cf, cb = timed(conc(fwd_one), 3), timed(conc(bwd_one), 3)
+    trees = {}
+    for j, (name, _) in enumerate(MIX):
+        tf, tb = timed(lambda: fwd_one(j), 3), timed(lambda: bwd_one(j), 3)
+        trees[name] = {"rollouts": nr[j], "forward_ms": tf * 1e3, "backward_ms": tb * 1e3,
+                       "precision": sa.dynamics_precision(models[j])[0],
+                       "kernel": "lanes per joint" if sa.dynamics_precision(models[j])[0] == "fp32"
+                       and sa.kernel_variant(models[j]) == "lanes_per_joint" else "thread per env"}
     res["rollouts_mixed"] = {"rollouts": nr, "steps": T, "dt": dt, "checkpoint_every": K,
-                             "forward_env_steps_s": 4096 * T / sf, "forward_ms": sf * 1e3,
-                             "backward_env_steps_s": 4096 * T / sb, "backward_ms": sb * 1e3,
-                             "precision": [sa.dynamics_precision(m)[0] for m in models],
-                             "kernel": "thread per env, one launch per step and tree"}
+                             "forward_env_steps_s": 4096 * T / cf, "forward_ms": cf * 1e3,
+                             "backward_env_steps_s": 4096 * T / cb, "backward_ms": cb * 1e3,
+                             "sequential": {"forward_ms": sf * 1e3, "backward_ms": sb * 1e3},
+                             "per_tree": trees,
+                             "note": "one launch per step and tree; the four trees on four streams"}
     return res
 
 

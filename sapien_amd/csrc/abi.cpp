@@ -1099,6 +1099,15 @@ int sapamd_coriolis_batched(const sapamd_model *m, int64_t B, const float *q, co
   });
 }
 
+namespace {
+// Trees of <= 32 joints under the lanes-per-joint kernels factor M in LDS (FP32
+// or, past the condition limit, FP64); the rest take the thread-per-env ABA /
+// FP64 factorization.
+bool lanes_dynamics(const sapamd_model *m) {
+  return m->effective_variant() == SAPAMD_KERNEL_LANES_PER_JOINT && tree_dyn_group(m->host->njoints()) != 0;
+}
+}  // namespace
+
 int sapamd_aba_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd, const float *tau,
                        float *qdd, void *stream) {
   return guarded([&] {
@@ -1108,11 +1117,12 @@ int sapamd_aba_batched(const sapamd_model *m, int64_t B, const float *q, const f
     need(qd, "qd");
     need(tau, "tau");
     need(qdd, "qdd");
-    if (m->fd64())
+    if (lanes_dynamics(m))
+      HIP_OK(launch_tree_step(m->t32(), m->fd64() ? m->t64() : nullptr, m->host->njoints(), B, 0, 0.f,
+                              const_cast<float *>(q), const_cast<float *>(qd), tau, m->nv(), qdd, nullptr, 0,
+                              S(stream)));
+    else if (m->fd64())
       HIP_OK(launch_fd_ldlt(m->t64(), m->host->njoints(), B, q, qd, tau, qdd, S(stream)));
-    else if (m->effective_variant() == SAPAMD_KERNEL_LANES_PER_JOINT && tree_dyn_group(m->host->njoints()))
-      HIP_OK(launch_tree_step(m->t32(), m->host->njoints(), B, 0, 0.f, const_cast<float *>(q),
-                              const_cast<float *>(qd), tau, m->nv(), qdd, nullptr, 0, S(stream)));
     else
       HIP_OK(launch_aba<float>(m->t32(), m->host->njoints(), B, q, qd, tau, qdd, S(stream)));
   });
@@ -1276,14 +1286,12 @@ namespace {
 // Forward dynamics of one step: the lanes-per-joint FP32 factorization for
 // trees of <= 32 joints (lanes-per-joint variant), the thread-per-environment
 // kernels otherwise (FP32 ABA, or FP64 CRBA + LTDL for ill-conditioned models).
-bool lanes_dynamics(const sapamd_model *m) {
-  return !m->fd64() && m->effective_variant() == SAPAMD_KERNEL_LANES_PER_JOINT &&
-         tree_dyn_group(m->host->njoints()) != 0;
-}
+
 void step_launch(const sapamd_model *m, int64_t B, float dt, float *q, float *qd, const float *tau,
                  int64_t tau_stride, float *save, int64_t save_stride, void *stream) {
   if (lanes_dynamics(m))
-    HIP_OK(launch_tree_step(m->t32(), m->host->njoints(), B, 1, dt, q, qd, tau, tau_stride, nullptr, save,
+    HIP_OK(launch_tree_step(m->t32(), m->fd64() ? m->t64() : nullptr, m->host->njoints(), B, 1, dt, q, qd, tau,
+                            tau_stride, nullptr, save,
                             save_stride, S(stream)));
   else
     HIP_OK(launch_rollout_step<float>(m->t32(), m->fd64() ? m->t64() : nullptr, m->host->njoints(), B, dt, q, qd,
@@ -1292,7 +1300,8 @@ void step_launch(const sapamd_model *m, int64_t B, float dt, float *q, float *qd
 void step_vjp_launch(const sapamd_model *m, int64_t B, float dt, const float *state, int64_t state_stride,
                      const float *tau, int64_t tau_stride, float *aq, float *aqd, float *gtau, void *stream) {
   if (lanes_dynamics(m))
-    HIP_OK(launch_tree_step_vjp(m->t32(), m->host->njoints(), B, dt, state, state_stride, tau, tau_stride, aq, aqd,
+    HIP_OK(launch_tree_step_vjp(m->t32(), m->fd64() ? m->t64() : nullptr, m->host->njoints(), B, dt, state,
+                                state_stride, tau, tau_stride, aq, aqd,
                                 gtau, S(stream)));
   else
     HIP_OK(launch_rollout_step_vjp<float>(m->t32(), m->fd64() ? m->t64() : nullptr, m->host->njoints(), B, dt, state,

@@ -20,6 +20,12 @@ template <int D> __device__ __forceinline__ float from_above(float x) { return d
 __device__ __forceinline__ float bperm(int src4, float x) {
   return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src4, __builtin_bit_cast(int, x)));
 }
+__device__ __forceinline__ double bperm(int src4, double x) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(src4, (int)(uint32_t)u);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(src4, (int)(uint32_t)(u >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
 
 template <int D> __device__ __forceinline__ S6<float> below6(S6<float> v) {
   return {{from_below<D>(v.l.x), from_below<D>(v.l.y), from_below<D>(v.l.z)},
@@ -29,7 +35,7 @@ template <int D> __device__ __forceinline__ S6<float> above6(S6<float> v) {
   return {{from_above<D>(v.l.x), from_above<D>(v.l.y), from_above<D>(v.l.z)},
           {from_above<D>(v.a.x), from_above<D>(v.a.y), from_above<D>(v.a.z)}};
 }
-__device__ __forceinline__ S6<float> sel(bool t, S6<float> a, S6<float> b) {
+template <class T> __device__ __forceinline__ S6<T> sel(bool t, S6<T> a, S6<T> b) {
   return {{t ? a.l.x : b.l.x, t ? a.l.y : b.l.y, t ? a.l.z : b.l.z},
           {t ? a.a.x : b.a.x, t ? a.a.y : b.a.y, t ? a.a.z : b.a.z}};
 }
@@ -47,22 +53,24 @@ __device__ __forceinline__ void joint_sincos(float x, float *s, float *c) {
     sincosf(x, s, c);
   }
 }
+__device__ __forceinline__ void joint_sincos(double x, double *s, double *c) { sincos(x, s, c); }
 
-struct Xf {
-  float R[9];
-  V3<float> p;
+template <class T> struct XfT {
+  T R[9];
+  V3<T> p;
 };
-__device__ __forceinline__ Xf compose(const Xf &a, const Xf &b) {
-  Xf o;
+using Xf = XfT<float>;
+template <class T> __device__ __forceinline__ XfT<T> compose(const XfT<T> &a, const XfT<T> &b) {
+  XfT<T> o;
   mm(a.R, b.R, o.R);
   o.p = mv(a.R, b.p) + a.p;
   return o;
 }
-__device__ __forceinline__ Xf identity_xf() {
-  Xf o;
+template <class T = float> __device__ __forceinline__ XfT<T> identity_xf() {
+  XfT<T> o;
 #pragma unroll
-  for (int k = 0; k < 9; ++k) o.R[k] = (k % 4 == 0) ? 1.f : 0.f;
-  o.p = {0.f, 0.f, 0.f};
+  for (int k = 0; k < 9; ++k) o.R[k] = (k % 4 == 0) ? T(1) : T(0);
+  o.p = {T(0), T(0), T(0)};
   return o;
 }
 template <int D> __device__ __forceinline__ Xf below_xf(const Xf &x) {
@@ -160,12 +168,13 @@ template <bool NT = false> __device__ __forceinline__ void store_pose(float *P, 
 
 // World inertia of a body whose rotational inertia is diagonal in its frame:
 // R diag(d) R^T = sum_k d_k r_k r_k^T (columns r_k of R).
-__device__ __forceinline__ WInertia<float> world_inertia_fast(const JointRec<float> &J, const float *R, V3<float> p) {
+template <class T>
+__device__ __forceinline__ WInertia<T> world_inertia_fast(const JointRec<T> &J, const T *R, V3<T> p) {
   if (!J.diagI) return world_inertia(J, R, p);
-  WInertia<float> Y;
+  WInertia<T> Y;
   Y.m = J.m;
   Y.c = mv(R, load3(J.c)) + p;
-  const float d0 = J.Ic[0], d1 = J.Ic[3], d2 = J.Ic[5];
+  const T d0 = J.Ic[0], d1 = J.Ic[3], d2 = J.Ic[5];
   Y.I[0] = d0 * R[0] * R[0] + d1 * R[1] * R[1] + d2 * R[2] * R[2];
   Y.I[1] = d0 * R[0] * R[3] + d1 * R[1] * R[4] + d2 * R[2] * R[5];
   Y.I[2] = d0 * R[0] * R[6] + d1 * R[1] * R[7] + d2 * R[2] * R[8];

@@ -169,17 +169,18 @@ hipError_t launch_rollout_step_vjp(const ModelTable<T> *tab, const ModelTable<do
 // Batched forward dynamics in FP64 (CRBA + RNEA bias + LTDL), FP32 in and out.
 hipError_t launch_fd_ldlt(const ModelTable<double> *tab64, int nj, int64_t B, const float *q, const float *qd,
                           const float *tau, float *qdd, hipStream_t s);
-// Lanes-per-joint forward dynamics for trees of <= 32 joints (FP32): CRBA +
-// RNEA bias + LDL^T in LDS. mode 0: out = qdd (B, nv); mode 1: one rollout step
-// in place on q, qd (save as launch_rollout_step). hipErrorNotSupported above
-// 32 joints.
+// Lanes-per-joint forward dynamics for trees of <= 32 joints: CRBA + RNEA
+// bias + LDL^T in LDS, in FP64 when tab64 is given, else FP32 (FP32 inputs,
+// outputs and state either way). mode 0: out = qdd (B, nv); mode 1: one rollout
+// step in place on q, qd (save as launch_rollout_step). hipErrorNotSupported
+// above 32 joints.
 int tree_dyn_group(int nj);
-hipError_t launch_tree_step(const ModelTable<float> *tab, int nj, int64_t B, int mode, float dt, float *q, float *qd,
-                            const float *tau, int64_t tau_stride, float *out, float *save, int64_t save_stride,
-                            hipStream_t s);
-hipError_t launch_tree_step_vjp(const ModelTable<float> *tab, int nj, int64_t B, float dt, const float *state,
-                                int64_t state_stride, const float *tau, int64_t tau_stride, float *aq, float *aqd,
-                                float *gtau, hipStream_t s);
+hipError_t launch_tree_step(const ModelTable<float> *tab, const ModelTable<double> *tab64, int nj, int64_t B,
+                            int mode, float dt, float *q, float *qd, const float *tau, int64_t tau_stride, float *out,
+                            float *save, int64_t save_stride, hipStream_t s);
+hipError_t launch_tree_step_vjp(const ModelTable<float> *tab, const ModelTable<double> *tab64, int nj, int64_t B,
+                                float dt, const float *state, int64_t state_stride, const float *tau,
+                                int64_t tau_stride, float *aq, float *aqd, float *gtau, hipStream_t s);
 // rows of (q | qd) at stride -> q, qd (B, nv)
 hipError_t launch_restore_state(int64_t B, int nv, const float *src, int64_t stride, float *q, float *qd,
                                 hipStream_t s);

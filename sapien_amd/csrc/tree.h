@@ -11,45 +11,46 @@
 namespace sapamd {
 namespace lanes {
 
-__device__ __forceinline__ S6<float> bperm6(int src4, S6<float> x) {
+template <class T> __device__ __forceinline__ S6<T> bperm6(int src4, S6<T> x) {
   return {{bperm(src4, x.l.x), bperm(src4, x.l.y), bperm(src4, x.l.z)},
           {bperm(src4, x.a.x), bperm(src4, x.a.y), bperm(src4, x.a.z)}};
 }
-__device__ __forceinline__ Xf bperm_xf(int src4, const Xf &x) {
-  Xf o;
+template <class T> __device__ __forceinline__ XfT<T> bperm_xf(int src4, const XfT<T> &x) {
+  XfT<T> o;
 #pragma unroll
   for (int k = 0; k < 9; ++k) o.R[k] = bperm(src4, x.R[k]);
   o.p = {bperm(src4, x.p.x), bperm(src4, x.p.y), bperm(src4, x.p.z)};
   return o;
 }
 
-// log2(G) rounds of pointer jumping toward the root.
-template <int G> struct Tree {
+// log2(G) rounds of pointer jumping toward the root; T is the scalar type of
+// the scanned values.
+template <int G, class T = float> struct Tree {
   static constexpr int K = G == 8 ? 3 : G == 16 ? 4 : 5;
   int base4;
   int anc[K];
   int size;
   int g;
-  __device__ __forceinline__ void prefix(Xf &x) const {
+  __device__ __forceinline__ void prefix(XfT<T> &x) const {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int a = anc[k];
-      const Xf y = bperm_xf(base4 + 4 * (a < 0 ? 0 : a), x);
+      const XfT<T> y = bperm_xf(base4 + 4 * (a < 0 ? 0 : a), x);
       x = a >= 0 ? compose(y, x) : x;
     }
   }
-  __device__ __forceinline__ void prefix(S6<float> &x) const {
+  __device__ __forceinline__ void prefix(S6<T> &x) const {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int a = anc[k];
-      const S6<float> y = bperm6(base4 + 4 * (a < 0 ? 0 : a), x);
+      const S6<T> y = bperm6(base4 + 4 * (a < 0 ? 0 : a), x);
       x = sel(a >= 0, x + y, x);
     }
   }
   // subtree sum of x over [g, g + size) (x = 0 on lanes without a joint)
-  template <int N> __device__ __forceinline__ void subtree(float (&x)[N]) const {
-    float blk[K][N];
-    float cur[N];
+  template <int N> __device__ __forceinline__ void subtree(T (&x)[N]) const {
+    T blk[K][N];
+    T cur[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) cur[i] = x[i];
 #pragma unroll
@@ -60,14 +61,14 @@ template <int G> struct Tree {
       const int s4 = base4 + 4 * (src < G ? src : 0);
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        const float y = bperm(s4, cur[i]);
+        const T y = bperm(s4, cur[i]);
         cur[i] = src < G ? cur[i] + y : cur[i];
       }
     }
     // blocks of size 2^K cover the whole group: size <= G
-    float acc[N];
+    T acc[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) acc[i] = 0.f;
+    for (int i = 0; i < N; ++i) acc[i] = T(0);
     // every lane runs the query (ds_bpermute must not sit in divergent code);
     // size == G (a single-root tree filling the group) has no bits below K and
     // takes the full block instead
@@ -78,7 +79,7 @@ template <int G> struct Tree {
       const int s4 = base4 + 4 * (pos < G ? pos : 0);
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        const float y = bperm(s4, blk[k][i]);
+        const T y = bperm(s4, blk[k][i]);
         acc[i] = take ? acc[i] + y : acc[i];
       }
       pos += take ? (1 << k) : 0;
@@ -86,8 +87,8 @@ template <int G> struct Tree {
 #pragma unroll
     for (int i = 0; i < N; ++i) x[i] = size >= G ? cur[i] : acc[i];
   }
-  __device__ __forceinline__ void subtree(S6<float> &f) const {
-    float x[6] = {f.l.x, f.l.y, f.l.z, f.a.x, f.a.y, f.a.z};
+  __device__ __forceinline__ void subtree(S6<T> &f) const {
+    T x[6] = {f.l.x, f.l.y, f.l.z, f.a.x, f.a.y, f.a.z};
     subtree<6>(x);
     f = {{x[0], x[1], x[2]}, {x[3], x[4], x[5]}};
   }

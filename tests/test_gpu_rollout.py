@@ -196,15 +196,18 @@ def _f32(x):
     return np.asarray(x, np.float32).astype(np.float64)
 
 
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("tree", ["chain7", "arm9", "quad18", "torso30"])
-def test_forward_dynamics_precision_policy(tree):
-    """FP64 (CRBA + LTDL) matches the FP64 oracle on the same FP32-rounded
+def test_forward_dynamics_precision_policy(tree, variant):
+    """FP64 (CRBA + LDL^T in LDS under the lanes-per-joint kernels, CRBA + LTDL
+    per thread otherwise) matches the FP64 oracle on the same FP32-rounded
     inputs to output rounding; the automatic choice (FP64 above cond 1e4,
     i.e. quad18's floating base) holds 1e-3 relative."""
     import sapien_amd as sa
 
     rob = S.robot(tree)
     m, o = _pair(rob)
+    sa.set_kernel_variant(m, variant)
     prec, cond = sa.dynamics_precision(m)
     assert prec == ("fp64" if cond > 1e4 else "fp32")
     assert prec == ("fp64" if tree == "quad18" else "fp32")
@@ -216,11 +219,12 @@ def test_forward_dynamics_precision_policy(tree):
     sa.set_dynamics_precision(m, "fp64")
     a64 = m.compute_forward_dynamics_batched(_t(q), _t(qd), _t(tau)).cpu().numpy()
     e_auto, e32, e64 = rel_err(auto, ref), rel_err(a32, ref), rel_err(a64, ref)
-    print(f"{tree}: cond {cond:.3g}  FP32 ABA {e32:.2e}  FP64 LTDL {e64:.2e}")
+    print(f"{tree} {variant}: cond {cond:.3g}  FP32 {e32:.2e}  FP64 {e64:.2e}")
     assert e64 <= 1e-6 and e_auto <= 1e-3
 
 
-def test_ill_conditioned_tree_switches_to_fp64():
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_ill_conditioned_tree_switches_to_fp64(variant):
     import sapien_amd as sa
 
     rob = S.chain7()
@@ -228,6 +232,7 @@ def test_ill_conditioned_tree_switches_to_fp64():
     m = sa.PinocchioModel(urdf, [0.0, 0.0, -9.81])
     m.set_joint_order(rob.joint_names)
     m.set_link_order(rob.link_names)
+    sa.set_kernel_variant(m, variant)
     o = OracleModel(urdf)
     o.set_joint_order(rob.joint_names)
     o.set_link_order(rob.link_names)
