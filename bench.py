@@ -249,8 +249,8 @@ def mixed_workloads(sa, S, np, torch, dev):
         tf, tb = timed(lambda: fwd_one(j), 3), timed(lambda: bwd_one(j), 3)
         trees[name] = {"rollouts": nr[j], "forward_ms": tf * 1e3, "backward_ms": tb * 1e3,
                        "precision": sa.dynamics_precision(models[j])[0],
-                       "kernel": "lanes per joint" if sa.dynamics_precision(models[j])[0] == "fp32"
-                       and sa.kernel_variant(models[j]) == "lanes_per_joint" else "thread per env"}
+                       "kernel": "lanes per joint" if sa.kernel_variant(models[j]) == "lanes_per_joint"
+                       and robs[j].nv <= 32 else "thread per env"}
     res["rollouts_mixed"] = {"rollouts": nr, "steps": T, "dt": dt, "checkpoint_every": K,
                              "forward_env_steps_s": 4096 * T / cf, "forward_ms": cf * 1e3,
                              "backward_env_steps_s": 4096 * T / cb, "backward_ms": cb * 1e3,

@@ -18,6 +18,8 @@
 // only wavefront-level ordering (no block barrier).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "lanes.h"
 #include "launch.h"
 #include "spatial.h"
@@ -42,45 +44,81 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// LDL^T of the symmetric positive definite n x n matrix in A (lower triangle
-// used), lane i owning row i: after the call A holds L below the diagonal and
-// D on it.
-template <int G, class Real> __device__ __forceinline__ void ldlt_factor(Real (*A)[G + 1], int i, int n) {
-  for (int k = 0; k < n; ++k) {
-    wave_sync();
-    const Real dk = A[k][k];
-    const bool act = i > k && i < n;
-    const Real l = act ? A[i][k] / dk : Real(0);
-    if (act)
-      for (int j = k + 1; j <= i; ++j) A[i][j] -= l * A[j][k];
-    wave_sync();
-    if (act) A[i][k] = l;
+// Compile-time loop: f(std::integral_constant<int, K>) for K in [B, E), so
+// register arrays indexed by K never fall back to the private segment (the
+// unroller's size limit does not apply).
+template <int B, int E, class F> __device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
   }
-  wave_sync();
-}
-// x = (L D L^T)^-1 r with lane i holding r_i; returns x_i. y: G floats of LDS.
-template <int G, class Real>
-__device__ __forceinline__ Real ldlt_solve(Real (*A)[G + 1], Real *y, int i, int n, Real r) {
-  for (int k = 0; k < n; ++k) {
-    if (i == k) y[k] = r;
-    wave_sync();
-    const Real yk = y[k];
-    if (i > k && i < n) r -= A[i][k] * yk;
-    wave_sync();
-  }
-  Real z = i < n ? r / A[i][i] : Real(0);
-  for (int k = n - 1; k >= 0; --k) {
-    if (i == k) y[k] = z;
-    wave_sync();
-    const Real xk = y[k];
-    if (i < k) z -= A[k][i] * xk;
-    wave_sync();
-  }
-  return z;
 }
 
-// Rows padded by one word: lane i walking row i and lane j walking row j hit
-// different banks.
+// LDL^T of the symmetric positive definite n x n matrix whose row i (lower
+// triangle, zeros above the diagonal) lane i holds in row[]: right-looking
+// elimination with the rows in registers, step k broadcasting column k through
+// the group's G-entry LDS buffer. After the call row[j < i] = L_ij; returns D_i.
+template <int G, class Real> __device__ __forceinline__ Real ldlt_factor(Real (&row)[G], Real *buf, int i, int n) {
+  Real d = Real(1);
+  static_for<0, G>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    if (k < n) {  // uniform
+      buf[i] = row[k];
+      wave_sync();
+      Real c[G];
+      static_for<k, G>([&](auto jc) { c[decltype(jc)::value] = buf[decltype(jc)::value]; });
+      wave_sync();
+      const bool act = i > k && i < n;
+      const Real l = act ? row[k] / c[k] : Real(0);
+      static_for<k + 1, G>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        row[j] = j <= i ? row[j] - l * c[j] : row[j];
+      });
+      if (act) row[k] = l;
+      if (i == k) d = row[k];
+    }
+  });
+  return d;
+}
+
+// R right-hand sides r[m] (lane i holding entry i) -> (L D L^T)^-1 r.
+// Forward substitution reads L from the lane's row; the rows are then
+// transposed through the group's LDS tile so backward substitution reads lane
+// i's column. Each step broadcasts the finished entry with ds_bpermute.
+template <int G, class Real, int R>
+__device__ __forceinline__ void ldlt_solve(const Real (&row)[G], Real d, Real (*A)[G + 1], int base4, int i, int n,
+                                           Real (&r)[R]) {
+  static_for<0, G>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    if (k < n) {
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        const Real yk = bperm(base4 + 4 * k, r[m]);
+        r[m] = i > k ? r[m] - row[k] * yk : r[m];
+      }
+    }
+  });
+#pragma unroll
+  for (int m = 0; m < R; ++m) r[m] = i < n ? r[m] / d : Real(0);
+  static_for<0, G>([&](auto jc) { A[i][decltype(jc)::value] = row[decltype(jc)::value]; });
+  wave_sync();
+  Real col[G];
+  static_for<0, G>([&](auto kc) { col[decltype(kc)::value] = A[decltype(kc)::value][i]; });
+  wave_sync();
+  static_for<0, G>([&](auto kc) {
+    constexpr int k = G - 1 - decltype(kc)::value;
+    if (k < n) {
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        const Real xk = bperm(base4 + 4 * k, r[m]);
+        r[m] = i < k ? r[m] - col[k] * xk : r[m];
+      }
+    }
+  });
+}
+
+// Per group: the transposition tile (rows padded by one word, so lanes
+// walking a row or a column hit different banks) and the column buffer.
 template <int G, class Real> struct DynLds {
   Real A[G][G + 1];
   Real y[G];
@@ -95,9 +133,11 @@ template <int G, class Real> struct TreeFd {
   XfT<Real> X;
   S6<Real> col, v;
 
-  // q, qd in joint order (this lane's joint); writes M (row g) and returns the
-  // bias b_g = RNEA(q, qd, 0)_g. The caller factors L->A.
-  __device__ __forceinline__ Real setup(const ModelTable<Real> *M, Real qj, Real qdj, DynLds<G, Real> &L) {
+  // q, qd in joint order (this lane's joint); loads row with the lower
+  // triangle of row g of M (zeros above the diagonal), assembled in the
+  // group's LDS tile, and returns the bias b_g = RNEA(q, qd, 0)_g.
+  __device__ __forceinline__ Real setup(const ModelTable<Real> *M, Real qj, Real qdj, Real (*A)[G + 1],
+                                        Real (&row)[G]) {
     Real s, c;
     joint_sincos(qj, &s, &c);
     joint_world(*J, qj, s, c, static_cast<const Real *>(nullptr), V3<Real>{0, 0, 0}, true, X.R, X.p);
@@ -128,10 +168,13 @@ template <int G, class Real> struct TreeFd {
     const S6<Real> Fg = apply(Yc, col);
     for (int i = 0; i < nj; ++i) {  // uniform loop: every lane takes part in the permutes
       const S6<Real> ci = bperm6(T.base4 + 4 * i, col);
-      const JointRec<Real> &Ji = joints_of(M)[i];
-      const bool anc = i <= g && g < i + Ji.subtree;  // i ancestor-or-self of g
-      if (valid && i <= g) L.A[g][i] = anc ? dot6(ci, Fg) : Real(0);
+      const bool anc = i <= g && g < i + joints_of(M)[i].subtree;  // i ancestor-or-self of g
+      A[g][i] = valid && anc ? dot6(ci, Fg) : Real(0);
     }
+    for (int i = nj; i < G; ++i) A[g][i] = Real(0);
+    wave_sync();
+    static_for<0, G>([&](auto ic) { row[decltype(ic)::value] = A[g][decltype(ic)::value]; });
+    wave_sync();
     return bias;
   }
 };
@@ -185,9 +228,12 @@ __global__ __launch_bounds__(BLOCK) void k_tree_step(const ModelTable<Real> *__r
   const int src = F.T.base4 + 4 * F.J->vuser;
   Real qj = bperm(src, qu), qdj = bperm(src, qdu), tj = bperm(src, tu);
   if (!F.valid) qj = qdj = tj = Real(0);
-  const Real bias = F.setup(M, qj, qdj, L);
-  ldlt_factor<G, Real>(L.A, g, F.nj);
-  const Real qddj = ldlt_solve<G, Real>(L.A, L.y, g, F.nj, tj - bias);
+  Real row[G];
+  const Real bias = F.setup(M, qj, qdj, L.A, row);
+  const Real d = ldlt_factor<G, Real>(row, L.y, g, F.nj);
+  Real r[1] = {tj - bias};
+  ldlt_solve<G, Real, 1>(row, d, L.A, F.T.base4, g, F.nj, r);
+  const Real qddj = r[0];
   if (!live) return;
   if (mode == 0) {
     if (F.valid) out[e * nv + F.J->vuser] = (float)qddj;
@@ -231,12 +277,14 @@ __global__ __launch_bounds__(BLOCK) void k_tree_step_vjp(const ModelTable<Real> 
   Real qj = bperm(src, qu), qdj = bperm(src, qdu), tj = bperm(src, tu);
   Real aqj = bperm(src, aqu), aqdj = bperm(src, aqdu);
   if (!F.valid) qj = qdj = tj = aqj = aqdj = Real(0);
-  const Real bias = F.setup(M, qj, qdj, L);
-  ldlt_factor<G, Real>(L.A, g, F.nj);
-  const Real qddj = ldlt_solve<G, Real>(L.A, L.y, g, F.nj, tj - bias);
+  Real row[G];
+  const Real bias = F.setup(M, qj, qdj, L.A, row);
+  const Real d = ldlt_factor<G, Real>(row, L.y, g, F.nj);
   const Real dtr = dt;
   const Real cj = aqdj + dtr * aqj;
-  const Real lam = ldlt_solve<G, Real>(L.A, L.y, g, F.nj, dtr * cj);
+  Real r[2] = {tj - bias, dtr * cj};  // qdd and lambda = M^-1 (dt c) in one pass
+  ldlt_solve<G, Real, 2>(row, d, L.A, F.T.base4, g, F.nj, r);
+  const Real qddj = r[0], lam = r[1];
 
   // RNEA vector-Jacobian product at (q, qd, qdd) for the upstream lambda
   const Tree<G, Real> &T = F.T;
