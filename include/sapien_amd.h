@@ -259,6 +259,16 @@ int sapamd_set_dynamics_precision(sapamd_model *m, int32_t mode);
 /* Effective precision (FP32 or FP64) and the cond(M) estimate. */
 int sapamd_get_dynamics_precision(const sapamd_model *m, int32_t *mode, double *cond_estimate);
 
+/* How sapamd_rollout_forward / sapamd_rollout_backward launch for trees of
+ * <= 32 joints under the lanes-per-joint kernels: FUSED (default) runs all T
+ * steps in one kernel with the state in registers (the adjoint recomputes each
+ * K-step segment inside the kernel); PER_STEP issues one launch per step as
+ * sapamd_rollout_step does. Same arithmetic either way. No reference
+ * counterpart: the reference steps its simulator one call at a time
+ * (pinocchio_model.cpp:221-226 for one forward-dynamics evaluation). */
+enum { SAPAMD_ROLLOUT_FUSED = 0, SAPAMD_ROLLOUT_PER_STEP = 1 };
+int sapamd_set_rollout_launch(sapamd_model *m, int32_t mode);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,6 +55,19 @@ def set_kernel_variant(model, name):
     check(lib().sapamd_set_kernel_variant(handle(model), _VARIANTS[name]))
 
 
+_ROLLOUT_LAUNCH = {"fused": 0, "per_step": 1}
+
+
+def set_rollout_launch(model, mode):
+    """'fused' (default): one launch per rollout (and per adjoint) for trees of
+    <= 32 joints under the lanes-per-joint kernels; 'per_step': one launch per step."""
+    from .native import check, handle, lib
+
+    if mode not in _ROLLOUT_LAUNCH:
+        raise ValueError(f"unknown rollout launch mode {mode!r}")
+    check(lib().sapamd_set_rollout_launch(handle(model), _ROLLOUT_LAUNCH[mode]))
+
+
 _PRECISIONS = {"auto": 0, "fp32": 1, "fp64": 2}
 
 
@@ -92,4 +105,4 @@ def _ik_multistart(self, link_index, poses, initial_qpos=None, n_init_qpos=20, a
 PinocchioModel.compute_inverse_kinematics_multistart_batched = _ik_multistart
 
 __all__ = ["PinocchioModel", "Pose", "hetero_batched", "clik_hetero_batched", "LIBRARY_PATH", "native_fk_jac_rnea", "kernel_variant",
-           "set_kernel_variant", "dynamics_precision", "set_dynamics_precision"]
+           "set_kernel_variant", "dynamics_precision", "set_dynamics_precision", "set_rollout_launch"]

@@ -139,6 +139,9 @@ struct sapamd_model {
   bool fd64() const {
     return dyn_precision == SAPAMD_DYNAMICS_FP64 || (dyn_precision == SAPAMD_DYNAMICS_AUTO && cond_est > kCondLimit);
   }
+  // Rollouts under the lanes-per-joint dynamics: one launch for all steps
+  // (default), or one launch per step (the path graph-captured step loops use).
+  bool per_step_rollouts = false;
 
   ~sapamd_model() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -1330,6 +1333,11 @@ int sapamd_rollout_forward(const sapamd_model *m, int64_t B, int32_t T, int32_t 
     need(q, "q");
     need(qd, "qd");
     need(tau, "tau");
+    if (lanes_dynamics(m) && !m->per_step_rollouts) {
+      HIP_OK(launch_tree_rollout(m->t32(), m->fd64() ? m->t64() : nullptr, m->host->njoints(), B, T, K, dt, q, qd,
+                                 tau, ckpt, S(stream)));
+      return;
+    }
     const int nv = m->nv();
     const int64_t S_ = (T + K - 1) / K;
     for (int t = 0; t < T; ++t) {
@@ -1349,6 +1357,11 @@ int sapamd_rollout_backward(const sapamd_model *m, int64_t B, int32_t T, int32_t
     for (const void *p : {(const void *)tau, (const void *)ckpt, (const void *)seg, (const void *)q_tmp,
                           (const void *)qd_tmp, (const void *)aq, (const void *)aqd, (const void *)gtau})
       need(p, "rollout buffer");
+    if (lanes_dynamics(m) && !m->per_step_rollouts) {
+      HIP_OK(launch_tree_rollout_vjp(m->t32(), m->fd64() ? m->t64() : nullptr, m->host->njoints(), B, T, K, dt, tau,
+                                     ckpt, seg, aq, aqd, gtau, S(stream)));
+      return;
+    }
     const int nv = m->nv();
     const int64_t S_ = (T + K - 1) / K;
     for (int64_t s = S_ - 1; s >= 0; --s) {
@@ -1361,6 +1374,15 @@ int sapamd_rollout_backward(const sapamd_model *m, int64_t B, int32_t T, int32_t
         step_vjp_launch(m, B, dt, seg + (int64_t)(t - t0) * 2 * nv, (int64_t)K * 2 * nv, tau + (int64_t)t * nv,
                         (int64_t)T * nv, aq, aqd, gtau + (int64_t)t * nv, stream);
     }
+  });
+}
+
+int sapamd_set_rollout_launch(sapamd_model *m, int32_t mode) {
+  return guarded([&] {
+    if (!m) throw std::invalid_argument("null model");
+    if (mode != SAPAMD_ROLLOUT_FUSED && mode != SAPAMD_ROLLOUT_PER_STEP)
+      throw std::invalid_argument("unknown rollout launch mode");
+    m->per_step_rollouts = mode == SAPAMD_ROLLOUT_PER_STEP;
   });
 }
 

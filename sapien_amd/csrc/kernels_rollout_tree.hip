@@ -203,6 +203,75 @@ __device__ __forceinline__ const ModelTable<Real> *stage_table(const ModelTable<
   return reinterpret_cast<const ModelTable<Real> *>(s_tab);
 }
 
+// Forward dynamics of this lane's joint at (q, qd, tau) (joint order).
+template <int G, class Real>
+__device__ __forceinline__ Real lane_fd(TreeFd<G, Real> &F, const ModelTable<Real> *M, DynLds<G, Real> &L, Real qj,
+                                        Real qdj, Real tj) {
+  Real row[G];
+  const Real bias = F.setup(M, qj, qdj, L.A, row);
+  const Real d = ldlt_factor<G, Real>(row, L.y, F.g, F.nj);
+  Real r[1] = {tj - bias};
+  ldlt_solve<G, Real, 1>(row, d, L.A, F.T.base4, F.g, F.nj, r);
+  return r[0];
+}
+
+// Adjoint of one semi-implicit Euler step at the pre-step state (q, qd) for
+// this lane's joint: (aq, aqd) post-step adjoint in, pre-step adjoint out;
+// returns dL/dtau_j = lambda_j.
+template <int G, class Real>
+__device__ __forceinline__ Real lane_step_vjp(TreeFd<G, Real> &F, const ModelTable<Real> *M, DynLds<G, Real> &L,
+                                              float dt, Real qj, Real qdj, Real tj, Real &aqj, Real &aqdj) {
+  Real row[G];
+  const Real bias = F.setup(M, qj, qdj, L.A, row);
+  const Real d = ldlt_factor<G, Real>(row, L.y, F.g, F.nj);
+  const Real dtr = dt;
+  const Real cj = aqdj + dtr * aqj;
+  Real r[2] = {tj - bias, dtr * cj};  // qdd and lambda = M^-1 (dt c) in one pass
+  ldlt_solve<G, Real, 2>(row, d, L.A, F.T.base4, F.g, F.nj, r);
+  const Real qddj = r[0], lam = r[1];
+
+  // RNEA vector-Jacobian product at (q, qd, qdd) for the upstream lambda
+  const Tree<G, Real> &T = F.T;
+  const S6<Real> col = F.col, v = F.v;
+  S6<Real> a = qddj * col + qdj * crossm(v, col);
+  T.prefix(a);
+  a.l = a.l - V3<Real>{M->gravity[0], M->gravity[1], M->gravity[2]};
+  const WInertia<Real> Y = world_inertia_fast(*F.J, F.X.R, F.X.p);
+  S6<Real> Ff = F.valid ? inertia_apply(Y, a) + crossf(v, inertia_apply(Y, v)) : zero6<Real>();
+  T.subtree(Ff);
+  const Real gj = F.valid ? lam : Real(0);
+  S6<Real> w = gj * col;
+  T.prefix(w);
+  const S6<Real> Iw = inertia_apply(Y, w), Iv = inertia_apply(Y, v);
+  S6<Real> Hw = zero6<Real>(), QS = zero6<Real>();
+  if (F.valid) {
+    Hw = Iw;
+    QS = crossf(w, Iv) - crossf(v, Iw) + inertia_apply(Y, crossm(v, w));
+  }
+  Real acc[12] = {Hw.l.x, Hw.l.y, Hw.l.z, Hw.a.x, Hw.a.y, Hw.a.z, QS.l.x, QS.l.y, QS.l.z, QS.a.x, QS.a.y, QS.a.z};
+  T.template subtree<12>(acc);
+  Hw = {{acc[0], acc[1], acc[2]}, {acc[3], acc[4], acc[5]}};
+  QS = {{acc[6], acc[7], acc[8]}, {acc[9], acc[10], acc[11]}};
+  const S6<Real> wl = w - gj * col, vl = v - qdj * col;
+  const S6<Real> al = a - qddj * col - qdj * crossm(v, col);
+  const Real dq = dot6(col, crossf(vl, QS + crossf(vl, Hw)) - crossf(wl, Ff) - crossf(al, Hw));
+  const Real dqd = -dot6(col, QS + Real(2) * crossf(vl, Hw));
+  aqj = aqj - dq;
+  aqdj = cj - dqd;
+  return lam;
+}
+
+// Shared prologue of the kernels: group lane, environment (clamped for the
+// tail groups), staged table and this lane's joint.
+#define SAPAMD_TREE_DYN_PROLOGUE                                           \
+  __shared__ float4 s_tab[kDynTableWords4<Real>];                          \
+  __shared__ DynLds<G, Real> s_dyn[BLOCK / G];                             \
+  const int g = threadIdx.x & (G - 1);                                     \
+  const int64_t env = (blockIdx.x * (int64_t)BLOCK + threadIdx.x) / G;     \
+  const bool live = env < B;                                               \
+  const int64_t e = live ? env : B - 1;                                    \
+  const int nv = Mg->nv;
+
 // mode 0: forward dynamics qdd = FD(q, qd, tau) into out (B, nv)
 // mode 1: rollout step in place on q, qd; save (pre-step q | qd) when given
 template <int G, class Real, int BLOCK>
@@ -211,13 +280,7 @@ __global__ __launch_bounds__(BLOCK) void k_tree_step(const ModelTable<Real> *__r
                                                      const float *__restrict__ tau, int64_t tau_stride,
                                                      float *__restrict__ out, float *__restrict__ save,
                                                      int64_t save_stride) {
-  __shared__ float4 s_tab[kDynTableWords4<Real>];
-  __shared__ DynLds<G, Real> s_dyn[BLOCK / G];
-  const int g = threadIdx.x & (G - 1);
-  const int64_t env = (blockIdx.x * (int64_t)BLOCK + threadIdx.x) / G;
-  const bool live = env < B;
-  const int64_t e = live ? env : B - 1;
-  const int nv = Mg->nv;
+  SAPAMD_TREE_DYN_PROLOGUE
   // loads first (one round trip): lane g reads user dof g
   const int gc = g < nv ? g : nv - 1;
   const float qu = q[e * nv + gc], qdu = qd[e * nv + gc], tu = tau[e * tau_stride + gc];
@@ -228,12 +291,7 @@ __global__ __launch_bounds__(BLOCK) void k_tree_step(const ModelTable<Real> *__r
   const int src = F.T.base4 + 4 * F.J->vuser;
   Real qj = bperm(src, qu), qdj = bperm(src, qdu), tj = bperm(src, tu);
   if (!F.valid) qj = qdj = tj = Real(0);
-  Real row[G];
-  const Real bias = F.setup(M, qj, qdj, L.A, row);
-  const Real d = ldlt_factor<G, Real>(row, L.y, g, F.nj);
-  Real r[1] = {tj - bias};
-  ldlt_solve<G, Real, 1>(row, d, L.A, F.T.base4, g, F.nj, r);
-  const Real qddj = r[0];
+  const Real qddj = lane_fd<G, Real>(F, M, L, qj, qdj, tj);
   if (!live) return;
   if (mode == 0) {
     if (F.valid) out[e * nv + F.J->vuser] = (float)qddj;
@@ -259,13 +317,7 @@ __global__ __launch_bounds__(BLOCK) void k_tree_step_vjp(const ModelTable<Real> 
                                                          const float *__restrict__ tau, int64_t tau_stride,
                                                          float *__restrict__ aq, float *__restrict__ aqd,
                                                          float *__restrict__ gtau) {
-  __shared__ float4 s_tab[kDynTableWords4<Real>];
-  __shared__ DynLds<G, Real> s_dyn[BLOCK / G];
-  const int g = threadIdx.x & (G - 1);
-  const int64_t env = (blockIdx.x * (int64_t)BLOCK + threadIdx.x) / G;
-  const bool live = env < B;
-  const int64_t e = live ? env : B - 1;
-  const int nv = Mg->nv;
+  SAPAMD_TREE_DYN_PROLOGUE
   const int gc = g < nv ? g : nv - 1;
   const float qu = state[e * state_stride + gc], qdu = state[e * state_stride + nv + gc];
   const float tu = tau[e * tau_stride + gc], aqu = aq[e * nv + gc], aqdu = aqd[e * nv + gc];
@@ -277,67 +329,133 @@ __global__ __launch_bounds__(BLOCK) void k_tree_step_vjp(const ModelTable<Real> 
   Real qj = bperm(src, qu), qdj = bperm(src, qdu), tj = bperm(src, tu);
   Real aqj = bperm(src, aqu), aqdj = bperm(src, aqdu);
   if (!F.valid) qj = qdj = tj = aqj = aqdj = Real(0);
-  Real row[G];
-  const Real bias = F.setup(M, qj, qdj, L.A, row);
-  const Real d = ldlt_factor<G, Real>(row, L.y, g, F.nj);
-  const Real dtr = dt;
-  const Real cj = aqdj + dtr * aqj;
-  Real r[2] = {tj - bias, dtr * cj};  // qdd and lambda = M^-1 (dt c) in one pass
-  ldlt_solve<G, Real, 2>(row, d, L.A, F.T.base4, g, F.nj, r);
-  const Real qddj = r[0], lam = r[1];
-
-  // RNEA vector-Jacobian product at (q, qd, qdd) for the upstream lambda
-  const Tree<G, Real> &T = F.T;
-  const S6<Real> col = F.col, v = F.v;
-  S6<Real> a = qddj * col + qdj * crossm(v, col);
-  T.prefix(a);
-  a.l = a.l - V3<Real>{M->gravity[0], M->gravity[1], M->gravity[2]};
-  const WInertia<Real> Y = world_inertia_fast(*F.J, F.X.R, F.X.p);
-  S6<Real> Ff = F.valid ? inertia_apply(Y, a) + crossf(v, inertia_apply(Y, v)) : zero6<Real>();
-  T.subtree(Ff);
-  const Real gj = F.valid ? lam : Real(0);
-  S6<Real> w = gj * col;
-  T.prefix(w);
-  const S6<Real> Iw = inertia_apply(Y, w), Iv = inertia_apply(Y, v);
-  S6<Real> Hw = zero6<Real>(), QS = zero6<Real>();
-  if (F.valid) {
-    Hw = Iw;
-    QS = crossf(w, Iv) - crossf(v, Iw) + inertia_apply(Y, crossm(v, w));
-  }
-  Real acc[12] = {Hw.l.x, Hw.l.y, Hw.l.z, Hw.a.x, Hw.a.y, Hw.a.z, QS.l.x, QS.l.y, QS.l.z, QS.a.x, QS.a.y, QS.a.z};
-  T.template subtree<12>(acc);
-  Hw = {{acc[0], acc[1], acc[2]}, {acc[3], acc[4], acc[5]}};
-  QS = {{acc[6], acc[7], acc[8]}, {acc[9], acc[10], acc[11]}};
+  const Real lam = lane_step_vjp<G, Real>(F, M, L, dt, qj, qdj, tj, aqj, aqdj);
   if (!live || !F.valid) return;
-  const S6<Real> wl = w - gj * col, vl = v - qdj * col;
-  const S6<Real> al = a - qddj * col - qdj * crossm(v, col);
-  const Real dq = dot6(col, crossf(vl, QS + crossf(vl, Hw)) - crossf(wl, Ff) - crossf(al, Hw));
-  const Real dqd = -dot6(col, QS + Real(2) * crossf(vl, Hw));
   const int u = F.J->vuser;
-  aq[e * nv + u] = (float)(aqj - dq);
-  aqd[e * nv + u] = (float)(cj - dqd);
+  aq[e * nv + u] = (float)aqj;
+  aqd[e * nv + u] = (float)aqdj;
   gtau[e * tau_stride + u] = (float)lam;
+}
+
+// T steps of semi-implicit Euler in one launch, the state held in registers
+// (lane j: q_j, qd_j in FP32, exactly as the one-step kernel rounds it).
+// tau (B, T, nv); ckpt (B, ceil(T / K), 2 nv) receives the pre-step state of
+// steps 0, K, 2K, ... when given; q, qd (B, nv) hold the initial state on entry
+// and the final state on exit.
+template <int G, class Real, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_tree_rollout(const ModelTable<Real> *__restrict__ Mg, int64_t B, int T,
+                                                        int K, float dt, float *__restrict__ q,
+                                                        float *__restrict__ qd, const float *__restrict__ tau,
+                                                        float *__restrict__ ckpt) {
+  SAPAMD_TREE_DYN_PROLOGUE
+  const ModelTable<Real> *M = stage_table(Mg, s_tab);
+  DynLds<G, Real> &L = s_dyn[threadIdx.x / G];
+  TreeFd<G, Real> F;
+  init_fd<G, Real>(F, M, g);
+  const int u = F.J->vuser;  // lanes past the last joint repeat its dof and store nothing
+  const bool owner = live && F.valid;
+  const int64_t C = (T + K - 1) / K;
+  const float *tr = tau + e * (int64_t)T * nv + u;
+  float qs = q[e * nv + u], qds = qd[e * nv + u];
+  float tnext = tr[0];
+  for (int t = 0; t < T; ++t) {
+    const float tcur = tnext;
+    if (t + 1 < T) tnext = tr[(int64_t)(t + 1) * nv];
+    if (ckpt && owner && t % K == 0) {
+      float *c = ckpt + (e * C + t / K) * 2 * nv;
+      c[u] = qs;
+      c[nv + u] = qds;
+    }
+    const Real qddj = lane_fd<G, Real>(F, M, L, F.valid ? Real(qs) : Real(0), F.valid ? Real(qds) : Real(0),
+                                       F.valid ? Real(tcur) : Real(0));
+    const float v1 = qds + dt * (float)qddj;
+    qs = qs + dt * v1;
+    qds = v1;
+  }
+  if (owner) {
+    q[e * nv + u] = qs;
+    qd[e * nv + u] = qds;
+  }
+}
+
+// Adjoint of k_tree_rollout in one launch: segments of K steps from the last,
+// each recomputed from its checkpoint with the pre-step states kept in seg
+// (B, K, 2 nv; each lane reads back only what it wrote), then the step
+// adjoints in reverse. aq, aqd: dL/d(final state) in, dL/d(initial state)
+// out; gtau (B, T, nv) = dL/dtau.
+template <int G, class Real, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_tree_rollout_vjp(const ModelTable<Real> *__restrict__ Mg, int64_t B, int T,
+                                                            int K, float dt, const float *__restrict__ tau,
+                                                            const float *__restrict__ ckpt, float *__restrict__ seg,
+                                                            float *__restrict__ aq, float *__restrict__ aqd,
+                                                            float *__restrict__ gtau) {
+  SAPAMD_TREE_DYN_PROLOGUE
+  const ModelTable<Real> *M = stage_table(Mg, s_tab);
+  DynLds<G, Real> &L = s_dyn[threadIdx.x / G];
+  TreeFd<G, Real> F;
+  init_fd<G, Real>(F, M, g);
+  const int u = F.J->vuser;
+  const bool owner = live && F.valid;
+  const int C = (T + K - 1) / K;
+  const float *tr = tau + e * (int64_t)T * nv + u;
+  float *gr = gtau + e * (int64_t)T * nv + u;
+  float *sg = seg + e * (int64_t)K * 2 * nv + u;
+  Real aqj = F.valid ? Real(aq[e * nv + u]) : Real(0), aqdj = F.valid ? Real(aqd[e * nv + u]) : Real(0);
+  for (int c = C - 1; c >= 0; --c) {
+    const int t0 = c * K, t1 = min(T, t0 + K);
+    const float *ck = ckpt + ((int64_t)e * C + c) * 2 * nv + u;
+    float qs = ck[0], qds = ck[nv];
+    for (int t = t0; t < t1; ++t) {
+      if (owner) {
+        sg[(int64_t)(t - t0) * 2 * nv] = qs;
+        sg[(int64_t)(t - t0) * 2 * nv + nv] = qds;
+      }
+      if (t + 1 == t1) break;  // the last pre-step state needs no step after it
+      const Real qddj = lane_fd<G, Real>(F, M, L, F.valid ? Real(qs) : Real(0), F.valid ? Real(qds) : Real(0),
+                                         F.valid ? Real(tr[(int64_t)t * nv]) : Real(0));
+      const float v1 = qds + dt * (float)qddj;
+      qs = qs + dt * v1;
+      qds = v1;
+    }
+    for (int t = t1 - 1; t >= t0; --t) {
+      const Real qj = F.valid ? Real(sg[(int64_t)(t - t0) * 2 * nv]) : Real(0);
+      const Real qdj = F.valid ? Real(sg[(int64_t)(t - t0) * 2 * nv + nv]) : Real(0);
+      const Real tj = F.valid ? Real(tr[(int64_t)t * nv]) : Real(0);
+      const Real lam = lane_step_vjp<G, Real>(F, M, L, dt, qj, qdj, tj, aqj, aqdj);
+      // the one-step kernel hands the adjoint on in FP32: round the same way
+      aqj = F.valid ? Real((float)aqj) : Real(0);
+      aqdj = F.valid ? Real((float)aqdj) : Real(0);
+      if (owner) gr[(int64_t)t * nv] = (float)lam;
+    }
+  }
+  if (owner) {
+    aq[e * nv + u] = (float)aqj;
+    aqd[e * nv + u] = (float)aqdj;
+  }
 }
 
 // FP32: 256-thread blocks; FP64: 128-thread blocks (half the LDS per block).
 template <class Real> constexpr int kStepBlock = sizeof(Real) == 4 ? 256 : 128;
 
+// Launches KERNEL<G, Real, BL> for the group size of nj joints.
+#define SAPAMD_TREE_DYN_LAUNCH(KERNEL, ...)                                                          \
+  do {                                                                                               \
+    constexpr int BL = kStepBlock<Real>;                                                             \
+    const int G = tree_dyn_group(nj);                                                                \
+    const dim3 grid((unsigned)((B * G + BL - 1) / BL));                                              \
+    if (G == 8)                                                                                      \
+      hipLaunchKernelGGL((KERNEL<8, Real, BL>), grid, dim3(BL), 0, s, __VA_ARGS__);                  \
+    else if (G == 16)                                                                                \
+      hipLaunchKernelGGL((KERNEL<16, Real, BL>), grid, dim3(BL), 0, s, __VA_ARGS__);                 \
+    else                                                                                             \
+      hipLaunchKernelGGL((KERNEL<32, Real, BL>), grid, dim3(BL), 0, s, __VA_ARGS__);                 \
+  } while (0)
+
 template <class Real>
 hipError_t step_impl(const ModelTable<Real> *tab, int nj, int64_t B, int mode, float dt, float *q, float *qd,
                      const float *tau, int64_t tau_stride, float *out, float *save, int64_t save_stride,
                      hipStream_t s) {
-  constexpr int BL = kStepBlock<Real>;
-  const int G = tree_dyn_group(nj);
-  const dim3 grid((unsigned)((B * G + BL - 1) / BL));
-  if (G == 8)
-    hipLaunchKernelGGL((k_tree_step<8, Real, BL>), grid, dim3(BL), 0, s, tab, B, mode, dt, q, qd, tau, tau_stride, out,
-                       save, save_stride);
-  else if (G == 16)
-    hipLaunchKernelGGL((k_tree_step<16, Real, BL>), grid, dim3(BL), 0, s, tab, B, mode, dt, q, qd, tau, tau_stride,
-                       out, save, save_stride);
-  else
-    hipLaunchKernelGGL((k_tree_step<32, Real, BL>), grid, dim3(BL), 0, s, tab, B, mode, dt, q, qd, tau, tau_stride,
-                       out, save, save_stride);
+  SAPAMD_TREE_DYN_LAUNCH(k_tree_step, tab, B, mode, dt, q, qd, tau, tau_stride, out, save, save_stride);
   return hipGetLastError();
 }
 
@@ -345,18 +463,21 @@ template <class Real>
 hipError_t step_vjp_impl(const ModelTable<Real> *tab, int nj, int64_t B, float dt, const float *state,
                          int64_t state_stride, const float *tau, int64_t tau_stride, float *aq, float *aqd,
                          float *gtau, hipStream_t s) {
-  constexpr int BL = kStepBlock<Real>;
-  const int G = tree_dyn_group(nj);
-  const dim3 grid((unsigned)((B * G + BL - 1) / BL));
-  if (G == 8)
-    hipLaunchKernelGGL((k_tree_step_vjp<8, Real, BL>), grid, dim3(BL), 0, s, tab, B, dt, state, state_stride, tau,
-                       tau_stride, aq, aqd, gtau);
-  else if (G == 16)
-    hipLaunchKernelGGL((k_tree_step_vjp<16, Real, BL>), grid, dim3(BL), 0, s, tab, B, dt, state, state_stride, tau,
-                       tau_stride, aq, aqd, gtau);
-  else
-    hipLaunchKernelGGL((k_tree_step_vjp<32, Real, BL>), grid, dim3(BL), 0, s, tab, B, dt, state, state_stride, tau,
-                       tau_stride, aq, aqd, gtau);
+  SAPAMD_TREE_DYN_LAUNCH(k_tree_step_vjp, tab, B, dt, state, state_stride, tau, tau_stride, aq, aqd, gtau);
+  return hipGetLastError();
+}
+
+template <class Real>
+hipError_t rollout_impl(const ModelTable<Real> *tab, int nj, int64_t B, int T, int K, float dt, float *q, float *qd,
+                        const float *tau, float *ckpt, hipStream_t s) {
+  SAPAMD_TREE_DYN_LAUNCH(k_tree_rollout, tab, B, T, K, dt, q, qd, tau, ckpt);
+  return hipGetLastError();
+}
+
+template <class Real>
+hipError_t rollout_vjp_impl(const ModelTable<Real> *tab, int nj, int64_t B, int T, int K, float dt, const float *tau,
+                            const float *ckpt, float *seg, float *aq, float *aqd, float *gtau, hipStream_t s) {
+  SAPAMD_TREE_DYN_LAUNCH(k_tree_rollout_vjp, tab, B, T, K, dt, tau, ckpt, seg, aq, aqd, gtau);
   return hipGetLastError();
 }
 
@@ -380,6 +501,24 @@ hipError_t launch_tree_step_vjp(const ModelTable<float> *tab, const ModelTable<d
   if (!tree_dyn_group(nj)) return hipErrorNotSupported;
   return tab64 ? step_vjp_impl<double>(tab64, nj, B, dt, state, state_stride, tau, tau_stride, aq, aqd, gtau, s)
                : step_vjp_impl<float>(tab, nj, B, dt, state, state_stride, tau, tau_stride, aq, aqd, gtau, s);
+}
+
+hipError_t launch_tree_rollout(const ModelTable<float> *tab, const ModelTable<double> *tab64, int nj, int64_t B,
+                               int T, int K, float dt, float *q, float *qd, const float *tau, float *ckpt,
+                               hipStream_t s) {
+  if (B <= 0 || T <= 0) return hipSuccess;
+  if (!tree_dyn_group(nj) || K < 1) return hipErrorNotSupported;
+  return tab64 ? rollout_impl<double>(tab64, nj, B, T, K, dt, q, qd, tau, ckpt, s)
+               : rollout_impl<float>(tab, nj, B, T, K, dt, q, qd, tau, ckpt, s);
+}
+
+hipError_t launch_tree_rollout_vjp(const ModelTable<float> *tab, const ModelTable<double> *tab64, int nj, int64_t B,
+                                   int T, int K, float dt, const float *tau, const float *ckpt, float *seg, float *aq,
+                                   float *aqd, float *gtau, hipStream_t s) {
+  if (B <= 0 || T <= 0) return hipSuccess;
+  if (!tree_dyn_group(nj) || K < 1) return hipErrorNotSupported;
+  return tab64 ? rollout_vjp_impl<double>(tab64, nj, B, T, K, dt, tau, ckpt, seg, aq, aqd, gtau, s)
+               : rollout_vjp_impl<float>(tab, nj, B, T, K, dt, tau, ckpt, seg, aq, aqd, gtau, s);
 }
 
 }  // namespace sapamd

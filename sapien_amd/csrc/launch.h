@@ -181,6 +181,14 @@ hipError_t launch_tree_step(const ModelTable<float> *tab, const ModelTable<doubl
 hipError_t launch_tree_step_vjp(const ModelTable<float> *tab, const ModelTable<double> *tab64, int nj, int64_t B,
                                 float dt, const float *state, int64_t state_stride, const float *tau,
                                 int64_t tau_stride, float *aq, float *aqd, float *gtau, hipStream_t s);
+// T rollout steps in one launch (state in registers; ckpt (B, ceil(T/K), 2 nv)
+// optional), and its checkpointed adjoint in one launch (seg (B, K, 2 nv)).
+hipError_t launch_tree_rollout(const ModelTable<float> *tab, const ModelTable<double> *tab64, int nj, int64_t B,
+                               int T, int K, float dt, float *q, float *qd, const float *tau, float *ckpt,
+                               hipStream_t s);
+hipError_t launch_tree_rollout_vjp(const ModelTable<float> *tab, const ModelTable<double> *tab64, int nj, int64_t B,
+                                   int T, int K, float dt, const float *tau, const float *ckpt, float *seg, float *aq,
+                                   float *aqd, float *gtau, hipStream_t s);
 // rows of (q | qd) at stride -> q, qd (B, nv)
 hipError_t launch_restore_state(int64_t B, int nv, const float *src, int64_t stride, float *q, float *qd,
                                 hipStream_t s);

@@ -95,6 +95,32 @@ def test_rollout_adjoint_dot_product(tree, variant):
         assert abs(lhs - rhs) <= 1e-3 * max(scale, 1e-12), (b, lhs, rhs)
 
 
+@pytest.mark.parametrize("tree", ["chain7", "arm9", "quad18", "torso30"])
+def test_fused_rollout_matches_per_step(tree):
+    """All T steps in one launch (state in registers, segments recomputed
+    inside the adjoint kernel) against one launch per step: final state,
+    checkpoints and all three gradients. 37 rollouts (a partly filled block),
+    T = 29 with K = 8 (a short last segment)."""
+    import sapien_amd as sa
+
+    rob = S.robot(tree)
+    m, _ = _pair(rob)
+    B, T, K = 37, 29, 8
+    q0, qd0, tau = _problem(rob, B, T, 1510)
+    g = _t(np.random.default_rng(11).normal(size=(B, rob.nv)))
+    res = {}
+    for mode in ("per_step", "fused"):
+        sa.set_rollout_launch(m, mode)
+        qT, qdT, ck = m.rollout_batched(_t(q0), _t(qd0), _t(tau), DT, K)
+        grads = m.rollout_backward_batched(_t(tau), ck, DT, g, g, K)
+        res[mode] = [x.cpu().numpy() for x in (qT, qdT, ck, *grads)]
+    names = ("qT", "qdT", "ckpt", "dq0", "dqd0", "dtau")
+    for name, a, b in zip(names, res["per_step"], res["fused"]):
+        e = rel_err(b, a)
+        print(f"{tree} {name}: rel diff {e:.1e}")
+        assert e <= 1e-5, (name, e)
+
+
 def test_rollout_checkpoint_interval_invariance():
     """The gradient does not depend on the checkpoint interval (1, 5, 16, > T)."""
     rob = S.chain7()
