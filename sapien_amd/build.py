@@ -26,7 +26,7 @@ OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SAPAMD_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels_generic.hip", "kernels_fast.hip", "kernels_adjoint.hip", "kernels_chain_vjp.hip", "kernels_tree.hip", "kernels_rollout.hip", "kernels_rollout_tree.hip", "abi.cpp"]
+HIP_SOURCES = ["kernels_generic.hip", "kernels_fast.hip", "kernels_adjoint.hip", "kernels_chain_vjp.hip", "kernels_tree.hip", "kernels_rollout.hip", "kernels_rollout_tree.hip", "kernels_clik_lanes.hip", "abi.cpp"]
 HOST_SOURCES = ["model.cpp"]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-Wno-unused-command-line-argument", "-I" + os.path.join(ROOT, "include")]

@@ -1028,6 +1028,15 @@ int sapamd_hetero_batched(const sapamd_model *const *models, int32_t nmodels, co
   });
 }
 
+namespace {
+// Persistent blocks of the lanes-per-joint CLIK: two 256-thread blocks per CU
+// (two wavefronts per SIMD at its register count), fewer for small batches.
+int clik_lanes_blocks(int64_t total, int max_chain, int cus) {
+  const int per_block = 256 / clik_lanes_group(max_chain);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(2 * cus, (total + per_block - 1) / per_block));
+}
+}  // namespace
+
 int sapamd_clik_hetero_batched(const sapamd_model *const *models, int32_t nmodels, const int64_t *B,
                                const int32_t *links, const float *const *targets, const float *const *q0,
                                const int32_t *const *masks, double eps, int32_t max_iterations, double dt,
@@ -1084,8 +1093,14 @@ int sapamd_clik_hetero_batched(const sapamd_model *const *models, int32_t nmodel
     int cus = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, models[0]->device) == hipSuccess) cus = prop.multiProcessorCount;
-    const int64_t waves = std::min<int64_t>(4 * cus, (A.total + 63) / 64);
     HIP_OK(launch_fill_i32(A.counter, 0, 1, S(stream)));
+    bool lanes = clik_lanes_group(max_chain) != 0 && A.total <= INT32_MAX;
+    for (int i = 0; i < nmodels; ++i) lanes = lanes && models[i]->variant != SAPAMD_KERNEL_THREAD_PER_ENV;
+    if (lanes) {
+      HIP_OK(launch_clik_lanes(A, max_chain, clik_lanes_blocks(A.total, max_chain, cus), S(stream)));
+      return;
+    }
+    const int64_t waves = std::min<int64_t>(4 * cus, (A.total + 63) / 64);
     HIP_OK(launch_clik_hetero(A, max_chain, (int)waves, S(stream)));
   });
 }
@@ -1154,6 +1169,29 @@ void clik_launch(const sapamd_model *m, int64_t B, int64_t nstart, int32_t link,
   int dev_cus = 256;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, m->device) == hipSuccess) dev_cus = prop.multiProcessorCount;
+  if (clik_lanes_group(P.nchain) && total <= INT32_MAX && m->variant != SAPAMD_KERNEL_THREAD_PER_ENV) {
+    // one segment of the lanes-per-joint CLIK (problems taken in index order,
+    // so multi-start problems run start-major)
+    HeteroClikArgs A{};
+    ClikSegment &g = A.seg[0];
+    g.tab = m->t32();
+    g.P = P;
+    g.target = target;
+    g.q0 = q0;
+    g.q_out = q_out;
+    g.err = err;
+    g.success = success;
+    g.iters = iterations;
+    g.B = total;
+    g.first = 0;
+    A.nseg = 1;
+    A.total = total;
+    A.counter = m->d_counter;
+    HIP_OK(launch_fill_i32(m->d_counter, 0, 1, S(stream)));
+    if (solved) HIP_OK(launch_fill_i32(solved, INT32_MAX, B, S(stream)));
+    HIP_OK(launch_clik_lanes(A, P.nchain, clik_lanes_blocks(total, P.nchain, dev_cus), S(stream)));
+    return;
+  }
   const int64_t waves = (total + 63) / 64;
   // Persistent launch with a work queue once there are more problems than
   // resident lanes (and always for multi-start, so that starts are taken in

@@ -117,6 +117,12 @@ struct HeteroClikArgs {
   int32_t *counter;
 };
 hipError_t launch_clik_hetero(const HeteroClikArgs &A, int max_chain, int waves, hipStream_t s);
+// Lanes-per-joint CLIK (FP32, chains of <= 16 joints): one problem per group of
+// 8 or 16 lanes, persistent blocks of 256 threads pulling problems from
+// A.counter (zeroed before the launch). Serves the per-model batched and the
+// mixed-tree launches; clik_lanes_group returns 0 above 16 chain joints.
+int clik_lanes_group(int max_chain);
+hipError_t launch_clik_lanes(const HeteroClikArgs &A, int max_chain, int blocks, hipStream_t s);
 
 // Lanes-per-joint fused FK + Jacobian + RNEA for serial chains of <= 16 joints
 // (FP32), G lanes per environment x C joints per lane. Returns

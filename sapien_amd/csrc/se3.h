@@ -1,0 +1,141 @@
+// SE(3) helpers of the CLIK kernels (device, templated on float/double):
+// body-frame images of motion vectors, the SE(3) log and its Jacobian, and the
+// damped 6x6 solve, as used by compute_inverse_kinematics
+// (pinocchio_model.cpp:228-300).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "spatial.h"
+
+namespace sapamd {
+
+// Body (local) frame image of a world motion vector under link pose (R, p):
+// Ad(M)^-1 m = [R^T (l - p x a); R^T a].
+template <class T> __device__ __forceinline__ S6<T> to_local(const T *R, V3<T> p, S6<T> m) {
+  return {mtv(R, m.l - cross(p, m.a)), mtv(R, m.a)};
+}
+
+template <class T> __device__ __forceinline__ void vec_of(S6<T> x, T *v) {
+  v[0] = x.l.x; v[1] = x.l.y; v[2] = x.l.z; v[3] = x.a.x; v[4] = x.a.y; v[5] = x.a.z;
+}
+
+// ---------------------------------------------------------------- SE(3) log for CLIK
+// Rotation log with theta = atan2(|axial|, (tr - 1) / 2), well conditioned at
+// every angle; near theta = pi the axis comes from the largest diagonal entry.
+template <class T> __device__ __forceinline__ V3<T> log3_dev(const T *R, T &theta) {
+  const T PI = T(3.14159265358979323846);
+  T c = (R[0] + R[4] + R[8] - T(1)) * T(0.5);
+  V3<T> ax = {T(0.5) * (R[7] - R[5]), T(0.5) * (R[2] - R[6]), T(0.5) * (R[3] - R[1])};
+  T s = sqrt(dot(ax, ax));
+  theta = atan2(s, c);
+  if (theta < PI - T(1e-2)) {
+    T k = s > T(1e-7) ? theta / s : T(1) + theta * theta / T(6);
+    return k * ax;
+  }
+  T omc = T(1) - c;
+  T d0 = R[0] - c, d1 = R[4] - c, d2 = R[8] - c;
+  V3<T> k;
+  if (d0 >= d1 && d0 >= d2) {
+    k.x = sqrt(fmax(d0 / omc, T(0)));
+    k.y = (R[1] + R[3]) / (T(2) * omc * k.x);
+    k.z = (R[2] + R[6]) / (T(2) * omc * k.x);
+  } else if (d1 >= d2) {
+    k.y = sqrt(fmax(d1 / omc, T(0)));
+    k.x = (R[1] + R[3]) / (T(2) * omc * k.y);
+    k.z = (R[5] + R[7]) / (T(2) * omc * k.y);
+  } else {
+    k.z = sqrt(fmax(d2 / omc, T(0)));
+    k.x = (R[2] + R[6]) / (T(2) * omc * k.z);
+    k.y = (R[5] + R[7]) / (T(2) * omc * k.z);
+  }
+  if (dot(k, ax) < T(0)) k = T(-1) * k;
+  return theta * k;
+}
+
+// alpha = (t/2) cot(t/2), beta = (1 - alpha) / t^2, dbeta = beta'(t) / t; series below 0.5.
+template <class T> __device__ __forceinline__ void log_coeffs_dev(T t, T &alpha, T &beta, T &dbeta) {
+  T t2 = t * t;
+  if (t < T(0.5)) {
+    alpha = T(1) - t2 * (T(1) / T(12) + t2 * (T(1) / T(720) + t2 * (T(1) / T(30240) + t2 / T(1209600))));
+    beta = T(1) / T(12) + t2 * (T(1) / T(720) + t2 * (T(1) / T(30240) + t2 * (T(1) / T(1209600) + t2 / T(47900160))));
+    dbeta = T(1) / T(360) + t2 * (T(1) / T(7560) + t2 * (T(1) / T(201600) + t2 / T(5987520)));
+  } else {
+    T st, ct;
+    sincos_t(t, &st, &ct);
+    T omc = T(1) - ct;
+    alpha = t * st / (T(2) * omc);
+    beta = T(1) / t2 - st / (T(2) * t * omc);
+    dbeta = T(-2) / (t2 * t2) + (T(1) + st / t) / (T(2) * t2 * omc);
+  }
+}
+
+template <class T>
+__device__ __forceinline__ S6<T> log6_dev(const T *R, V3<T> p) {
+  T t;
+  V3<T> w = log3_dev(R, t);
+  T al, be, db;
+  log_coeffs_dev(t, al, be, db);
+  return {al * p - T(0.5) * cross(w, p) + (be * dot(w, p)) * w, w};
+}
+
+// Jlog6 (6x6 row-major): [[A, C A], [0, A]], A = alpha I + beta w w^T + [w]x / 2,
+// C = dbeta (w.p) w w^T - (t^2 dbeta + 2 beta) p w^T + beta w p^T + beta (w.p) I + [p]x / 2.
+template <class T> __device__ __forceinline__ void jlog6_dev(const T *R, V3<T> p, T *Jl) {
+  T t;
+  V3<T> w = log3_dev(R, t);
+  T al, be, db;
+  log_coeffs_dev(t, al, be, db);
+  T wv[3] = {w.x, w.y, w.z}, pv[3] = {p.x, p.y, p.z};
+  T A[9], C[9];
+  T wp = dot(w, p);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      A[3 * i + j] = be * wv[i] * wv[j] + (i == j ? al : T(0));
+      C[3 * i + j] = db * wp * wv[i] * wv[j] - (t * t * db + T(2) * be) * pv[i] * wv[j] + be * wv[i] * pv[j] +
+                     (i == j ? be * wp : T(0));
+    }
+  A[1] -= T(0.5) * w.z; A[2] += T(0.5) * w.y; A[5] -= T(0.5) * w.x;
+  A[3] += T(0.5) * w.z; A[6] -= T(0.5) * w.y; A[7] += T(0.5) * w.x;
+  C[1] -= T(0.5) * p.z; C[2] += T(0.5) * p.y; C[5] -= T(0.5) * p.x;
+  C[3] += T(0.5) * p.z; C[6] -= T(0.5) * p.y; C[7] += T(0.5) * p.x;
+  T CA[9];
+  mm(C, A, CA);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      Jl[i * 6 + j] = A[3 * i + j];
+      Jl[i * 6 + 3 + j] = CA[3 * i + j];
+      Jl[(3 + i) * 6 + j] = T(0);
+      Jl[(3 + i) * 6 + 3 + j] = A[3 * i + j];
+    }
+}
+
+// y = (J J^T + damp I)^-1 e by LDL^T of the 6x6 SPD matrix.
+template <class T> __device__ __forceinline__ void ldlt6_solve(T *A, T *y) {
+  for (int j = 0; j < 6; ++j) {
+    T d = A[j * 6 + j];
+    for (int k = 0; k < j; ++k) d -= A[j * 6 + k] * A[j * 6 + k] * A[k * 6 + k];
+    A[j * 6 + j] = d;
+    for (int i = j + 1; i < 6; ++i) {
+      T s = A[i * 6 + j];
+      for (int k = 0; k < j; ++k) s -= A[i * 6 + k] * A[j * 6 + k] * A[k * 6 + k];
+      A[i * 6 + j] = s / d;
+    }
+  }
+  for (int i = 0; i < 6; ++i)
+    for (int k = 0; k < i; ++k) y[i] -= A[i * 6 + k] * y[k];
+  for (int i = 0; i < 6; ++i) y[i] /= A[i * 6 + i];
+  for (int i = 5; i >= 0; --i)
+    for (int k = i + 1; k < 6; ++k) y[i] -= A[k * 6 + i] * y[k];
+}
+
+template <class T> __device__ __forceinline__ void pose_to_mat(const T *P, T *R, V3<T> &p) {
+  T w = P[3], x = P[4], y = P[5], z = P[6];
+  T n = T(1) / sqrt(w * w + x * x + y * y + z * z);
+  w *= n; x *= n; y *= n; z *= n;
+  R[0] = T(1) - T(2) * (y * y + z * z); R[1] = T(2) * (x * y - w * z); R[2] = T(2) * (x * z + w * y);
+  R[3] = T(2) * (x * y + w * z); R[4] = T(1) - T(2) * (x * x + z * z); R[5] = T(2) * (y * z - w * x);
+  R[6] = T(2) * (x * z - w * y); R[7] = T(2) * (y * z + w * x); R[8] = T(1) - T(2) * (x * x + y * y);
+  p = {P[0], P[1], P[2]};
+}
+
+}  // namespace sapamd

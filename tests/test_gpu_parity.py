@@ -212,9 +212,14 @@ def test_batched_edge_cases():
         m.compute_forward_kinematics_batched(_t(np.zeros((4, 8))))
 
 
-def test_batched_clik():
+@pytest.mark.parametrize("variant", ["auto", "thread_per_env"])
+def test_batched_clik(variant):
+    """Batched CLIK on chain7, lanes-per-joint (auto) and thread-per-problem."""
+    import sapien_amd as sa
+
     rob = S.chain7()
     m, o = _pair(rob)
+    sa.set_kernel_variant(m, variant)
     B = 4096
     qs = S.samples(rob, B, 1004)[0]
     tgt = o.link_poses(qs)[:, 7]
@@ -354,6 +359,65 @@ def test_heterogeneous_batch_one_launch():
         assert rel_err(out["J"].cpu().numpy(), J) <= REL_TOL
         assert rel_err(out["tau"].cpu().numpy(), T) <= REL_TOL
         assert rel_err(out["M"].cpu().numpy(), M) <= REL_TOL
+
+
+@pytest.mark.parametrize("tree", ["chain7", "arm9", "quad18", "torso30"])
+def test_clik_kernel_families_agree(tree):
+    """Lanes-per-joint CLIK (one problem per 8 or 16 lanes) against the
+    thread-per-problem kernel and the FP64 oracle. The algorithm is the same up
+    to the association order of the FK products and J J^T sums.
+    Well-conditioned problems (q0 within 0.05 rad of a solution): four
+    iterations agree with each other and with the oracle to rounding.
+    Random starts with joint 0 masked: damped least squares (damp 1e-6) near
+    singular poses amplifies the FP32 differences of the first error (1e-5,
+    largest where the rotation error is near pi) by the conditioning of J J^T,
+    so individual trajectories part; the success rates match, and every
+    converged problem of either kernel reaches its target with the masked
+    joint left at q0."""
+    import sapien_amd as sa
+
+    rob = S.robot(tree)
+    m, o = _pair(rob)
+    link = rob.ee_link
+    # well conditioned: four iterations against each other and the oracle
+    B = 256
+    qs = S.samples(rob, B, 1722)[0]
+    q0n = (qs + np.random.default_rng(1723).normal(scale=0.05, size=qs.shape)).astype(np.float32).astype(np.float64)
+    t = o.link_poses(qs)[:, link].astype(np.float32).astype(np.float64)
+    short = {}
+    for variant in ("auto", "thread_per_env"):
+        sa.set_kernel_variant(m, variant)
+        short[variant] = [x.cpu().numpy() for x in
+                          m.compute_inverse_kinematics_batched(link, _t(t), _t(q0n), max_iterations=4)]
+    (Ql, okl, el, itl), (Qt, okt, et, itt) = short["auto"], short["thread_per_env"]
+    d = np.abs(Ql.astype(np.float64) - Qt).max()
+    dor = max(np.abs(Ql[b] - o.clik(link, t[b], q0n[b], max_iterations=4)[0]).max() for b in range(16))
+    print(f"{tree}: 4 iterations, lanes vs thread {d:.1e}, lanes vs oracle {dor:.1e}")
+    assert d <= 1e-4 and dor <= 1e-4
+    assert np.array_equal(okl, okt) and np.array_equal(itl, itt)
+
+    # random starts, joint 0 masked: statistics and correctness
+    B = 512
+    qs = S.samples(rob, B, 1720)[0]
+    q0n = S.samples(rob, B, 1721)[0]
+    qs[:, 0] = q0n[:, 0]  # the masked joint already at its target value
+    t = _t(o.link_poses(qs)[:, link])
+    q0 = _t(q0n)
+    mask = torch.ones(rob.nv, dtype=torch.int32, device=DEV)
+    mask[0] = 0
+    full = {}
+    for variant in ("auto", "thread_per_env"):
+        sa.set_kernel_variant(m, variant)
+        full[variant] = [x.cpu().numpy() for x in m.compute_inverse_kinematics_batched(link, t, q0, mask)]
+    fl, ft = full["auto"], full["thread_per_env"]
+    print(f"{tree}: success lanes {fl[1].mean():.3f} thread {ft[1].mean():.3f}; "
+          f"mean iterations {fl[3].mean():.1f} / {ft[3].mean():.1f}")
+    assert abs(fl[1].mean() - ft[1].mean()) <= 0.05
+    for Q, ok, err, _ in (fl, ft):
+        assert (np.linalg.norm(err[ok], axis=1) < 1e-4).all()
+        reached = o.link_poses(Q[ok].astype(np.float64))[:, link, :3]
+        assert np.abs(reached - t.cpu().numpy()[ok, :3]).max() < 2e-4
+        assert np.array_equal(Q[:, 0], q0n[:, 0].astype(np.float32))  # masked joint
 
 
 def test_heterogeneous_clik_one_launch():
