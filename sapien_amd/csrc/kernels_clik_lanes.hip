@@ -1,19 +1,24 @@
 // Lanes-per-joint CLIK: damped least squares on the SE(3) log, as
 // compute_inverse_kinematics (pinocchio_model.cpp:228-300), for problems whose
-// root-to-link joint chain has <= 16 joints. One problem per group of G = 8 or
-// 16 lanes inside a 16-lane row; lane k owns chain joint k.
-//   FK      each lane's local transform, prefix product by DPP row shifts; the
-//           link joint's pose broadcast to the group
+// root-to-link joint chain has <= 16 joints. One problem per group of G lanes
+// inside a 16-lane row, C consecutive chain joints per lane (G x C >= chain).
+//   FK      each lane composes its C local transforms; a DPP prefix product
+//           over the group gives the world transform before the lane's first
+//           joint; the link joint's pose (the inclusive product at its lane:
+//           joints past the chain end are identities) goes to the whole group
 //   error   log6 of oMj^-1 oMdes (every lane, same bits)
-//   J J^T   each lane's Jacobian column in the link frame through Jlog6; the 21
-//           products in FP64 (exact for FP32 factors) summed by a butterfly
-//           over the group, so every lane holds the same sum
+//   J J^T   each joint's column in the link frame through Jlog6; the 21
+//           products in FP64 (exact for FP32 factors), summed over the lane's
+//           joints and then by a butterfly over the group (every lane ends
+//           with the same bits)
 //   solve   the damped 6x6 LDL^T in FP64, redundantly in every lane
-// Wavefronts pull problems from a work queue (one index per group) and each
-// group leaves the iteration loop on its own problem's exit test: per-problem
-// early exit. Results follow the
-// thread-per-problem kernel (kernels_generic.hip) step for step; only the
-// association order of the FK products and of the J J^T sums differs.
+// Scheduling: one trip of the main loop is one CLIK iteration for every group
+// of the wavefront. A group whose problem ends (converged or out of
+// iterations) writes its result and takes the next problem at the following
+// trip, so one slow problem does not hold its neighbours idle. The queue is
+// fetched once per trip for all groups that need work (ballot, one atomic
+// from every lane), and every cross-lane operation sits in code all lanes
+// execute.
 #include <hip/hip_runtime.h>
 
 #include "lanes.h"
@@ -27,6 +32,15 @@ namespace sapamd {
 namespace {
 
 using namespace lanes;
+
+constexpr int kClikMaxChain = 16;
+
+// The chain joints' constants the loop reads, staged in LDS per segment.
+struct ChainRec {
+  float A[9], Bq[9], Cq[9];
+  float p[3], ap[3], axis[3];
+  int32_t kind, vuser, pad[2];
+};
 
 // 32-bit halves of a double moved by one DPP pattern (a full permutation of the
 // 16-lane row, so every lane has a source).
@@ -42,113 +56,203 @@ template <int CTRL> __device__ __forceinline__ double dpp_d(double x) {
 // partners compute a + b and b + a, which are equal, so all lanes end with the
 // same bits.
 template <int G> __device__ __forceinline__ double group_allsum(double x) {
-  x += dpp_d<0xB1>(x);   // quad_perm [1,0,3,2]
-  x += dpp_d<0x4E>(x);   // quad_perm [2,3,0,1]
-  x += dpp_d<0x141>(x);  // row_half_mirror: i <-> 7 - i
-  if constexpr (G == 16) x += dpp_d<0x140>(x);  // row_mirror: i <-> 15 - i
+  if constexpr (G >= 2) x += dpp_d<0xB1>(x);    // quad_perm [1,0,3,2]
+  if constexpr (G >= 4) x += dpp_d<0x4E>(x);    // quad_perm [2,3,0,1]
+  if constexpr (G >= 8) x += dpp_d<0x141>(x);   // row_half_mirror: i <-> 7 - i
+  if constexpr (G >= 16) x += dpp_d<0x140>(x);  // row_mirror: i <-> 15 - i
   return x;
 }
 
 __device__ __forceinline__ int bperm_i(int src4, int x) { return __builtin_amdgcn_ds_bpermute(src4, x); }
 
-// One problem of segment S on this group (all G lanes active).
-template <int G>
-__device__ void clik_group(const ClikSegment &S, int64_t prob, int g, int base4) {
-  const ClikParams &P = S.P;
-  const ModelTable<float> *M = S.tab;
-  const int nv = M->nv, nc = P.nchain;
-  const int64_t tgt = P.ntargets ? prob % P.ntargets : prob;
-  const int32_t start = P.ntargets ? (int32_t)(prob / P.ntargets) : 0;
-  float *qo = S.q_out + prob * nv;
-  const float *q0 = S.q0 ? S.q0 + prob * nv : nullptr;
-  if (P.solved) {
-    // a lower start already converged for this target: skip (lane 0's reading
-    // decides for the group; every lane loads, so the permute reads no value
-    // produced under a divergent branch)
-    int lowest = __hip_atomic_load(P.solved + tgt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    lowest = bperm_i(base4, lowest);
-    if (lowest < start) {
-      for (int u = g; u < nv; u += G) qo[u] = q0 ? q0[u] : 0.f;
-      if (g == 0) {
-        S.success[prob] = 0;
-        for (int r = 0; r < 6; ++r) S.err[prob * 6 + r] = 1e30f;
-        if (S.iters) S.iters[prob] = -1;
+// Local transform of a chain joint (selects, not branches: no divergence
+// inside a group before the scans).
+__device__ __forceinline__ Xf local_xf(const ChainRec &J, float q, bool act) {
+  float s, c;
+  sincosf(q, &s, &c);
+  const bool pri = J.kind == KIND_PRISMATIC;
+  const float sr = pri ? 0.f : s, omc = pri ? 0.f : 1.f - c, qp = pri ? q : 0.f;
+  Xf X;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) X.R[k] = J.A[k] + sr * J.Bq[k] + omc * J.Cq[k];
+  X.p = load3(J.p) + qp * load3(J.ap);
+  return act ? X : identity_xf();
+}
+
+// Per-group problem state (the same in every lane of the group, except the
+// per-joint entries).
+template <int C> struct ClikState {
+  int64_t prob;  // problem index within its segment
+  int seg, nc, it;
+  float Rd[9];
+  V3<float> pd;
+  float q[C], qbest[C], msk[C];
+  float minErr, bestErr[6];
+};
+
+enum : int { kIdle = 0, kRunning = 1, kRetired = 2 };
+
+template <int G, int C> __global__ __launch_bounds__(256) void k_clik_lanes(HeteroClikArgs A) {
+  __shared__ __align__(16) int s_raw[sizeof(ClikSegment) * kMaxSegments / 4];
+  __shared__ ChainRec s_chain[kMaxSegments][kClikMaxChain];
+  {
+    const int *src = reinterpret_cast<const int *>(A.seg);
+    for (int i = threadIdx.x; i < (int)(sizeof(s_raw) / 4); i += blockDim.x) s_raw[i] = src[i];
+    for (int i = threadIdx.x; i < A.nseg * kClikMaxChain; i += blockDim.x) {
+      const ClikSegment &S = A.seg[i / kClikMaxChain];
+      const int k = i % kClikMaxChain;
+      if (k >= S.P.nchain) continue;
+      const JointRec<float> &J = joints_of(S.tab)[S.P.chain[k]];
+      ChainRec &R = s_chain[i / kClikMaxChain][k];
+      for (int t = 0; t < 9; ++t) {
+        R.A[t] = J.A[t];
+        R.Bq[t] = J.Bq[t];
+        R.Cq[t] = J.Cq[t];
       }
-      return;
+      for (int t = 0; t < 3; ++t) {
+        R.p[t] = J.p[t];
+        R.ap[t] = J.ap[t];
+        R.axis[t] = J.axis[t];
+      }
+      R.kind = J.kind;
+      R.vuser = J.vuser;
     }
   }
-  const bool act = g < nc;
-  const JointRec<float> &Jg = joints_of(M)[P.chain[act ? g : 0]];
-  const int kind = Jg.kind, vu = Jg.vuser;
-  float JA[9], JB[9], JC[9];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    JA[k] = Jg.A[k];
-    JB[k] = Jg.Bq[k];
-    JC[k] = Jg.Cq[k];
-  }
-  const V3<float> Jp = load3(Jg.p), Jap = load3(Jg.ap), Jax = load3(Jg.axis);
+  __syncthreads();
+  const ClikSegment *s_seg = reinterpret_cast<const ClikSegment *>(s_raw);
 
-  // desired pose of the link's joint frame: oMdes = target * link2joint^-1
-  const LinkRec<float> &Lk = links_of(M)[P.link];
-  float Rt[9], Rd[9];
-  V3<float> pt;
-  pose_to_mat(S.target + tgt * 7, Rt, pt);
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j)
-      Rd[3 * i + j] = Rt[3 * i] * Lk.R[3 * j] + Rt[3 * i + 1] * Lk.R[3 * j + 1] + Rt[3 * i + 2] * Lk.R[3 * j + 2];
-  const V3<float> pd = pt - mv(Rd, load3(Lk.p));
+  const int lane = threadIdx.x & 63;
+  const int g = lane & (G - 1);
+  const int glane0 = lane & ~(G - 1);  // first lane of the group
+  const int base4 = glane0 * 4;
+  const uint64_t below = glane0 ? ((uint64_t)1 << glane0) - 1 : 0;
 
-  float qc = act ? (q0 ? q0[vu] : 0.f) : 0.f, qbest = qc;
-  const float msk = act ? (P.mask ? float(P.mask[prob * P.mask_ld + vu]) : 1.f) : 0.f;
-  float minErr = 1e10f, bestErr[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  bool ok = false;
-  int it = 0;
-  const int last4 = base4 + 4 * (nc > 0 ? nc - 1 : 0);
-  for (;; ++it) {
-    // FK of the chain: local transform, then the prefix product over the group
-    // (selects, not branches: no divergence inside the group before the scan)
-    Xf X;
-    {
-      float s, c;
-      sincosf(qc, &s, &c);
-      const bool pri = kind == KIND_PRISMATIC;
-      const float sr = pri ? 0.f : s, omc = pri ? 0.f : 1.f - c, qp = pri ? qc : 0.f;
+  ClikState<C> st;
+  st.prob = 0;
+  st.seg = 0;
+  st.nc = 0;
+  st.it = 0;
 #pragma unroll
-      for (int k = 0; k < 9; ++k) X.R[k] = JA[k] + sr * JB[k] + omc * JC[k];
-      X.p = Jp + qp * Jap;
-      if (!act) X = identity_xf();
+  for (int k = 0; k < 9; ++k) st.Rd[k] = (k % 4 == 0) ? 1.f : 0.f;
+  st.pd = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int c = 0; c < C; ++c) st.q[c] = st.qbest[c] = st.msk[c] = 0.f;
+  st.minErr = 1e10f;
+#pragma unroll
+  for (int r = 0; r < 6; ++r) st.bestErr[r] = 0.f;
+  int state = kIdle;
+
+  for (;;) {
+    // ---- refill: the groups without a problem take the next indices
+    const uint64_t want = __ballot(state == kIdle && g == 0);
+    if (want) {  // uniform
+      const int nwant = __popcll(want);
+      const int old = atomicAdd(A.counter, lane == 0 ? nwant : 0);
+      const int first = __builtin_amdgcn_readlane(old, 0);
+      const int64_t idx = (int64_t)first + __popcll(want & below);
+      // multi-start skip test: every lane loads (from a valid address in any
+      // case) and the group takes its first lane's reading
+      const int64_t ic = idx < A.total ? idx : 0;
+      int s = 0;
+      while (s + 1 < A.nseg && ic >= s_seg[s + 1].first) ++s;
+      const ClikSegment &S = s_seg[s];
+      const ClikParams &P = S.P;
+      const int64_t prob = ic - S.first;
+      const int64_t tgt = P.ntargets ? prob % P.ntargets : prob;
+      const int32_t start = P.ntargets ? (int32_t)(prob / P.ntargets) : 0;
+      int lowest =
+          __hip_atomic_load(P.solved ? P.solved + tgt : A.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      lowest = bperm_i(base4, lowest);
+      if (state == kIdle) {  // per group; no cross-lane operations inside
+        if (idx >= A.total) {
+          state = kRetired;
+        } else if (P.solved && lowest < start) {
+          // a lower start already converged for this target: skip
+          const int nv = S.tab->nv;
+          float *qo = S.q_out + prob * nv;
+          const float *q0 = S.q0 ? S.q0 + prob * nv : nullptr;
+          for (int u = g; u < nv; u += G) qo[u] = q0 ? q0[u] : 0.f;
+          if (g == 0) {
+            S.success[prob] = 0;
+            for (int r = 0; r < 6; ++r) S.err[prob * 6 + r] = 1e30f;
+            if (S.iters) S.iters[prob] = -1;
+          }
+        } else {
+          // set up the problem: oMdes = target * link2joint^-1, q0, mask
+          const ModelTable<float> *M = S.tab;
+          const int nv = M->nv;
+          st.prob = prob;
+          st.seg = s;
+          st.nc = P.nchain;
+          st.it = 0;
+          const LinkRec<float> &Lk = links_of(M)[P.link];
+          float Rt[9];
+          V3<float> pt;
+          pose_to_mat(S.target + tgt * 7, Rt, pt);
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+              st.Rd[3 * i + j] =
+                  Rt[3 * i] * Lk.R[3 * j] + Rt[3 * i + 1] * Lk.R[3 * j + 1] + Rt[3 * i + 2] * Lk.R[3 * j + 2];
+          st.pd = pt - mv(st.Rd, load3(Lk.p));
+          const float *q0 = S.q0 ? S.q0 + prob * nv : nullptr;
+#pragma unroll
+          for (int c = 0; c < C; ++c) {
+            const int j = g * C + c;
+            const bool act = j < st.nc;
+            const int vu = s_chain[s][act ? j : 0].vuser;
+            st.q[c] = act && q0 ? q0[vu] : 0.f;
+            st.qbest[c] = st.q[c];
+            st.msk[c] = act ? (P.mask ? float(P.mask[prob * P.mask_ld + vu]) : 1.f) : 0.f;
+          }
+          st.minErr = 1e10f;
+#pragma unroll
+          for (int r = 0; r < 6; ++r) st.bestErr[r] = 0.f;
+          state = kRunning;
+        }
+      }
     }
-    scan_xf<G>(g, X);
-    Xf E = bperm_xf(last4, X);  // the link joint's world pose
-    if (nc == 0) E = identity_xf();
+    if (__ballot(state != kRetired) == 0) break;  // uniform: queue dry and every group done
+
+    // ---- one iteration for every group (only running groups keep the result)
+    const ClikSegment &S = s_seg[st.seg];
+    const ClikParams &P = S.P;
+    const ChainRec *chain = s_chain[st.seg];
+    Xf W[C];
+    Xf part = identity_xf();
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int j = g * C + c;
+      W[c] = local_xf(chain[j < st.nc ? j : 0], st.q[c], j < st.nc);
+      part = c == 0 ? W[0] : compose(part, W[c]);
+    }
+    Xf incl = part;
+    scan_xf<G>(g, incl);
+    const Xf excl = sel_xf(g >= 1, below_xf<1>(incl), identity_xf());
+    // world transform of each of the lane's joints
+#pragma unroll
+    for (int c = 0; c < C; ++c) W[c] = compose(c == 0 ? excl : W[c - 1], W[c]);
+    const int eel = st.nc > 0 ? (st.nc - 1) / C : 0;
+    Xf E = bperm_xf(base4 + 4 * eel, incl);  // the link joint's world pose
+    if (st.nc == 0) E = identity_xf();
 
     // iMd = oMj^-1 oMdes and its log
     float Ri[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) Ri[3 * i + j] = E.R[i] * Rd[j] + E.R[3 + i] * Rd[3 + j] + E.R[6 + i] * Rd[6 + j];
-    const V3<float> pi = mtv(E.R, pd - E.p);
+      for (int j = 0; j < 3; ++j)
+        Ri[3 * i + j] = E.R[i] * st.Rd[j] + E.R[3 + i] * st.Rd[3 + j] + E.R[6 + i] * st.Rd[6 + j];
+    const V3<float> pi = mtv(E.R, st.pd - E.p);
     const S6<float> e = log6_dev(Ri, pi);
     float ev[6];
     vec_of(e, ev);
     const float en = sqrtf(dot6(e, e));
-    if (en < minErr) {
-      minErr = en;
-      qbest = qc;
-#pragma unroll
-      for (int r = 0; r < 6; ++r) bestErr[r] = ev[r];
-    }
-    if (en < float(P.eps)) {
-      ok = true;
-      break;
-    }
-    if (it >= P.max_iter) break;
 
-    // this lane's column of Jlog6(iMd^-1) * J_local, masked
+    // the step, computed by every lane: this lane's columns of
+    // Jlog6(iMd^-1) * J_local (masked), J J^T + damp I, its solve in FP64
+    // (cond ~1e6+ near singular poses)
     float Rinv[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -156,25 +260,40 @@ __device__ void clik_group(const ClikSegment &S, int64_t prob, int g, int base4)
       for (int j = 0; j < 3; ++j) Rinv[3 * i + j] = Ri[3 * j + i];
     float Jl[36];
     jlog6_dev(Rinv, -1.f * mtv(Ri, pi), Jl);
-    S6<float> cw = zero6<float>();
-    if (act) cw = kind == KIND_PRISMATIC ? S6<float>{mv(X.R, Jax), {0.f, 0.f, 0.f}}
-                                         : S6<float>{cross(X.p, mv(X.R, Jax)), mv(X.R, Jax)};
-    float cv[6];
-    vec_of(to_local(E.R, E.p, cw), cv);
-    float Jk[6];
+    float Jk[C][6];
+    double part21[21];
 #pragma unroll
-    for (int r = 0; r < 6; ++r) {
-      float acc = 0.f;
+    for (int r = 0; r < 21; ++r) part21[r] = 0.0;
 #pragma unroll
-      for (int m = 0; m < 6; ++m) acc += Jl[r * 6 + m] * cv[m];
-      Jk[r] = -acc * msk;
+    for (int c = 0; c < C; ++c) {
+      const int j = g * C + c;
+      const ChainRec &J = chain[j < st.nc ? j : 0];
+      const V3<float> aw = mv(W[c].R, load3(J.axis));
+      S6<float> cw = J.kind == KIND_PRISMATIC ? S6<float>{aw, {0.f, 0.f, 0.f}} : S6<float>{cross(W[c].p, aw), aw};
+      if (j >= st.nc) cw = zero6<float>();
+      float cv[6];
+      vec_of(to_local(E.R, E.p, cw), cv);
+#pragma unroll
+      for (int r = 0; r < 6; ++r) {
+        float acc = 0.f;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) acc += Jl[r * 6 + m] * cv[m];
+        Jk[c][r] = -acc * st.msk[c];
+      }
+      int t = 0;
+#pragma unroll
+      for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int m = 0; m <= r; ++m) part21[t++] += double(Jk[c][r]) * double(Jk[c][m]);
     }
-    // J J^T + damp I and its solve in FP64 (cond ~1e6+ near singular poses)
     double JJt[36], y[6];
+    {
+      int t = 0;
 #pragma unroll
-    for (int r = 0; r < 6; ++r)
+      for (int r = 0; r < 6; ++r)
 #pragma unroll
-      for (int m = 0; m <= r; ++m) JJt[r * 6 + m] = group_allsum<G>(double(Jk[r]) * double(Jk[m]));
+        for (int m = 0; m <= r; ++m) JJt[r * 6 + m] = group_allsum<G>(part21[t++]);
+    }
 #pragma unroll
     for (int r = 0; r < 6; ++r) {
       JJt[r * 6 + r] += P.damp;
@@ -183,80 +302,75 @@ __device__ void clik_group(const ClikSegment &S, int64_t prob, int g, int base4)
       y[r] = double(ev[r]);
     }
     ldlt6_solve(JJt, y);
-    float v = 0.f;
-#pragma unroll
-    for (int r = 0; r < 6; ++r) v -= Jk[r] * float(y[r]);
-    qc += v * float(P.dt);
-  }
 
-  // result: chain dofs from the best iterate (continuous wrapped), the rest from q0
-  for (int u = g; u < nv; u += G) {
-    bool chain_dof = false;
-    for (int k = 0; k < nc; ++k) chain_dof |= joints_of(M)[P.chain[k]].vuser == u;
-    if (!chain_dof) qo[u] = q0 ? q0[u] : 0.f;
-  }
-  if (act) {
-    float v = qbest;
-    if (kind == KIND_CONTINUOUS) {
-      float s, c;
-      sincosf(v, &s, &c);
-      v = atan2f(s, c);
-    }
-    qo[vu] = v;
-  }
-  if (g == 0) {
-    S.success[prob] = ok ? 1 : 0;
+    if (state == kRunning) {  // per group; no cross-lane operations inside
+      if (en < st.minErr) {
+        st.minErr = en;
 #pragma unroll
-    for (int r = 0; r < 6; ++r) S.err[prob * 6 + r] = bestErr[r];
-    if (S.iters) S.iters[prob] = it;
-    if (ok && P.solved) __hip_atomic_fetch_min(P.solved + tgt, start, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-
-// Persistent: each wavefront takes the next 64 / G problem indices from the
-// queue (one per group) until it runs dry; the segment table sits in LDS
-// (indexed per group). The fetch is wavefront-wide on purpose: the groups of a
-// wavefront share one instruction stream, so a group cannot start its next
-// problem before the others finish theirs anyway, and every lane executes the
-// atomic, so the fetched index never travels from a lane that took a
-// different branch.
-template <int G> __global__ __launch_bounds__(256) void k_clik_lanes(HeteroClikArgs A) {
-  __shared__ __align__(16) int s_raw[sizeof(ClikSegment) * kMaxSegments / 4];
-  {
-    const int *src = reinterpret_cast<const int *>(A.seg);
-    for (int i = threadIdx.x; i < (int)(sizeof(s_raw) / 4); i += blockDim.x) s_raw[i] = src[i];
-  }
-  __syncthreads();
-  const ClikSegment *s_seg = reinterpret_cast<const ClikSegment *>(s_raw);
-  const int lane = threadIdx.x & 63;
-  const int g = lane & (G - 1);
-  const int base4 = (lane & ~(G - 1)) * 4;
-  constexpr int kPerWave = 64 / G;
-  for (;;) {
-    // all 64 lanes active here: lane 0 adds kPerWave, the others add 0, so lane
-    // 0's old value is this wavefront's first index
-    const int old = atomicAdd(A.counter, lane == 0 ? kPerWave : 0);
-    const int first = __builtin_amdgcn_readlane(old, 0);
-    if ((int64_t)first >= A.total) break;  // uniform over the wavefront
-    const int idx = first + lane / G;
-    if ((int64_t)idx < A.total) {
-      int s = 0;
-      while (s + 1 < A.nseg && idx >= s_seg[s + 1].first) ++s;
-      clik_group<G>(s_seg[s], idx - s_seg[s].first, g, base4);
+        for (int c = 0; c < C; ++c) st.qbest[c] = st.q[c];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) st.bestErr[r] = ev[r];
+      }
+      const bool ok = en < float(P.eps);
+      if (ok || st.it >= P.max_iter) {
+        // result: chain dofs from the best iterate (continuous wrapped), the rest from q0
+        const int nv = S.tab->nv;
+        float *qo = S.q_out + st.prob * nv;
+        const float *q0 = S.q0 ? S.q0 + st.prob * nv : nullptr;
+        for (int u = g; u < nv; u += G) {
+          bool chain_dof = false;
+          for (int k = 0; k < st.nc; ++k) chain_dof |= chain[k].vuser == u;
+          if (!chain_dof) qo[u] = q0 ? q0[u] : 0.f;
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const int j = g * C + c;
+          if (j < st.nc) {
+            float v = st.qbest[c];
+            if (chain[j].kind == KIND_CONTINUOUS) {
+              float s, co;
+              sincosf(v, &s, &co);
+              v = atan2f(s, co);
+            }
+            qo[chain[j].vuser] = v;
+          }
+        }
+        if (g == 0) {
+          S.success[st.prob] = ok ? 1 : 0;
+#pragma unroll
+          for (int r = 0; r < 6; ++r) S.err[st.prob * 6 + r] = st.bestErr[r];
+          if (S.iters) S.iters[st.prob] = st.it;
+          if (ok && P.solved) {
+            const int64_t tgt = P.ntargets ? st.prob % P.ntargets : st.prob;
+            const int32_t start = P.ntargets ? (int32_t)(st.prob / P.ntargets) : 0;
+            __hip_atomic_fetch_min(P.solved + tgt, start, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+        }
+        state = kIdle;
+      } else {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          float v = 0.f;
+#pragma unroll
+          for (int r = 0; r < 6; ++r) v -= Jk[c][r] * float(y[r]);
+          st.q[c] += v * float(P.dt);
+        }
+        ++st.it;
+      }
     }
   }
 }
 
 }  // namespace
 
-int clik_lanes_group(int max_chain) { return max_chain <= 8 ? 8 : max_chain <= 16 ? 16 : 0; }
+int clik_lanes_group(int max_chain) { return max_chain <= kClikMaxChain ? 4 : 0; }
 
 hipError_t launch_clik_lanes(const HeteroClikArgs &A, int max_chain, int blocks, hipStream_t s) {
   if (A.total <= 0) return hipSuccess;
-  const int G = clik_lanes_group(max_chain);
-  if (!G || A.total > INT32_MAX) return hipErrorNotSupported;
-  if (G == 8) hipLaunchKernelGGL(k_clik_lanes<8>, dim3((unsigned)blocks), dim3(256), 0, s, A);
-  else hipLaunchKernelGGL(k_clik_lanes<16>, dim3((unsigned)blocks), dim3(256), 0, s, A);
+  if (!clik_lanes_group(max_chain) || A.total > INT32_MAX) return hipErrorNotSupported;
+  // one shape for every chain length, so a problem gives the same bits in a
+  // per-model launch and in a mixed-tree launch
+  hipLaunchKernelGGL((k_clik_lanes<4, 4>), dim3((unsigned)blocks), dim3(256), 0, s, A);
   return hipGetLastError();
 }
 

@@ -118,9 +118,10 @@ struct HeteroClikArgs {
 };
 hipError_t launch_clik_hetero(const HeteroClikArgs &A, int max_chain, int waves, hipStream_t s);
 // Lanes-per-joint CLIK (FP32, chains of <= 16 joints): one problem per group of
-// 8 or 16 lanes, persistent blocks of 256 threads pulling problems from
-// A.counter (zeroed before the launch). Serves the per-model batched and the
-// mixed-tree launches; clik_lanes_group returns 0 above 16 chain joints.
+// 4 lanes x 4 chain joints, persistent blocks of 256 threads pulling problems
+// from A.counter (zeroed before the launch). Serves the per-model batched and
+// the mixed-tree launches; clik_lanes_group returns the group size, 0 above
+// 16 chain joints.
 int clik_lanes_group(int max_chain);
 hipError_t launch_clik_lanes(const HeteroClikArgs &A, int max_chain, int blocks, hipStream_t s);
 
