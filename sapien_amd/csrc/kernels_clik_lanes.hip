@@ -69,7 +69,7 @@ __device__ __forceinline__ int bperm_i(int src4, int x) { return __builtin_amdgc
 // inside a group before the scans).
 __device__ __forceinline__ Xf local_xf(const ChainRec &J, float q, bool act) {
   float s, c;
-  sincosf(q, &s, &c);
+  joint_sincos(q, &s, &c);
   const bool pri = J.kind == KIND_PRISMATIC;
   const float sr = pri ? 0.f : s, omc = pri ? 0.f : 1.f - c, qp = pri ? q : 0.f;
   Xf X;
@@ -245,7 +245,9 @@ template <int G, int C> __global__ __launch_bounds__(256) void k_clik_lanes(Hete
       for (int j = 0; j < 3; ++j)
         Ri[3 * i + j] = E.R[i] * st.Rd[j] + E.R[3 + i] * st.Rd[3 + j] + E.R[6 + i] * st.Rd[6 + j];
     const V3<float> pi = mtv(E.R, st.pd - E.p);
-    const S6<float> e = log6_dev(Ri, pi);
+    // (with Jlog6(iMd^-1) for the step from the same rotation log)
+    float Jl[36];
+    const S6<float> e = log6_jlog6_inv(Ri, pi, Jl);
     float ev[6];
     vec_of(e, ev);
     const float en = sqrtf(dot6(e, e));
@@ -253,13 +255,6 @@ template <int G, int C> __global__ __launch_bounds__(256) void k_clik_lanes(Hete
     // the step, computed by every lane: this lane's columns of
     // Jlog6(iMd^-1) * J_local (masked), J J^T + damp I, its solve in FP64
     // (cond ~1e6+ near singular poses)
-    float Rinv[9];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) Rinv[3 * i + j] = Ri[3 * j + i];
-    float Jl[36];
-    jlog6_dev(Rinv, -1.f * mtv(Ri, pi), Jl);
     float Jk[C][6];
     double part21[21];
 #pragma unroll

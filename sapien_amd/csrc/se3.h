@@ -80,11 +80,8 @@ __device__ __forceinline__ S6<T> log6_dev(const T *R, V3<T> p) {
 
 // Jlog6 (6x6 row-major): [[A, C A], [0, A]], A = alpha I + beta w w^T + [w]x / 2,
 // C = dbeta (w.p) w w^T - (t^2 dbeta + 2 beta) p w^T + beta w p^T + beta (w.p) I + [p]x / 2.
-template <class T> __device__ __forceinline__ void jlog6_dev(const T *R, V3<T> p, T *Jl) {
-  T t;
-  V3<T> w = log3_dev(R, t);
-  T al, be, db;
-  log_coeffs_dev(t, al, be, db);
+template <class T>
+__device__ __forceinline__ void jlog6_core(V3<T> w, T t, T al, T be, T db, V3<T> p, T *Jl) {
   T wv[3] = {w.x, w.y, w.z}, pv[3] = {p.x, p.y, p.z};
   T A[9], C[9];
   T wp = dot(w, p);
@@ -109,21 +106,45 @@ template <class T> __device__ __forceinline__ void jlog6_dev(const T *R, V3<T> p
     }
 }
 
-// y = (J J^T + damp I)^-1 e by LDL^T of the 6x6 SPD matrix.
+template <class T> __device__ __forceinline__ void jlog6_dev(const T *R, V3<T> p, T *Jl) {
+  T t;
+  V3<T> w = log3_dev(R, t);
+  T al, be, db;
+  log_coeffs_dev(t, al, be, db);
+  jlog6_core(w, t, al, be, db, p, Jl);
+}
+
+// log6 of (R, p) and Jlog6 of its inverse (R^T, -R^T p) from one rotation log:
+// log3(R^T) = -log3(R) (the axial vector changes sign exactly, the angle and
+// the coefficients are the same), so this equals log6_dev(R, p) and
+// jlog6_dev(R^T, -R^T p) bit for bit.
+template <class T> __device__ __forceinline__ S6<T> log6_jlog6_inv(const T *R, V3<T> p, T *Jl) {
+  T t;
+  V3<T> w = log3_dev(R, t);
+  T al, be, db;
+  log_coeffs_dev(t, al, be, db);
+  jlog6_core(T(-1) * w, t, al, be, db, T(-1) * mtv(R, p), Jl);
+  return {al * p - T(0.5) * cross(w, p) + (be * dot(w, p)) * w, w};
+}
+
+// y = (J J^T + damp I)^-1 e by LDL^T of the 6x6 SPD matrix, one division per
+// pivot.
 template <class T> __device__ __forceinline__ void ldlt6_solve(T *A, T *y) {
+  T inv[6];
   for (int j = 0; j < 6; ++j) {
     T d = A[j * 6 + j];
     for (int k = 0; k < j; ++k) d -= A[j * 6 + k] * A[j * 6 + k] * A[k * 6 + k];
     A[j * 6 + j] = d;
+    inv[j] = T(1) / d;
     for (int i = j + 1; i < 6; ++i) {
       T s = A[i * 6 + j];
       for (int k = 0; k < j; ++k) s -= A[i * 6 + k] * A[j * 6 + k] * A[k * 6 + k];
-      A[i * 6 + j] = s / d;
+      A[i * 6 + j] = s * inv[j];
     }
   }
   for (int i = 0; i < 6; ++i)
     for (int k = 0; k < i; ++k) y[i] -= A[i * 6 + k] * y[k];
-  for (int i = 0; i < 6; ++i) y[i] /= A[i * 6 + i];
+  for (int i = 0; i < 6; ++i) y[i] *= inv[i];
   for (int i = 5; i >= 0; --i)
     for (int k = i + 1; k < 6; ++k) y[i] -= A[k * 6 + i] * y[k];
 }
