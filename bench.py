@@ -12,8 +12,12 @@ Beside the headline number it reports:
   roofline      algorithmic bytes per launch / mean launch duration (HIP events
                 on the launch stream over the timed graph replays), against
                 8 TB/s HBM3E; traffic from rocprofv3 FETCH_SIZE + WRITE_SIZE;
+                the FP32 VALU work (rocprofv3 SQ_INSTS_VALU_* counters) against
+                the 157.3 TFLOP/s vector peak; the bandwidth an elementwise
+                copy reaches on the box;
   cpu_baseline  the FP64 C++ oracle (same algorithms) timed on this host's cores
-                on a bounded sample;
+                on a bounded sample, on one core, and through its per-call
+                Python API (one configuration per call);
   plain_gpu     the drop-in per-call path (one configuration per call, FP64 on
                 the GPU, host copies) on a bounded sample.
 """
@@ -51,6 +55,11 @@ def parse():
 
 
 COPY_BYTES = 256 << 20  # calibration copy: 256 MiB read + 256 MiB written
+# FP32 VALU work of the fused kernel, one rocprofv3 --pmc pass (wavefront
+# instruction counts, and the FLOP count the SQ derives per instruction)
+VALU_COUNTERS = ("SQ_INSTS_VALU_FLOPS_FP32", "SQ_INSTS_VALU_FLOPS_FP32_TRANS", "SQ_INSTS_VALU_FMA_F32",
+                 "SQ_INSTS_VALU_ADD_F32", "SQ_INSTS_VALU_MUL_F32", "SQ_INSTS_VALU_TRANS_F32", "SQ_INSTS_VALU")
+FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32, vendor figure
 
 
 def probe_traffic(shape=None):
@@ -107,9 +116,10 @@ def measure_traffic(shape=None, timeout=120):
     if shutil.which("rocprofv3") is None:
         return None
     res = {}
-    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+    for counters in (("FETCH_SIZE",), ("WRITE_SIZE",), VALU_COUNTERS):
+        counter = counters[0]
         d = tempfile.mkdtemp(prefix="sapamd_pmc_")
-        cmd = ["rocprofv3", "--pmc", counter, "-d", d, "-o", "pmc", "--output-format", "csv", "--",
+        cmd = ["rocprofv3", "--pmc", *counters, "-d", d, "-o", "pmc", "--output-format", "csv", "--",
                sys.executable, os.path.abspath(__file__), "--probe-traffic"]
         if shape is not None:
             cmd += ["--probe-shape", str(shape[0]), str(shape[1])]
@@ -120,18 +130,27 @@ def measure_traffic(shape=None, timeout=120):
             os.killpg(p.pid, signal.SIGKILL)
             p.wait()
             return None
-        kern, copy = [], []
+        kern, copy, valu = [], [], {c: [] for c in VALU_COUNTERS}
         for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
             with open(f) as fh:
                 for row in csv.DictReader(fh):
-                    if row.get("Counter_Name") != counter:
+                    cname = row.get("Counter_Name")
+                    if cname not in counters:
                         continue
                     name, v = row.get("Kernel_Name", ""), float(row.get("Counter_Value", "nan"))
-                    if "k_chain_fused" in name or "k_kinematics" in name or "k_tree_fused" in name:
+                    fused = "k_chain_fused" in name or "k_kinematics" in name or "k_tree_fused" in name
+                    if counters is VALU_COUNTERS:
+                        if fused:
+                            valu[cname].append(v)
+                    elif fused:
                         kern.append(v)
                     elif "neg" in name:
                         copy.append(v)
         shutil.rmtree(d, ignore_errors=True)
+        if counters is VALU_COUNTERS:
+            if all(valu.values()):
+                res["VALU"] = {c: sorted(v)[len(v) // 2] for c, v in valu.items()}
+            continue
         if not kern or not copy:
             return None
         kern = sorted(kern)[len(kern) // 2]
@@ -368,6 +387,21 @@ def main():
     achieved = BYTES_PER_EVAL * B / (kern_ms * 1e-3) / 1e9
     traffic = None if args.no_traffic else measure_traffic(tuned[B][:2] if tuned else None)
 
+    # achievable HBM bandwidth on this box: an elementwise pass over 256 MiB
+    # (read + write), the calibration kernel of the traffic counters
+    csrc = torch.ones(COPY_BYTES // 4, device=dev)
+    cdst = torch.empty_like(csrc)
+    for _ in range(3):
+        torch.neg(csrc, out=cdst)
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    c0.record(stream)
+    for _ in range(10):
+        torch.neg(csrc, out=cdst)
+    c1.record(stream)
+    torch.cuda.synchronize()
+    copy_gbs = 2 * COPY_BYTES * 10 / (c0.elapsed_time(c1) * 1e-3) / 1e9
+    del csrc, cdst
+
     launch_modes = {
         "graph": {"ms_per_step": elapsed_graph / args.steps * 1e3, "events_ms_per_launch": graph_ms,
                   "replay_matches_eager": graph_ok},
@@ -394,9 +428,27 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (traffic["FETCH_SIZE"]["bytes"] + traffic["WRITE_SIZE"]["bytes"]) if traffic else None,
-                     "kernel_ms": kern_ms, "bytes_per_launch": BYTES_PER_EVAL * B},
+                     "kernel_ms": kern_ms, "bytes_per_launch": BYTES_PER_EVAL * B,
+                     "copy_gbs": {"value": copy_gbs, "what": "torch.neg over 256 MiB (read + write), 10 launches, "
+                                                             "HIP events: the bandwidth an elementwise kernel reaches "
+                                                             "on this box"}},
         "launch_modes": launch_modes,
     }
+    if traffic and "VALU" in traffic:
+        v = traffic["VALU"]
+        # the SQ counts FLOPs per wavefront instruction (1 per add / mul /
+        # transcendental, 2 per FMA): x 64 lanes
+        flops = 64 * (v["SQ_INSTS_VALU_FLOPS_FP32"] + v["SQ_INSTS_VALU_FLOPS_FP32_TRANS"])
+        tflops = flops / (kern_ms * 1e-3) / 1e12
+        out["roofline"]["valu_fp32"] = {
+            "flops_per_launch": flops, "achieved_tflops": tflops, "peak_tflops": FP32_PEAK_TFLOPS,
+            "frac": tflops / FP32_PEAK_TFLOPS,
+            "from_instruction_counts": 64 * (2 * v["SQ_INSTS_VALU_FMA_F32"] + v["SQ_INSTS_VALU_ADD_F32"] +
+                                             v["SQ_INSTS_VALU_MUL_F32"] + v["SQ_INSTS_VALU_TRANS_F32"]),
+            "valu_instructions_per_launch": v["SQ_INSTS_VALU"],
+            "method": "rocprofv3 --pmc SQ_INSTS_VALU_FLOPS_FP32(+_TRANS) x 64 lanes, cross-checked by "
+                      "64 x (2 FMA + ADD + MUL + TRANS)_F32 instruction counts; SQ_INSTS_VALU; median of 16 "
+                      "launches, over the launch duration kernel_ms"}
     if traffic:
         out["roofline"]["traffic_detail"] = {
             "fetch_bytes": traffic["FETCH_SIZE"]["bytes"], "write_bytes": traffic["WRITE_SIZE"]["bytes"],
@@ -493,6 +545,27 @@ def main():
                                "sample": f"{reps} x {chunk} seeded chain7 samples, FP64 FK + 8 link poses + WORLD "
                                          f"Jacobian + RNEA per sample, C++ oracle statically split over {threads} "
                                          f"threads ({secs:.1f} s)"}
+        # one core: a quarter of the chunk until about a quarter of cpu_seconds
+        q1, qd1, qdd1 = (x[: chunk // 4] for x in (q, qd, qdd))
+        b1 = tuple(b[: chunk // 4] for b in bufs)
+        s1, r1 = 0.0, 0
+        while s1 < args.cpu_seconds / 4 and r1 < 100:
+            s1 += o.fk_jac_rnea_timed(q1, qd1, qdd1, 1, out=b1)[0]
+            r1 += 1
+        out["cpu_baseline"]["single_core"] = {
+            "value": r1 * (chunk // 4) / s1, "unit": "evals/s", "cores": 1,
+            "sample": f"{r1} x {chunk // 4} samples, the same C++ oracle on one thread ({s1:.1f} s)"}
+        # the plain CPU reference as a user calls it: 1024 samples, one
+        # configuration per call, FK + all link poses + WORLD Jacobian (FP64)
+        qa = S.samples(rob, 1024, 1000)[0]
+        t0 = time.perf_counter()
+        for i in range(1024):
+            o.link_poses(qa[i:i + 1])
+            o.jacobian_world(qa[i:i + 1])
+        out["cpu_baseline"]["per_call_api"] = {
+            "value": 1024 / (time.perf_counter() - t0), "unit": "evals/s", "cores": 1,
+            "sample": "1024 seeded chain7 samples, one call each for the link poses and the WORLD Jacobian through "
+                      "the oracle's Python API (FK + 8 link poses + full Jacobian, FP64)"}
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out))
 
