@@ -122,6 +122,7 @@ __device__ __forceinline__ void ldlt_solve(const Real (&row)[G], Real d, Real (*
 template <int G, class Real> struct DynLds {
   Real A[G][G + 1];
   Real y[G];
+  Real S[G][6];  // each joint's world column, for the rows of M
 };
 
 // One environment's kinematics and M / bias; the factorization is left in L.
@@ -136,7 +137,7 @@ template <int G, class Real> struct TreeFd {
   // q, qd in joint order (this lane's joint); loads row with the lower
   // triangle of row g of M (zeros above the diagonal), assembled in the
   // group's LDS tile, and returns the bias b_g = RNEA(q, qd, 0)_g.
-  __device__ __forceinline__ Real setup(const ModelTable<Real> *M, Real qj, Real qdj, Real (*A)[G + 1],
+  __device__ __forceinline__ Real setup(const ModelTable<Real> *M, Real qj, Real qdj, DynLds<G, Real> &L,
                                         Real (&row)[G]) {
     Real s, c;
     joint_sincos(qj, &s, &c);
@@ -166,14 +167,25 @@ template <int G, class Real> struct TreeFd {
 #pragma unroll
     for (int k = 0; k < 6; ++k) Yc.I[k] = y[4 + k];
     const S6<Real> Fg = apply(Yc, col);
-    for (int i = 0; i < nj; ++i) {  // uniform loop: every lane takes part in the permutes
-      const S6<Real> ci = bperm6(T.base4 + 4 * i, col);
-      const bool anc = i <= g && g < i + joints_of(M)[i].subtree;  // i ancestor-or-self of g
-      A[g][i] = valid && anc ? dot6(ci, Fg) : Real(0);
-    }
-    for (int i = nj; i < G; ++i) A[g][i] = Real(0);
+    // every lane publishes its column; lane g reads those of its ancestors
+    // (one address across the group at a time: broadcast reads, no chain of
+    // permutes)
+    L.S[g][0] = col.l.x;
+    L.S[g][1] = col.l.y;
+    L.S[g][2] = col.l.z;
+    L.S[g][3] = col.a.x;
+    L.S[g][4] = col.a.y;
+    L.S[g][5] = col.a.z;
     wave_sync();
-    static_for<0, G>([&](auto ic) { row[decltype(ic)::value] = A[g][decltype(ic)::value]; });
+#pragma unroll 4
+    for (int i = 0; i < nj; ++i) {
+      const S6<Real> ci = {{L.S[i][0], L.S[i][1], L.S[i][2]}, {L.S[i][3], L.S[i][4], L.S[i][5]}};
+      const bool anc = i <= g && g < i + joints_of(M)[i].subtree;  // i ancestor-or-self of g
+      L.A[g][i] = valid && anc ? dot6(ci, Fg) : Real(0);
+    }
+    for (int i = nj; i < G; ++i) L.A[g][i] = Real(0);
+    wave_sync();
+    static_for<0, G>([&](auto ic) { row[decltype(ic)::value] = L.A[g][decltype(ic)::value]; });
     wave_sync();
     return bias;
   }
@@ -208,7 +220,7 @@ template <int G, class Real>
 __device__ __forceinline__ Real lane_fd(TreeFd<G, Real> &F, const ModelTable<Real> *M, DynLds<G, Real> &L, Real qj,
                                         Real qdj, Real tj) {
   Real row[G];
-  const Real bias = F.setup(M, qj, qdj, L.A, row);
+  const Real bias = F.setup(M, qj, qdj, L, row);
   const Real d = ldlt_factor<G, Real>(row, L.y, F.g, F.nj);
   Real r[1] = {tj - bias};
   ldlt_solve<G, Real, 1>(row, d, L.A, F.T.base4, F.g, F.nj, r);
@@ -222,7 +234,7 @@ template <int G, class Real>
 __device__ __forceinline__ Real lane_step_vjp(TreeFd<G, Real> &F, const ModelTable<Real> *M, DynLds<G, Real> &L,
                                               float dt, Real qj, Real qdj, Real tj, Real &aqj, Real &aqdj) {
   Real row[G];
-  const Real bias = F.setup(M, qj, qdj, L.A, row);
+  const Real bias = F.setup(M, qj, qdj, L, row);
   const Real d = ldlt_factor<G, Real>(row, L.y, F.g, F.nj);
   const Real dtr = dt;
   const Real cj = aqdj + dtr * aqj;
