@@ -47,6 +47,15 @@ __device__ __forceinline__ void wave_sync() {
 // Compile-time loop: f(std::integral_constant<int, K>) for K in [B, E), so
 // register arrays indexed by K never fall back to the private segment (the
 // unroller's size limit does not apply).
+// The lane's index within its group, opaque to the optimizer: the
+// factorization compares it with every column index, and inside a time-step
+// loop those loop-invariant compares would otherwise be hoisted out of the
+// loop and spilled (a compare is one instruction, a spill round trip two).
+__device__ __forceinline__ int opaque_lane(int g) {
+  asm volatile("" : "+v"(g));
+  return g;
+}
+
 template <int B, int E, class F> __device__ __forceinline__ void static_for(F &&f) {
   if constexpr (B < E) {
     f(std::integral_constant<int, B>{});
@@ -57,63 +66,65 @@ template <int B, int E, class F> __device__ __forceinline__ void static_for(F &&
 // LDL^T of the symmetric positive definite n x n matrix whose row i (lower
 // triangle, zeros above the diagonal) lane i holds in row[]: right-looking
 // elimination with the rows in registers, step k broadcasting column k through
-// the group's G-entry LDS buffer. After the call row[j < i] = L_ij; returns D_i.
+// the group's G-entry LDS buffer. After the call row[j < i] = L_ij (entries
+// above the diagonal hold junk: the update runs over the whole row to need no
+// per-entry select, and nothing reads them); returns D_i. Lanes i >= n hold
+// zero rows. The steps run over all G columns (the ones past n change
+// nothing), so no step depends on a runtime bound.
 template <int G, class Real> __device__ __forceinline__ Real ldlt_factor(Real (&row)[G], Real *buf, int i, int n) {
   Real d = Real(1);
   static_for<0, G>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
-    if (k < n) {  // uniform
-      buf[i] = row[k];
-      wave_sync();
-      Real c[G];
-      static_for<k, G>([&](auto jc) { c[decltype(jc)::value] = buf[decltype(jc)::value]; });
-      wave_sync();
-      const bool act = i > k && i < n;
-      const Real l = act ? row[k] / c[k] : Real(0);
-      static_for<k + 1, G>([&](auto jc) {
-        constexpr int j = decltype(jc)::value;
-        row[j] = j <= i ? row[j] - l * c[j] : row[j];
-      });
-      if (act) row[k] = l;
-      if (i == k) d = row[k];
-    }
+    buf[i] = row[k];
+    wave_sync();
+    Real c[G];
+    static_for<k, G>([&](auto jc) { c[decltype(jc)::value] = buf[decltype(jc)::value]; });
+    wave_sync();
+    const bool act = i > k && i < n;
+    const Real l = act ? row[k] / c[k] : Real(0);
+    static_for<k + 1, G>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      row[j] -= l * c[j];
+    });
+    row[k] = act ? l : row[k];
+    d = i == k ? row[k] : d;
   });
   return d;
 }
 
-// R right-hand sides r[m] (lane i holding entry i) -> (L D L^T)^-1 r.
-// Forward substitution reads L from the lane's row; the rows are then
-// transposed through the group's LDS tile so backward substitution reads lane
-// i's column. Each step broadcasts the finished entry with ds_bpermute.
+// R right-hand sides r[m] (lane i holding entry i, zero for lanes i >= n) ->
+// (L D L^T)^-1 r. Forward substitution reads L from the lane's row; the rows
+// are then transposed through the group's LDS tile so backward substitution
+// reads lane i's column. Each step broadcasts the finished entry with
+// ds_bpermute; the factor entries are masked once (strictly lower row,
+// strictly below-diagonal column), so the steps need no selects.
 template <int G, class Real, int R>
 __device__ __forceinline__ void ldlt_solve(const Real (&row)[G], Real d, Real (*A)[G + 1], int base4, int i, int n,
                                            Real (&r)[R]) {
+  Real lo[G];
   static_for<0, G>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
-    if (k < n) {
+    lo[k] = k < i ? row[k] : Real(0);
+  });
+  static_for<0, G>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
 #pragma unroll
-      for (int m = 0; m < R; ++m) {
-        const Real yk = bperm(base4 + 4 * k, r[m]);
-        r[m] = i > k ? r[m] - row[k] * yk : r[m];
-      }
-    }
+    for (int m = 0; m < R; ++m) r[m] -= lo[k] * bperm(base4 + 4 * k, r[m]);
   });
 #pragma unroll
   for (int m = 0; m < R; ++m) r[m] = i < n ? r[m] / d : Real(0);
   static_for<0, G>([&](auto jc) { A[i][decltype(jc)::value] = row[decltype(jc)::value]; });
   wave_sync();
   Real col[G];
-  static_for<0, G>([&](auto kc) { col[decltype(kc)::value] = A[decltype(kc)::value][i]; });
+  static_for<0, G>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    col[k] = k > i ? A[k][i] : Real(0);
+  });
   wave_sync();
   static_for<0, G>([&](auto kc) {
     constexpr int k = G - 1 - decltype(kc)::value;
-    if (k < n) {
 #pragma unroll
-      for (int m = 0; m < R; ++m) {
-        const Real xk = bperm(base4 + 4 * k, r[m]);
-        r[m] = i < k ? r[m] - col[k] * xk : r[m];
-      }
-    }
+    for (int m = 0; m < R; ++m) r[m] -= col[k] * bperm(base4 + 4 * k, r[m]);
   });
 }
 
@@ -221,9 +232,10 @@ __device__ __forceinline__ Real lane_fd(TreeFd<G, Real> &F, const ModelTable<Rea
                                         Real qdj, Real tj) {
   Real row[G];
   const Real bias = F.setup(M, qj, qdj, L, row);
-  const Real d = ldlt_factor<G, Real>(row, L.y, F.g, F.nj);
+  const int i = opaque_lane(F.g);
+  const Real d = ldlt_factor<G, Real>(row, L.y, i, F.nj);
   Real r[1] = {tj - bias};
-  ldlt_solve<G, Real, 1>(row, d, L.A, F.T.base4, F.g, F.nj, r);
+  ldlt_solve<G, Real, 1>(row, d, L.A, F.T.base4, i, F.nj, r);
   return r[0];
 }
 
@@ -235,11 +247,12 @@ __device__ __forceinline__ Real lane_step_vjp(TreeFd<G, Real> &F, const ModelTab
                                               float dt, Real qj, Real qdj, Real tj, Real &aqj, Real &aqdj) {
   Real row[G];
   const Real bias = F.setup(M, qj, qdj, L, row);
-  const Real d = ldlt_factor<G, Real>(row, L.y, F.g, F.nj);
+  const int i = opaque_lane(F.g);
+  const Real d = ldlt_factor<G, Real>(row, L.y, i, F.nj);
   const Real dtr = dt;
   const Real cj = aqdj + dtr * aqj;
   Real r[2] = {tj - bias, dtr * cj};  // qdd and lambda = M^-1 (dt c) in one pass
-  ldlt_solve<G, Real, 2>(row, d, L.A, F.T.base4, F.g, F.nj, r);
+  ldlt_solve<G, Real, 2>(row, d, L.A, F.T.base4, i, F.nj, r);
   const Real qddj = r[0], lam = r[1];
 
   // RNEA vector-Jacobian product at (q, qd, qdd) for the upstream lambda
