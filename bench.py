@@ -205,8 +205,37 @@ def mixed_workloads(sa, S, np, torch, dev):
         qds.append(torch.from_numpy(qd).to(dev))
         qdds.append(torch.from_numpy(qdd).to(dev))
     sec = timed(lambda: sa.hetero_batched(models, qs, qds, qdds), 50)
-    res["hetero_fk_jac_rnea_crba"] = {"envs": nb, "evals_s": 8192 / sec, "ms": sec * 1e3,
-                                      "note": "one launch, outputs allocated per call"}
+    # the same launch through the C ABI over preallocated outputs (poses, J,
+    # tau, full symmetric M), HIP events on the launch stream
+    from sapien_amd.native import launcher_hetero
+
+    outs_h = [(torch.empty(b, rob.nlinks, 7, device=dev), torch.empty(b, 6, rob.nv, device=dev),
+               torch.empty(b, rob.nv, device=dev), torch.empty(b, rob.nv, rob.nv, device=dev))
+              for rob, b in zip(robs, nb)]
+    hl = launcher_hetero(models, nb, qs, qds, qdds, *[[o[k] for o in outs_h] for k in range(4)])
+    hs = torch.cuda.current_stream(dev)
+    for _ in range(20):
+        hl(hs.cuda_stream)
+    h0, h1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = _t.perf_counter()
+    h0.record(hs)
+    for _ in range(500):
+        hl(hs.cuda_stream)
+    h1.record(hs)
+    torch.cuda.synchronize()
+    wall = (_t.perf_counter() - t0) / 500
+    kms = h0.elapsed_time(h1) / 500
+    # algorithmic bytes: q, qd, qdd in; poses, J, tau and the upper triangle of M out
+    hbytes = sum(b * 4 * (3 * r.nv + 7 * r.nlinks + 6 * r.nv + r.nv + r.nv * (r.nv + 1) // 2)
+                 for r, b in zip(robs, nb))
+    res["hetero_fk_jac_rnea_crba"] = {
+        "envs": nb, "evals_s": 8192 / wall, "ms": wall * 1e3, "kernel_ms": kms,
+        "bytes_per_launch": hbytes, "achieved_gbs": hbytes / (kms * 1e-3) / 1e9,
+        "frac": hbytes / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "python_api_ms": sec * 1e3,
+        "note": "one launch through the C ABI over preallocated outputs (M written in full, the byte count takes "
+                "its upper triangle); python_api_ms = the Python method, outputs allocated per call"}
 
     # CLIK: 65,536 problems, targets = FK of random configurations, q0 = neutral
     nc = _split(65536)

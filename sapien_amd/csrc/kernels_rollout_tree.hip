@@ -35,14 +35,6 @@ constexpr int kTreeDynJoints = 32;
 template <class Real>
 constexpr int kDynTableWords4 = (int)((kHeaderBytes + sizeof(JointRec<Real>) * kTreeDynJoints) / 16);
 
-// Orders one wavefront's LDS accesses across lanes: the fences keep the
-// compiler from moving loads or stores across the point (a wavefront's own LDS
-// operations complete in order, so no wait is emitted at wavefront scope).
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Compile-time loop: f(std::integral_constant<int, K>) for K in [B, E), so
 // register arrays indexed by K never fall back to the private segment (the
@@ -189,10 +181,12 @@ template <int G, class Real> struct TreeFd {
     L.S[g][5] = col.a.z;
     wave_sync();
 #pragma unroll 4
-    for (int i = 0; i < nj; ++i) {
+    for (int i = 0; i < nj; ++i) {  // branch-free: loads issued up front, product selected
       const S6<Real> ci = {{L.S[i][0], L.S[i][1], L.S[i][2]}, {L.S[i][3], L.S[i][4], L.S[i][5]}};
-      const bool anc = i <= g && g < i + joints_of(M)[i].subtree;  // i ancestor-or-self of g
-      L.A[g][i] = valid && anc ? dot6(ci, Fg) : Real(0);
+      const int sub_i = joints_of(M)[i].subtree;
+      const Real dij = dot6(ci, Fg);
+      const bool anc = valid && i <= g && g < i + sub_i;  // i ancestor-or-self of g
+      L.A[g][i] = anc ? dij : Real(0);
     }
     for (int i = nj; i < G; ++i) L.A[g][i] = Real(0);
     wave_sync();

@@ -15,6 +15,8 @@
 // col_i . (Yc_j col_j), for i a descendant it is col_j . (Yc_i col_i), else 0.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "lanes.h"
 #include "launch.h"
 #include "spatial.h"
@@ -36,10 +38,15 @@ struct TreeIO {
   int64_t ld_q = 0, ld_qd = 0, ld_qdd = 0, ld_tau = 0, pose_env = 0, pose_link = 0;
 };
 
-// One environment per G-lane group; M is the LDS copy of the model table.
+// Dynamic LDS of a launch that computes M: per lane its joint column and
+// composite-inertia force (12 floats).
+__host__ __device__ constexpr int crba_lds_bytes(int) { return 256 * 12 * 4; }
+
+// One environment per G-lane group; M is the LDS copy of the model table;
+// crba is the block's dynamic LDS (used when flags has KF_CRBA).
 template <int G>
 __device__ __forceinline__ void tree_body(const ModelTable<float> *M, const ModelTable<float> *Mg, int64_t B,
-                                          int64_t env, int flags, TreeIO io) {
+                                          int64_t env, int flags, TreeIO io, float *crba) {
   const int g = threadIdx.x & (G - 1);
   const bool live = env < B;
   const int64_t e = live ? env : B - 1;
@@ -138,14 +145,30 @@ __device__ __forceinline__ void tree_body(const ModelTable<float> *M, const Mode
 #pragma unroll
     for (int i = 0; i < 6; ++i) Yc.I[i] = y[4 + i];
     const S6<float> Fj = apply(Yc, col);
-    for (int i = 0; i < nj; ++i) {  // uniform loop over the other joints
-      const int s4 = T.base4 + 4 * i;
-      const S6<float> ci = bperm6(s4, col), Fi = bperm6(s4, Fj);
+    // every lane publishes its column and force; lane j then reads those of
+    // the other joints (one address across the group at a time) and stores
+    // row u of M
+    float *cf = crba + (threadIdx.x & ~(G - 1) & 255) * 12;
+    cf[g * 12 + 0] = col.l.x; cf[g * 12 + 1] = col.l.y; cf[g * 12 + 2] = col.l.z;
+    cf[g * 12 + 3] = col.a.x; cf[g * 12 + 4] = col.a.y; cf[g * 12 + 5] = col.a.z;
+    cf[g * 12 + 6] = Fj.l.x; cf[g * 12 + 7] = Fj.l.y; cf[g * 12 + 8] = Fj.l.z;
+    cf[g * 12 + 9] = Fj.a.x; cf[g * 12 + 10] = Fj.a.y; cf[g * 12 + 11] = Fj.a.z;
+    wave_sync();
+    // branch-free: every load is issued up front, both products computed and
+    // selected
+    float *Mrow = io.M + (e * nv + u) * nv;
+    const bool out = live && valid;
+#pragma unroll 4
+    for (int i = 0; i < nj; ++i) {
+      const float *ri = cf + i * 12;
+      const S6<float> ci = {{ri[0], ri[1], ri[2]}, {ri[3], ri[4], ri[5]}};
+      const S6<float> Fi = {{ri[6], ri[7], ri[8]}, {ri[9], ri[10], ri[11]}};
       const JointRec<float> &Ji = joints_of(M)[i];
-      const bool i_anc_j = i <= g && g < i + Ji.subtree;
+      const int sub_i = Ji.subtree, vu_i = Ji.vuser;
+      const float dij = dot6(ci, Fj), dji = dot6(col, Fi);
+      const bool i_anc_j = i <= g && g < i + sub_i;
       const bool j_anc_i = g < i && i < g + T.size;
-      const float mij = i_anc_j ? dot6(ci, Fj) : j_anc_i ? dot6(col, Fi) : 0.f;
-      if (live && valid) st<NT>(mij, io.M + (e * nv + u) * nv + Ji.vuser);
+      if (out) st<false>(i_anc_j ? dij : j_anc_i ? dji : 0.f, Mrow + vu_i);
     }
   }
 }
@@ -268,10 +291,11 @@ template <int G>
 __global__ __launch_bounds__(256) void k_tree_fused(const ModelTable<float> *__restrict__ Mg, int64_t B, int flags,
                                                     TreeIO io) {
   __shared__ float4 s_tab[kTreeTableWords4];
+  extern __shared__ float s_crba[];  // crba_lds_bytes(G) when KF_CRBA, else empty
   stage_table(s_tab, Mg);
   __syncthreads();
   const int64_t env = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
-  tree_body<G>(reinterpret_cast<const ModelTable<float> *>(s_tab), Mg, B, env, flags, io);
+  tree_body<G>(reinterpret_cast<const ModelTable<float> *>(s_tab), Mg, B, env, flags, io, s_crba);
 }
 
 template <int G>
@@ -289,6 +313,7 @@ __global__ __launch_bounds__(256) void k_tree_vjp(const ModelTable<float> *__res
 // Heterogeneous launch: block ranges map to segments (one tree type each).
 __global__ __launch_bounds__(256) void k_hetero(HeteroArgs args, int flags) {
   __shared__ float4 s_tab[kTreeTableWords4];
+  extern __shared__ float s_crba[];  // crba_lds_bytes(largest G) when KF_CRBA, else empty
   int s = 0;
   while (s + 1 < args.nseg && (int64_t)blockIdx.x >= args.seg[s + 1].block0) ++s;
   const HeteroSegment &S = args.seg[s];
@@ -303,9 +328,9 @@ __global__ __launch_bounds__(256) void k_hetero(HeteroArgs args, int flags) {
   if (!S.tau) f &= ~KF_RNEA;
   if (!S.M) f &= ~KF_CRBA;
   switch (S.G) {  // uniform per block
-    case 8: tree_body<8>(M, S.tab, S.B, local / 8, f, io); break;
-    case 16: tree_body<16>(M, S.tab, S.B, local / 16, f, io); break;
-    default: tree_body<32>(M, S.tab, S.B, local / 32, f, io); break;
+    case 8: tree_body<8>(M, S.tab, S.B, local / 8, f, io, s_crba); break;
+    case 16: tree_body<16>(M, S.tab, S.B, local / 16, f, io, s_crba); break;
+    default: tree_body<32>(M, S.tab, S.B, local / 32, f, io, s_crba); break;
   }
 }
 
@@ -328,9 +353,11 @@ hipError_t launch_tree_fused(const ModelTable<float> *tab, int nj, int64_t B, in
   io.pose_link = out.pose_link;
   const int64_t per_block = 256 / G;
   dim3 grid((unsigned)((B + per_block - 1) / per_block));
-  if (G == 8) hipLaunchKernelGGL(k_tree_fused<8>, grid, dim3(256), 0, s, tab, B, flags, io);
-  else if (G == 16) hipLaunchKernelGGL(k_tree_fused<16>, grid, dim3(256), 0, s, tab, B, flags, io);
-  else hipLaunchKernelGGL(k_tree_fused<32>, grid, dim3(256), 0, s, tab, B, flags, io);
+  if (!out.M) flags &= ~KF_CRBA;  // the CRBA tile is allocated only with an M output
+  const unsigned lds = (flags & KF_CRBA) ? (unsigned)crba_lds_bytes(G) : 0u;
+  if (G == 8) hipLaunchKernelGGL(k_tree_fused<8>, grid, dim3(256), lds, s, tab, B, flags, io);
+  else if (G == 16) hipLaunchKernelGGL(k_tree_fused<16>, grid, dim3(256), lds, s, tab, B, flags, io);
+  else hipLaunchKernelGGL(k_tree_fused<32>, grid, dim3(256), lds, s, tab, B, flags, io);
   return hipGetLastError();
 }
 
@@ -354,7 +381,11 @@ hipError_t launch_hetero(const HeteroArgs &args, int64_t total_blocks, int flags
   if (total_blocks <= 0) return hipSuccess;
   for (int i = 0; i < args.nseg; ++i)
     if (!tree_group(args.seg[i].nj) || args.seg[i].G != tree_group(args.seg[i].nj)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_hetero, dim3((unsigned)total_blocks), dim3(256), 0, s, args, flags);
+  unsigned lds = 0;
+  if (flags & KF_CRBA)
+    for (int i = 0; i < args.nseg; ++i)
+      if (args.seg[i].M) lds = std::max(lds, (unsigned)crba_lds_bytes(args.seg[i].G));
+  hipLaunchKernelGGL(k_hetero, dim3((unsigned)total_blocks), dim3(256), lds, s, args, flags);
   return hipGetLastError();
 }
 

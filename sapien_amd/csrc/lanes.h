@@ -40,6 +40,15 @@ template <class T> __device__ __forceinline__ S6<T> sel(bool t, S6<T> a, S6<T> b
           {t ? a.a.x : b.a.x, t ? a.a.y : b.a.y, t ? a.a.z : b.a.z}};
 }
 
+// Orders one wavefront's LDS accesses across lanes: the fences keep the
+// compiler from moving loads or stores across the point (a wavefront's own LDS
+// operations complete in order, so no wait is emitted at wavefront scope).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // sin and cos of a joint angle. For |x| < 16 rad the hardware sine / cosine
 // (v_sin_f32 / v_cos_f32, argument scaled to revolutions): absolute error
 // below 1e-6 there, measured against the FP64 oracle through the kernels'

@@ -90,3 +90,29 @@ def launcher_fk_jac_rnea(model):
             check(rc)
 
     return launch
+
+
+def launcher_hetero(models, B, q, qd, qdd, poses, J, tau, M):
+    """f(stream) launching sapamd_hetero_batched (FK + J + RNEA + CRBA of
+    several models in one launch) over fixed per-model device tensors (lists,
+    one entry per model; an output entry may be None)."""
+    L = lib()
+    fn = L.sapamd_hetero_batched
+    P, I32 = ctypes.c_void_p, ctypes.c_int32
+    fn.argtypes = [P, I32, P, P, P, P, P, P, P, P, P]
+    fn.restype = ctypes.c_int
+    n = len(models)
+    hs = (ctypes.c_void_p * n)(*[handle(m).value for m in models])
+    Bs = (ctypes.c_int64 * n)(*B)
+
+    def arr(xs):
+        return (ctypes.c_void_p * n)(*[None if x is None else x.data_ptr() for x in xs])
+
+    args = [hs, n, Bs, arr(q), arr(qd), arr(qdd), arr(poses), arr(J), arr(tau), arr(M)]
+
+    def launch(stream):
+        rc = fn(*args, ctypes.c_void_p(stream))
+        if rc:
+            check(rc)
+
+    return launch
