@@ -466,8 +466,9 @@ def main():
     if traffic and "VALU" in traffic:
         v = traffic["VALU"]
         # the SQ counts FLOPs per wavefront instruction (1 per add / mul /
-        # transcendental, 2 per FMA): x 64 lanes
-        flops = 64 * (v["SQ_INSTS_VALU_FLOPS_FP32"] + v["SQ_INSTS_VALU_FLOPS_FP32_TRANS"])
+        # transcendental, 2 per FMA; transcendentals included, _TRANS reports
+        # them separately): x 64 lanes
+        flops = 64 * v["SQ_INSTS_VALU_FLOPS_FP32"]
         tflops = flops / (kern_ms * 1e-3) / 1e12
         out["roofline"]["valu_fp32"] = {
             "flops_per_launch": flops, "achieved_tflops": tflops, "peak_tflops": FP32_PEAK_TFLOPS,
@@ -475,7 +476,7 @@ def main():
             "from_instruction_counts": 64 * (2 * v["SQ_INSTS_VALU_FMA_F32"] + v["SQ_INSTS_VALU_ADD_F32"] +
                                              v["SQ_INSTS_VALU_MUL_F32"] + v["SQ_INSTS_VALU_TRANS_F32"]),
             "valu_instructions_per_launch": v["SQ_INSTS_VALU"],
-            "method": "rocprofv3 --pmc SQ_INSTS_VALU_FLOPS_FP32(+_TRANS) x 64 lanes, cross-checked by "
+            "method": "rocprofv3 --pmc SQ_INSTS_VALU_FLOPS_FP32 x 64 lanes, cross-checked by "
                       "64 x (2 FMA + ADD + MUL + TRANS)_F32 instruction counts; SQ_INSTS_VALU; median of 16 "
                       "launches, over the launch duration kernel_ms"}
     if traffic:
