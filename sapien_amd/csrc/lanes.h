@@ -16,6 +16,12 @@ template <int CTRL> __device__ __forceinline__ float dppf(float x) {
 template <int D> __device__ __forceinline__ float from_below(float x) { return dppf<0x110 + D>(x); }
 // value held by lane g + D of the same 16-lane row (0 past the row end)
 template <int D> __device__ __forceinline__ float from_above(float x) { return dppf<0x100 + D>(x); }
+template <int D> __device__ __forceinline__ double from_above(double x) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x100 + D, 0xF, 0xF, true);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0x100 + D, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
 // value held by wave lane src4 / 4
 __device__ __forceinline__ float bperm(int src4, float x) {
   return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src4, __builtin_bit_cast(int, x)));

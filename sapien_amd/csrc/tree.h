@@ -47,6 +47,17 @@ template <int G, class T = float> struct Tree {
       x = sel(a >= 0, x + y, x);
     }
   }
+  // value of lane g + 2^k: a DPP row shift when the group lies in one 16-lane
+  // row (G <= 16; the caller masks lanes whose source is past the group),
+  // ds_bpermute otherwise
+  template <int k> __device__ __forceinline__ T from_lane_above(T v) const {
+    if constexpr (G <= 16) {
+      return from_above<(1 << k)>(v);
+    } else {
+      const int src = g + (1 << k);
+      return bperm(base4 + 4 * (src < G ? src : 0), v);
+    }
+  }
   // subtree sum of x over [g, g + size) (x = 0 on lanes without a joint)
   template <int N> __device__ __forceinline__ void subtree(T (&x)[N]) const {
     T blk[K][N];
@@ -57,12 +68,18 @@ template <int G, class T = float> struct Tree {
     for (int k = 0; k < K; ++k) {
 #pragma unroll
       for (int i = 0; i < N; ++i) blk[k][i] = cur[i];
-      const int src = g + (1 << k);
-      const int s4 = base4 + 4 * (src < G ? src : 0);
+      const bool in = g + (1 << k) < G;
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        const T y = bperm(s4, cur[i]);
-        cur[i] = src < G ? cur[i] + y : cur[i];
+        T y;
+        switch (k) {  // k is a constant once the loop is unrolled
+          case 0: y = from_lane_above<0>(cur[i]); break;
+          case 1: y = from_lane_above<1>(cur[i]); break;
+          case 2: y = from_lane_above<2>(cur[i]); break;
+          case 3: y = from_lane_above<3>(cur[i]); break;
+          default: y = from_lane_above<4>(cur[i]); break;
+        }
+        cur[i] = in ? cur[i] + y : cur[i];
       }
     }
     // blocks of size 2^K cover the whole group: size <= G
