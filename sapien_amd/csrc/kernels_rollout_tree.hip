@@ -61,20 +61,21 @@ template <int B, int E, class F> __device__ __forceinline__ void static_for(F &&
 // the group's G-entry LDS buffer. After the call row[j < i] = L_ij (entries
 // above the diagonal hold junk: the update runs over the whole row to need no
 // per-entry select, and nothing reads them); returns D_i. Lanes i >= n hold
-// zero rows. The steps run over all G columns (the ones past n change
-// nothing), so no step depends on a runtime bound.
-template <int G, class Real> __device__ __forceinline__ Real ldlt_factor(Real (&row)[G], Real *buf, int i, int n) {
+// zero rows. The steps run over the first NMAX >= n columns, a compile-time
+// bound (steps past n change nothing), so no step depends on a runtime value.
+template <int G, int NMAX, class Real>
+__device__ __forceinline__ Real ldlt_factor(Real (&row)[G], Real *buf, int i, int n) {
   Real d = Real(1);
-  static_for<0, G>([&](auto kc) {
+  static_for<0, NMAX>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
     buf[i] = row[k];
     wave_sync();
     Real c[G];
-    static_for<k, G>([&](auto jc) { c[decltype(jc)::value] = buf[decltype(jc)::value]; });
+    static_for<k, NMAX>([&](auto jc) { c[decltype(jc)::value] = buf[decltype(jc)::value]; });
     wave_sync();
     const bool act = i > k && i < n;
     const Real l = act ? row[k] / c[k] : Real(0);
-    static_for<k + 1, G>([&](auto jc) {
+    static_for<k + 1, NMAX>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
       row[j] -= l * c[j];
     });
@@ -90,31 +91,31 @@ template <int G, class Real> __device__ __forceinline__ Real ldlt_factor(Real (&
 // reads lane i's column. Each step broadcasts the finished entry with
 // ds_bpermute; the factor entries are masked once (strictly lower row,
 // strictly below-diagonal column), so the steps need no selects.
-template <int G, class Real, int R>
+template <int G, int NMAX, class Real, int R>
 __device__ __forceinline__ void ldlt_solve(const Real (&row)[G], Real d, Real (*A)[G + 1], int base4, int i, int n,
                                            Real (&r)[R]) {
   Real lo[G];
-  static_for<0, G>([&](auto kc) {
+  static_for<0, NMAX>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
     lo[k] = k < i ? row[k] : Real(0);
   });
-  static_for<0, G>([&](auto kc) {
+  static_for<0, NMAX>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
 #pragma unroll
     for (int m = 0; m < R; ++m) r[m] -= lo[k] * bperm(base4 + 4 * k, r[m]);
   });
 #pragma unroll
   for (int m = 0; m < R; ++m) r[m] = i < n ? r[m] / d : Real(0);
-  static_for<0, G>([&](auto jc) { A[i][decltype(jc)::value] = row[decltype(jc)::value]; });
+  static_for<0, NMAX>([&](auto jc) { A[i][decltype(jc)::value] = row[decltype(jc)::value]; });
   wave_sync();
   Real col[G];
-  static_for<0, G>([&](auto kc) {
+  static_for<0, NMAX>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
     col[k] = k > i ? A[k][i] : Real(0);
   });
   wave_sync();
-  static_for<0, G>([&](auto kc) {
-    constexpr int k = G - 1 - decltype(kc)::value;
+  static_for<0, NMAX>([&](auto kc) {
+    constexpr int k = NMAX - 1 - decltype(kc)::value;
 #pragma unroll
     for (int m = 0; m < R; ++m) r[m] -= col[k] * bperm(base4 + 4 * k, r[m]);
   });
@@ -199,7 +200,7 @@ template <int G, class Real> struct TreeFd {
 template <int G, class Real>
 __device__ __forceinline__ void init_fd(TreeFd<G, Real> &F, const ModelTable<Real> *M, int g) {
   F.g = g;
-  F.nj = M->nj;
+  F.nj = __builtin_amdgcn_readfirstlane(M->nj);  // the same in every lane: a scalar
   F.valid = g < F.nj;
   F.J = joints_of(M) + (F.valid ? g : F.nj - 1);
   F.T.g = g;
@@ -221,32 +222,32 @@ __device__ __forceinline__ const ModelTable<Real> *stage_table(const ModelTable<
 }
 
 // Forward dynamics of this lane's joint at (q, qd, tau) (joint order).
-template <int G, class Real>
+template <int G, int NMAX, class Real>
 __device__ __forceinline__ Real lane_fd(TreeFd<G, Real> &F, const ModelTable<Real> *M, DynLds<G, Real> &L, Real qj,
                                         Real qdj, Real tj) {
   Real row[G];
   const Real bias = F.setup(M, qj, qdj, L, row);
   const int i = opaque_lane(F.g);
-  const Real d = ldlt_factor<G, Real>(row, L.y, i, F.nj);
+  const Real d = ldlt_factor<G, NMAX, Real>(row, L.y, i, F.nj);
   Real r[1] = {tj - bias};
-  ldlt_solve<G, Real, 1>(row, d, L.A, F.T.base4, i, F.nj, r);
+  ldlt_solve<G, NMAX, Real, 1>(row, d, L.A, F.T.base4, i, F.nj, r);
   return r[0];
 }
 
 // Adjoint of one semi-implicit Euler step at the pre-step state (q, qd) for
 // this lane's joint: (aq, aqd) post-step adjoint in, pre-step adjoint out;
 // returns dL/dtau_j = lambda_j.
-template <int G, class Real>
+template <int G, int NMAX, class Real>
 __device__ __forceinline__ Real lane_step_vjp(TreeFd<G, Real> &F, const ModelTable<Real> *M, DynLds<G, Real> &L,
                                               float dt, Real qj, Real qdj, Real tj, Real &aqj, Real &aqdj) {
   Real row[G];
   const Real bias = F.setup(M, qj, qdj, L, row);
   const int i = opaque_lane(F.g);
-  const Real d = ldlt_factor<G, Real>(row, L.y, i, F.nj);
+  const Real d = ldlt_factor<G, NMAX, Real>(row, L.y, i, F.nj);
   const Real dtr = dt;
   const Real cj = aqdj + dtr * aqj;
   Real r[2] = {tj - bias, dtr * cj};  // qdd and lambda = M^-1 (dt c) in one pass
-  ldlt_solve<G, Real, 2>(row, d, L.A, F.T.base4, i, F.nj, r);
+  ldlt_solve<G, NMAX, Real, 2>(row, d, L.A, F.T.base4, i, F.nj, r);
   const Real qddj = r[0], lam = r[1];
 
   // RNEA vector-Jacobian product at (q, qd, qdd) for the upstream lambda
@@ -293,7 +294,7 @@ __device__ __forceinline__ Real lane_step_vjp(TreeFd<G, Real> &F, const ModelTab
 
 // mode 0: forward dynamics qdd = FD(q, qd, tau) into out (B, nv)
 // mode 1: rollout step in place on q, qd; save (pre-step q | qd) when given
-template <int G, class Real, int BLOCK>
+template <int G, class Real, int BLOCK, int NMAX>
 __global__ __launch_bounds__(BLOCK) void k_tree_step(const ModelTable<Real> *__restrict__ Mg, int64_t B, int mode,
                                                      float dt, float *__restrict__ q, float *__restrict__ qd,
                                                      const float *__restrict__ tau, int64_t tau_stride,
@@ -310,7 +311,7 @@ __global__ __launch_bounds__(BLOCK) void k_tree_step(const ModelTable<Real> *__r
   const int src = F.T.base4 + 4 * F.J->vuser;
   Real qj = bperm(src, qu), qdj = bperm(src, qdu), tj = bperm(src, tu);
   if (!F.valid) qj = qdj = tj = Real(0);
-  const Real qddj = lane_fd<G, Real>(F, M, L, qj, qdj, tj);
+  const Real qddj = lane_fd<G, NMAX, Real>(F, M, L, qj, qdj, tj);
   if (!live) return;
   if (mode == 0) {
     if (F.valid) out[e * nv + F.J->vuser] = (float)qddj;
@@ -330,7 +331,7 @@ __global__ __launch_bounds__(BLOCK) void k_tree_step(const ModelTable<Real> *__r
 
 // Adjoint of one step; state = pre-step (q | qd) rows, aq / aqd hold the
 // post-step adjoint on entry and the pre-step adjoint on exit.
-template <int G, class Real, int BLOCK>
+template <int G, class Real, int BLOCK, int NMAX>
 __global__ __launch_bounds__(BLOCK) void k_tree_step_vjp(const ModelTable<Real> *__restrict__ Mg, int64_t B, float dt,
                                                          const float *__restrict__ state, int64_t state_stride,
                                                          const float *__restrict__ tau, int64_t tau_stride,
@@ -348,7 +349,7 @@ __global__ __launch_bounds__(BLOCK) void k_tree_step_vjp(const ModelTable<Real> 
   Real qj = bperm(src, qu), qdj = bperm(src, qdu), tj = bperm(src, tu);
   Real aqj = bperm(src, aqu), aqdj = bperm(src, aqdu);
   if (!F.valid) qj = qdj = tj = aqj = aqdj = Real(0);
-  const Real lam = lane_step_vjp<G, Real>(F, M, L, dt, qj, qdj, tj, aqj, aqdj);
+  const Real lam = lane_step_vjp<G, NMAX, Real>(F, M, L, dt, qj, qdj, tj, aqj, aqdj);
   if (!live || !F.valid) return;
   const int u = F.J->vuser;
   aq[e * nv + u] = (float)aqj;
@@ -361,7 +362,7 @@ __global__ __launch_bounds__(BLOCK) void k_tree_step_vjp(const ModelTable<Real> 
 // tau (B, T, nv); ckpt (B, ceil(T / K), 2 nv) receives the pre-step state of
 // steps 0, K, 2K, ... when given; q, qd (B, nv) hold the initial state on entry
 // and the final state on exit.
-template <int G, class Real, int BLOCK>
+template <int G, class Real, int BLOCK, int NMAX>
 __global__ __launch_bounds__(BLOCK) void k_tree_rollout(const ModelTable<Real> *__restrict__ Mg, int64_t B, int T,
                                                         int K, float dt, float *__restrict__ q,
                                                         float *__restrict__ qd, const float *__restrict__ tau,
@@ -385,7 +386,7 @@ __global__ __launch_bounds__(BLOCK) void k_tree_rollout(const ModelTable<Real> *
       c[u] = qs;
       c[nv + u] = qds;
     }
-    const Real qddj = lane_fd<G, Real>(F, M, L, F.valid ? Real(qs) : Real(0), F.valid ? Real(qds) : Real(0),
+    const Real qddj = lane_fd<G, NMAX, Real>(F, M, L, F.valid ? Real(qs) : Real(0), F.valid ? Real(qds) : Real(0),
                                        F.valid ? Real(tcur) : Real(0));
     const float v1 = qds + dt * (float)qddj;
     qs = qs + dt * v1;
@@ -402,7 +403,7 @@ __global__ __launch_bounds__(BLOCK) void k_tree_rollout(const ModelTable<Real> *
 // (B, K, 2 nv; each lane reads back only what it wrote), then the step
 // adjoints in reverse. aq, aqd: dL/d(final state) in, dL/d(initial state)
 // out; gtau (B, T, nv) = dL/dtau.
-template <int G, class Real, int BLOCK>
+template <int G, class Real, int BLOCK, int NMAX>
 __global__ __launch_bounds__(BLOCK) void k_tree_rollout_vjp(const ModelTable<Real> *__restrict__ Mg, int64_t B, int T,
                                                             int K, float dt, const float *__restrict__ tau,
                                                             const float *__restrict__ ckpt, float *__restrict__ seg,
@@ -430,7 +431,7 @@ __global__ __launch_bounds__(BLOCK) void k_tree_rollout_vjp(const ModelTable<Rea
         sg[(int64_t)(t - t0) * 2 * nv + nv] = qds;
       }
       if (t + 1 == t1) break;  // the last pre-step state needs no step after it
-      const Real qddj = lane_fd<G, Real>(F, M, L, F.valid ? Real(qs) : Real(0), F.valid ? Real(qds) : Real(0),
+      const Real qddj = lane_fd<G, NMAX, Real>(F, M, L, F.valid ? Real(qs) : Real(0), F.valid ? Real(qds) : Real(0),
                                          F.valid ? Real(tr[(int64_t)t * nv]) : Real(0));
       const float v1 = qds + dt * (float)qddj;
       qs = qs + dt * v1;
@@ -440,7 +441,7 @@ __global__ __launch_bounds__(BLOCK) void k_tree_rollout_vjp(const ModelTable<Rea
       const Real qj = F.valid ? Real(sg[(int64_t)(t - t0) * 2 * nv]) : Real(0);
       const Real qdj = F.valid ? Real(sg[(int64_t)(t - t0) * 2 * nv + nv]) : Real(0);
       const Real tj = F.valid ? Real(tr[(int64_t)t * nv]) : Real(0);
-      const Real lam = lane_step_vjp<G, Real>(F, M, L, dt, qj, qdj, tj, aqj, aqdj);
+      const Real lam = lane_step_vjp<G, NMAX, Real>(F, M, L, dt, qj, qdj, tj, aqj, aqdj);
       // the one-step kernel hands the adjoint on in FP32: round the same way
       aqj = F.valid ? Real((float)aqj) : Real(0);
       aqdj = F.valid ? Real((float)aqdj) : Real(0);
@@ -456,18 +457,24 @@ __global__ __launch_bounds__(BLOCK) void k_tree_rollout_vjp(const ModelTable<Rea
 // FP32: 256-thread blocks; FP64: 128-thread blocks (half the LDS per block).
 template <class Real> constexpr int kStepBlock = sizeof(Real) == 4 ? 256 : 128;
 
-// Launches KERNEL<G, Real, BL> for the group size of nj joints.
+// Launches KERNEL<G, Real, BL, NMAX> for nj joints: G lanes per environment,
+// the factorization bounded by NMAX, the smallest of {8, 12, 16, 24, 32} >= nj
+// within the group size.
 #define SAPAMD_TREE_DYN_LAUNCH(KERNEL, ...)                                                          \
   do {                                                                                               \
     constexpr int BL = kStepBlock<Real>;                                                             \
     const int G = tree_dyn_group(nj);                                                                \
     const dim3 grid((unsigned)((B * G + BL - 1) / BL));                                              \
     if (G == 8)                                                                                      \
-      hipLaunchKernelGGL((KERNEL<8, Real, BL>), grid, dim3(BL), 0, s, __VA_ARGS__);                  \
+      hipLaunchKernelGGL((KERNEL<8, Real, BL, 8>), grid, dim3(BL), 0, s, __VA_ARGS__);               \
+    else if (G == 16 && nj <= 12)                                                                    \
+      hipLaunchKernelGGL((KERNEL<16, Real, BL, 12>), grid, dim3(BL), 0, s, __VA_ARGS__);             \
     else if (G == 16)                                                                                \
-      hipLaunchKernelGGL((KERNEL<16, Real, BL>), grid, dim3(BL), 0, s, __VA_ARGS__);                 \
+      hipLaunchKernelGGL((KERNEL<16, Real, BL, 16>), grid, dim3(BL), 0, s, __VA_ARGS__);             \
+    else if (nj <= 24)                                                                               \
+      hipLaunchKernelGGL((KERNEL<32, Real, BL, 24>), grid, dim3(BL), 0, s, __VA_ARGS__);             \
     else                                                                                             \
-      hipLaunchKernelGGL((KERNEL<32, Real, BL>), grid, dim3(BL), 0, s, __VA_ARGS__);                 \
+      hipLaunchKernelGGL((KERNEL<32, Real, BL, 32>), grid, dim3(BL), 0, s, __VA_ARGS__);             \
   } while (0)
 
 template <class Real>
