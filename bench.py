@@ -486,6 +486,39 @@ def main():
             "method": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch, median of 16 launches over 16 "
                       "buffer sets), scaled so an elementwise kernel over 256 MiB reads and writes 256 MiB"}
 
+    # the same kernel over other batch sizes (not the metric): how the launch
+    # cost amortizes. Buffer sets rotate over > 600 MB so that inputs come
+    # from HBM; autotuned shape per size; HIP events on the launch stream.
+    sweep = []
+    for Bs in (4096, 65536, 262144, 1048576):
+        shp = tune.autotune(m, batch_sizes=(Bs,), use_cache=False)[Bs][:2] if tuned else None
+        nset = max(2, -(-600_000_000 // (BYTES_PER_EVAL * Bs)))
+        bsets = []
+        for k in range(nset):
+            qb = sets[k % NSETS]
+            reps = -(-Bs // B)
+            bsets.append([x.repeat(reps, 1)[:Bs].contiguous() for x in (qb["q"], qb["qd"], qb["qdd"])] +
+                         [torch.empty(Bs, L, 7, device=dev), torch.empty(Bs, 6, n, device=dev),
+                          torch.empty(Bs, n, device=dev)])
+        for i in range(2 * nset):
+            launch(Bs, *bsets[i % nset], stream.cuda_stream)
+        s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps_t = max(50, nset)
+        torch.cuda.synchronize()
+        s0.record(stream)
+        for i in range(reps_t):
+            launch(Bs, *bsets[i % nset], stream.cuda_stream)
+        s1.record(stream)
+        torch.cuda.synchronize()
+        kus = s0.elapsed_time(s1) / reps_t * 1e3
+        gbs = BYTES_PER_EVAL * Bs / (kus * 1e-6) / 1e9
+        sweep.append({"envs": Bs, "kernel_us": kus, "evals_s": Bs / (kus * 1e-6), "achieved_gbs": gbs,
+                      "frac": gbs / HBM_PEAK_GBS, "shape": list(shp) if shp else None, "buffer_sets": nset})
+        del bsets
+    if tuned:
+        tune.set_shape(m, B, tuned[B][0], tuned[B][1])
+    out["batch_sweep"] = sweep
+
     # gradient throughput: 8192 environments of the 7-dof and 9-dof trees,
     # loss = |p_ee - FK(q*)|^2 + |tau_RNEA(q, qd, qdd) - tau*|^2. Forward-only
     # = the fused FK + RNEA evaluation of that loss's inputs; forward + backward
