@@ -48,11 +48,17 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
                                                      float *__restrict__ Jout, float *__restrict__ tau,
                                                      Layout lay) {
   constexpr bool NT = G >= 8;  // streaming output stores (see st())
+  // 8 x 1: interleaved pairs (lanes.h), joint j of environment p of a row on
+  // lane 2j + p, so the scans are select-free DPP shifts of 2D lanes
+  constexpr bool IL = G == 8 && C == 1;
+  constexpr int lstep = IL ? 2 : 1;  // lane distance between consecutive joint slots
   // model header + joint records staged in LDS once per block
   __shared__ float4 s_tab[kTableWords4];
-  const int g = threadIdx.x & (G - 1);
-  const int base4 = ((threadIdx.x & 63) & ~(G - 1)) * 4;
-  const int64_t env = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
+  const int lane = threadIdx.x & 63;
+  const int g = IL ? (lane & 15) >> 1 : (int)(threadIdx.x & (G - 1));
+  const int lane0 = IL ? ((lane & ~15) | (lane & 1)) : (lane & ~(G - 1));  // the group's joint slot 0
+  const int64_t env = IL ? (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / 16 * 2 + (lane & 1)
+                         : (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
   const bool live = env < B;
   const int64_t e = live ? env : B - 1;
   const int nj = Mg->nj, nv = Mg->nv, L = Mg->nlinks;
@@ -94,7 +100,7 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
       qdj[t] = qdu[t];
       qddj[t] = qddu[t];
     } else {
-      const int src = base4 + 4 * (u / C), slot = u % C;
+      const int src = 4 * (lane0 + lstep * (u / C)), slot = u % C;
       qj[t] = qdj[t] = qddj[t] = 0.f;
 #pragma unroll
       for (int s = 0; s < C; ++s) {
@@ -124,7 +130,8 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
   // need the exclusive prefix (none when C = 1).
   if constexpr (G > 1) {
     Xf incl = X[C - 1];
-    scan_xf<G>(g, incl);
+    if constexpr (IL) scan_xf_il(incl);
+    else scan_xf<G>(g, incl);
     if constexpr (C > 1) {
       const Xf E = excl_xf<G>(g, incl);
 #pragma unroll
@@ -177,7 +184,7 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
       const int l = l0 + g;
       const LinkRec<float> &LL = links_of(Mg)[l < L ? l : L - 1];  // link records stay in global memory
       const int jl = LL.joint < 0 ? 0 : LL.joint;
-      const int src = base4 + 4 * (jl / C), slot = jl % C;
+      const int src = 4 * (lane0 + lstep * (jl / C)), slot = jl % C;
       Xf W;
 #pragma unroll
       for (int s = 0; s < C; ++s) {
@@ -206,7 +213,8 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
   for (int t = 0; t < C; ++t) v[t] = (t > 0 ? v[t - 1] : zero6<float>()) + qdj[t] * col[t];
   if constexpr (G > 1) {
     S6<float> incl = v[C - 1];
-    scan_sum<G>(g, incl);
+    if constexpr (IL) scan_sum_il(incl);
+    else scan_sum<G>(g, incl);
     if constexpr (C > 1) {
       const S6<float> E = excl_sum<G>(g, incl);
 #pragma unroll
@@ -220,7 +228,8 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
   S6<float> Ea = zero6<float>();
   if constexpr (G > 1) {
     S6<float> incl = a[C - 1];
-    scan_sum<G>(g, incl);
+    if constexpr (IL) scan_sum_il(incl);
+    else scan_sum<G>(g, incl);
     if constexpr (C > 1) Ea = excl_sum<G>(g, incl);
     a[C - 1] = incl;
   }
@@ -238,7 +247,8 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
   for (int t = C - 2; t >= 0; --t) f[t] = f[t] + f[t + 1];  // suffix sums inside the lane
   if constexpr (G > 1) {
     S6<float> incl = f[0];
-    scan_suffix<G>(g, incl);
+    if constexpr (IL) scan_suffix_il(incl);
+    else scan_suffix<G>(g, incl);
     if constexpr (C > 1) {
       const S6<float> E = excl_suffix<G>(g, incl);
 #pragma unroll

@@ -137,6 +137,51 @@ template <int G> __device__ __forceinline__ S6<float> excl_suffix(int g, S6<floa
   else return sel(g + 1 < G, above6<1>(incl), zero6<float>());
 }
 
+// Interleaved pairs (8 lanes per environment): the two environments of a
+// 16-lane row take the even and the odd lanes, joint j of environment p on
+// lane 2j + p. A step of D joints is then a DPP row shift of 2D lanes: it never
+// mixes the two environments, and a source past the row edge yields the
+// instruction's `old` operand (identity entry or zero), so the scans need no
+// selects.
+template <int CTRL> __device__ __forceinline__ float dpp_old(float old, float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old), __builtin_bit_cast(int, x),
+                                                               CTRL, 0xF, 0xF, false));
+}
+template <int D> __device__ __forceinline__ Xf below_xf_il(const Xf &x) {  // joint j - D, identity before joint 0
+  Xf o;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) o.R[k] = dpp_old<0x110 + 2 * D>((k % 4 == 0) ? 1.f : 0.f, x.R[k]);
+  o.p = {dpp_old<0x110 + 2 * D>(0.f, x.p.x), dpp_old<0x110 + 2 * D>(0.f, x.p.y), dpp_old<0x110 + 2 * D>(0.f, x.p.z)};
+  return o;
+}
+template <int D> __device__ __forceinline__ S6<float> below6_il(S6<float> v) {
+  return {{dpp_old<0x110 + 2 * D>(0.f, v.l.x), dpp_old<0x110 + 2 * D>(0.f, v.l.y), dpp_old<0x110 + 2 * D>(0.f, v.l.z)},
+          {dpp_old<0x110 + 2 * D>(0.f, v.a.x), dpp_old<0x110 + 2 * D>(0.f, v.a.y), dpp_old<0x110 + 2 * D>(0.f, v.a.z)}};
+}
+template <int D> __device__ __forceinline__ S6<float> above6_il(S6<float> v) {
+  return {{dpp_old<0x100 + 2 * D>(0.f, v.l.x), dpp_old<0x100 + 2 * D>(0.f, v.l.y), dpp_old<0x100 + 2 * D>(0.f, v.l.z)},
+          {dpp_old<0x100 + 2 * D>(0.f, v.a.x), dpp_old<0x100 + 2 * D>(0.f, v.a.y), dpp_old<0x100 + 2 * D>(0.f, v.a.z)}};
+}
+// inclusive prefix product / prefix sum / suffix sum over the 8 joints
+template <int D = 1> __device__ __forceinline__ void scan_xf_il(Xf &x) {
+  if constexpr (D < 8) {
+    x = compose(below_xf_il<D>(x), x);
+    scan_xf_il<2 * D>(x);
+  }
+}
+template <int D = 1> __device__ __forceinline__ void scan_sum_il(S6<float> &x) {
+  if constexpr (D < 8) {
+    x = x + below6_il<D>(x);
+    scan_sum_il<2 * D>(x);
+  }
+}
+template <int D = 1> __device__ __forceinline__ void scan_suffix_il(S6<float> &x) {
+  if constexpr (D < 8) {
+    x = x + above6_il<D>(x);
+    scan_suffix_il<2 * D>(x);
+  }
+}
+
 // Shoemake's quaternion with Eigen's branch rule (w first when the trace is
 // positive, else the largest diagonal entry), evaluated without divergence.
 __device__ __forceinline__ void quat_branchless(const float *R, float *q) {
