@@ -31,9 +31,15 @@ __global__ __launch_bounds__(256) void k_chain_vjp(const ModelTable<float> *__re
                                                    const float *__restrict__ qdd, const float *__restrict__ gtu_src,
                                                    VjpArgs<float> A) {
   __shared__ float4 s_tab[kTableWords4];
-  const int g = threadIdx.x & (G - 1);
-  const int base4 = ((threadIdx.x & 63) & ~(G - 1)) * 4;
-  const int64_t env = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
+  // 8 lanes x 1 joint: interleaved pairs (joint j of the row's environment p
+  // on lane 2j + p), the scans are DPP row shifts with edge defaults.
+  constexpr bool IL = G == 8 && C == 1;
+  constexpr int lstep = IL ? 2 : 1;  // lane distance between consecutive joint slots
+  const int lane = threadIdx.x & 63;
+  const int g = IL ? (lane & 15) >> 1 : (int)(threadIdx.x & (G - 1));
+  const int lane0 = IL ? ((lane & ~15) | (lane & 1)) : (lane & ~(G - 1));  // the group's joint slot 0
+  const int64_t env = IL ? (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / 16 * 2 + (lane & 1)
+                         : (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
   const bool live = env < B;
   const int64_t e = live ? env : B - 1;
   const int nj = Mg->nj, nv = Mg->nv, L = Mg->nlinks;
@@ -74,7 +80,7 @@ __global__ __launch_bounds__(256) void k_chain_vjp(const ModelTable<float> *__re
       qddj[t] = qddu[t];
       gj[t] = gu[t];
     } else {
-      const int src = base4 + 4 * (u / C), slot = u % C;
+      const int src = 4 * (lane0 + lstep * (u / C)), slot = u % C;
       qj[t] = qdj[t] = qddj[t] = gj[t] = 0.f;
 #pragma unroll
       for (int s = 0; s < C; ++s) {
@@ -101,10 +107,14 @@ __global__ __launch_bounds__(256) void k_chain_vjp(const ModelTable<float> *__re
   }
   if constexpr (G > 1) {
     Xf incl = X[C - 1];
-    scan_xf<G>(g, incl);
-    const Xf E = excl_xf<G>(g, incl);
+    if constexpr (IL) scan_xf_il(incl);
+    else scan_xf<G>(g, incl);
+    if constexpr (C > 1) {
+      const Xf E = excl_xf<G>(g, incl);
 #pragma unroll
-    for (int t = 0; t < C; ++t) X[t] = compose(E, X[t]);
+      for (int t = 0; t < C - 1; ++t) X[t] = compose(E, X[t]);
+    }
+    X[C - 1] = incl;
   }
   S6<float> col[C];
 #pragma unroll
@@ -144,25 +154,34 @@ __global__ __launch_bounds__(256) void k_chain_vjp(const ModelTable<float> *__re
     for (int t = 0; t < C; ++t) v[t] = (t > 0 ? v[t - 1] : zero6<float>()) + qdj[t] * col[t];
     if constexpr (G > 1) {
       S6<float> incl = v[C - 1];
-      scan_sum<G>(g, incl);
-      const S6<float> E = excl_sum<G>(g, incl);
+      if constexpr (IL) scan_sum_il(incl);
+      else scan_sum<G>(g, incl);
+      if constexpr (C > 1) {
+        const S6<float> E = excl_sum<G>(g, incl);
 #pragma unroll
-      for (int t = 0; t < C; ++t) v[t] = v[t] + E;
+        for (int t = 0; t < C - 1; ++t) v[t] = v[t] + E;
+      }
+      v[C - 1] = incl;
     }
 #pragma unroll
     for (int t = 0; t < C; ++t)
       a[t] = (t > 0 ? a[t - 1] : zero6<float>()) + qddj[t] * col[t] + qdj[t] * crossm(v[t], col[t]);
+    // a[C - 1] takes the inclusive scan, the other slots the exclusive one
     S6<float> Ea = zero6<float>();
     if constexpr (G > 1) {
       S6<float> incl = a[C - 1];
-      scan_sum<G>(g, incl);
-      Ea = excl_sum<G>(g, incl);
+      if constexpr (IL) scan_sum_il(incl);
+      else scan_sum<G>(g, incl);
+      if constexpr (C > 1) Ea = excl_sum<G>(g, incl);
+      a[C - 1] = incl;
     }
-    Ea.l = Ea.l - V3<float>{M->gravity[0], M->gravity[1], M->gravity[2]};
+    const V3<float> grav = {M->gravity[0], M->gravity[1], M->gravity[2]};
+    Ea.l = Ea.l - grav;
     WInertia<float> Y[C];
 #pragma unroll
     for (int t = 0; t < C; ++t) {
-      a[t] = a[t] + Ea;
+      if (G > 1 && t == C - 1) a[t].l = a[t].l - grav;
+      else a[t] = a[t] + Ea;
       Y[t] = world_inertia_fast(*J[t], X[t].R, X[t].p);
       F[t] = valid[t] ? inertia_apply(Y[t], a[t]) + crossf(v[t], inertia_apply(Y[t], v[t])) : zero6<float>();
     }
@@ -170,10 +189,14 @@ __global__ __launch_bounds__(256) void k_chain_vjp(const ModelTable<float> *__re
     for (int t = C - 2; t >= 0; --t) F[t] = F[t] + F[t + 1];
     if constexpr (G > 1) {
       S6<float> incl = F[0];
-      scan_suffix<G>(g, incl);
-      const S6<float> E = excl_suffix<G>(g, incl);
+      if constexpr (IL) scan_suffix_il(incl);
+      else scan_suffix<G>(g, incl);
+      if constexpr (C > 1) {
+        const S6<float> E = excl_suffix<G>(g, incl);
 #pragma unroll
-      for (int t = 0; t < C; ++t) F[t] = F[t] + E;
+        for (int t = 1; t < C; ++t) F[t] = F[t] + E;
+      }
+      F[0] = incl;
     }
     // ---- upstream gradient on tau, then the virtual velocity w
 #pragma unroll
@@ -191,10 +214,14 @@ __global__ __launch_bounds__(256) void k_chain_vjp(const ModelTable<float> *__re
     for (int t = 0; t < C; ++t) w[t] = (t > 0 ? w[t - 1] : zero6<float>()) + gj[t] * col[t];
     if constexpr (G > 1) {
       S6<float> incl = w[C - 1];
-      scan_sum<G>(g, incl);
-      const S6<float> E = excl_sum<G>(g, incl);
+      if constexpr (IL) scan_sum_il(incl);
+      else scan_sum<G>(g, incl);
+      if constexpr (C > 1) {
+        const S6<float> E = excl_sum<G>(g, incl);
 #pragma unroll
-      for (int t = 0; t < C; ++t) w[t] = w[t] + E;
+        for (int t = 0; t < C - 1; ++t) w[t] = w[t] + E;
+      }
+      w[C - 1] = incl;
     }
 #pragma unroll
     for (int t = 0; t < C; ++t) {
@@ -215,14 +242,23 @@ __global__ __launch_bounds__(256) void k_chain_vjp(const ModelTable<float> *__re
     }
     if constexpr (G > 1) {
       S6<float> i1 = Hw[0], i2 = QS[0];
-      scan_suffix<G>(g, i1);
-      scan_suffix<G>(g, i2);
-      const S6<float> E1 = excl_suffix<G>(g, i1), E2 = excl_suffix<G>(g, i2);
-#pragma unroll
-      for (int t = 0; t < C; ++t) {
-        Hw[t] = Hw[t] + E1;
-        QS[t] = QS[t] + E2;
+      if constexpr (IL) {
+        scan_suffix_il(i1);
+        scan_suffix_il(i2);
+      } else {
+        scan_suffix<G>(g, i1);
+        scan_suffix<G>(g, i2);
       }
+      if constexpr (C > 1) {
+        const S6<float> E1 = excl_suffix<G>(g, i1), E2 = excl_suffix<G>(g, i2);
+#pragma unroll
+        for (int t = 1; t < C; ++t) {
+          Hw[t] = Hw[t] + E1;
+          QS[t] = QS[t] + E2;
+        }
+      }
+      Hw[0] = i1;
+      QS[0] = i2;
     }
     // ---- dqd, dqdd and the RNEA part of dq (w, v, a now hold parent values)
 #pragma unroll
@@ -240,10 +276,14 @@ __global__ __launch_bounds__(256) void k_chain_vjp(const ModelTable<float> *__re
   for (int t = C - 2; t >= 0; --t) FP[t] = FP[t] + FP[t + 1];
   if constexpr (G > 1) {
     S6<float> incl = FP[0];
-    scan_suffix<G>(g, incl);
-    const S6<float> E = excl_suffix<G>(g, incl);
+    if constexpr (IL) scan_suffix_il(incl);
+    else scan_suffix<G>(g, incl);
+    if constexpr (C > 1) {
+      const S6<float> E = excl_suffix<G>(g, incl);
 #pragma unroll
-    for (int t = 0; t < C; ++t) FP[t] = FP[t] + E;
+      for (int t = 1; t < C; ++t) FP[t] = FP[t] + E;
+    }
+    FP[0] = incl;
   }
 #pragma unroll
   for (int t = 0; t < C; ++t) {
@@ -253,7 +293,8 @@ __global__ __launch_bounds__(256) void k_chain_vjp(const ModelTable<float> *__re
     if (A.dq) A.dq[e * nv + J[t]->vuser] = dq;
   }
   if (lossmode && A.loss) {
-    scan_suffix1<G>(g, loss);
+    if constexpr (IL) scan_suffix1_il(loss);
+    else scan_suffix1<G>(g, loss);
     if (live && g == 0) A.loss[e] = loss;
   }
 }

@@ -181,6 +181,12 @@ template <int D = 1> __device__ __forceinline__ void scan_suffix_il(S6<float> &x
     scan_suffix_il<2 * D>(x);
   }
 }
+template <int D = 1> __device__ __forceinline__ void scan_suffix1_il(float &x) {  // total on joint 0's lane
+  if constexpr (D < 8) {
+    x = x + dpp_old<0x100 + 2 * D>(0.f, x);
+    scan_suffix1_il<2 * D>(x);
+  }
+}
 
 // Shoemake's quaternion with Eigen's branch rule (w first when the trace is
 // positive, else the largest diagonal entry), evaluated without divergence.

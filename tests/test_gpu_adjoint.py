@@ -143,3 +143,52 @@ def test_autograd_functions():
     (3.0 * loss.sum()).backward()
     _, _, dq, _, _ = m.pose_torque_loss_batched(_t(q), _t(qd), _t(qdd), _t(tau_star), 7, t)
     assert torch.allclose(qt2.grad, 3.0 * dq)
+
+
+def test_chain_vjp_every_shape_with_permuted_orders():
+    """Every lanes-per-joint shape of the chain adjoint kernel (including the
+    interleaved 8 x 1 one) with a permuted user joint and link order and a
+    ragged batch: RNEA VJP against the exact FP64 derivatives, pose VJP and the
+    fused loss against the thread-per-environment kernel."""
+    import sapien_amd as sa
+    from sapien_amd import tune
+
+    rob = S.chain7()
+    rng = np.random.default_rng(1406)
+    jn = [rob.joint_names[i] for i in rng.permutation(len(rob.joint_names))]
+    lperm = list(rng.permutation(len(rob.link_names)))
+    ln = [rob.link_names[i] for i in lperm]
+    ee = lperm.index(rob.ee_link)
+    m = sa.PinocchioModel(rob.urdf, [0.0, 0.0, -9.81])
+    m.set_joint_order(jn)
+    m.set_link_order(ln)
+    o = OracleModel(rob.urdf)
+    o.set_joint_order(jn)
+    o.set_link_order(ln)
+    B = 300
+    q, qd, qdd, tau_star = S.samples(rob, B, 1407)
+    g = rng.normal(size=q.shape)
+    gP = rng.normal(size=(B, rob.nlinks, 7))
+    t = _t(o.link_poses(S.samples(rob, B, 1408)[0])[:, ee, :3])
+    Dq, Dqd, Dqdd = o.rnea_derivs(q, qd, qdd)
+    refs = [np.einsum("bi,bik->bk", g, D) for D in (Dq, Dqd, Dqdd)]
+    sa.set_kernel_variant(m, "thread_per_env")
+    ref_pose = m.kinematics_vjp_batched(_t(q), grad_poses=_t(gP))[0].cpu().numpy()
+    ref_loss = [x.cpu().numpy() for x in m.pose_torque_loss_batched(_t(q), _t(qd), _t(qdd), _t(tau_star), ee, t)]
+    sa.set_kernel_variant(m, "lanes_per_joint")
+    try:
+        for G, C in tune.shapes():
+            if G * C < rob.nv:
+                continue
+            tune.set_shape(m, 1 << 40, G, C)
+            got = m.kinematics_vjp_batched(_t(q), _t(qd), _t(qdd), grad_tau=_t(g))
+            for x, ref in zip(got, refs):
+                assert rel_err(x.cpu().numpy(), ref) <= GRAD_TOL, (G, C)
+            dq = m.kinematics_vjp_batched(_t(q), grad_poses=_t(gP))[0].cpu().numpy()
+            assert rel_err(dq, ref_pose) <= GRAD_TOL, (G, C)
+            res = m.pose_torque_loss_batched(_t(q), _t(qd), _t(qdd), _t(tau_star), ee, t)
+            assert np.allclose(res[0].cpu().numpy(), ref_loss[0], rtol=1e-4), (G, C)
+            for x, ref in zip(res[1:], ref_loss[1:]):
+                assert rel_err(x.cpu().numpy(), ref) <= GRAD_TOL, (G, C)
+    finally:
+        tune.set_shape(m, 0, 0, 0)
