@@ -526,6 +526,7 @@ def main():
     from sapien_amd.native import launcher_pose_torque_loss
 
     grads = {}
+    mixed = {}  # per tree: (forward, forward + backward) over the first 4096 environments
     for gname in ("chain7", "arm9"):
         grob = S.robot(gname)
         gm = m if gname == "chain7" else sa.PinocchioModel(grob.urdf, [0.0, 0.0, -9.81])
@@ -542,14 +543,16 @@ def main():
         fwd = sa.native_fk_jac_rnea(gm)
         fb = launcher_pose_torque_loss(gm)
         if sa.kernel_variant(gm) == "lanes_per_joint":
-            tune.autotune(gm, batch_sizes=(GB,), use_cache=False)
+            tune.autotune(gm, batch_sizes=(GB // 2, GB), use_cache=False)
 
-        def run_f():
-            fwd(GB, gq, gqd, gqdd, bufs["P"], None, bufs["T"], stream.cuda_stream)
+        def run_f(nb=GB, fwd=fwd, gq=gq, gqd=gqd, gqdd=gqdd, bufs=bufs):
+            fwd(nb, gq, gqd, gqdd, bufs["P"], None, bufs["T"], stream.cuda_stream)
 
-        def run_fb():
-            fb(GB, gq, gqd, gqdd, gts, grob.ee_link, gt, bufs["loss"], None, bufs["dq"], bufs["dqd"], bufs["dqdd"],
+        def run_fb(nb=GB, fb=fb, gq=gq, gqd=gqd, gqdd=gqdd, gts=gts, ee=grob.ee_link, gt=gt, bufs=bufs):
+            fb(nb, gq, gqd, gqdd, gts, ee, gt, bufs["loss"], None, bufs["dq"], bufs["dqd"], bufs["dqdd"],
                stream.cuda_stream)
+
+        mixed[gname] = (lambda f=run_f: f(GB // 2), lambda f=run_fb: f(GB // 2))
 
         res = {}
         for key, fn in (("forward_evals_s", run_f), ("fwd_bwd_evals_s", run_fb)):
@@ -564,6 +567,23 @@ def main():
         res["ratio"] = res["fwd_bwd_evals_s"] / res["forward_evals_s"]
         res["kernel"] = sa.kernel_variant(gm)
         grads[gname] = res
+    # one batch of 4096 + 4096 environments of the two trees: a forward (or a
+    # loss + gradient) step launches both trees' kernels back to back
+    res = {"envs": {"chain7": 4096, "arm9": 4096}}
+    for key, idx in (("forward_evals_s", 0), ("fwd_bwd_evals_s", 1)):
+        fns = [mixed[k][idx] for k in ("chain7", "arm9")]
+        for _ in range(20):
+            for fn in fns:
+                fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(500):
+            for fn in fns:
+                fn()
+        torch.cuda.synchronize()
+        res[key] = 8192 * 500 / (time.perf_counter() - t0)
+    res["ratio"] = res["fwd_bwd_evals_s"] / res["forward_evals_s"]
+    grads["mixed_chain7_arm9"] = res
     out["gradients"] = grads
 
     if not args.no_workloads:

@@ -35,7 +35,10 @@ using namespace lanes;
 
 constexpr int kClikMaxChain = 16;
 
-// The chain joints' constants the loop reads, staged in LDS per segment.
+// The chain joints' constants the loop reads, staged in LDS per segment. The
+// staged form is branch-free: Bq = Cq = 0 for prismatic joints, ap = 0 for the
+// others, and slot kClikMaxChain is an identity record (axis 0) that every
+// lane past the chain end reads.
 struct ChainRec {
   float A[9], Bq[9], Cq[9];
   float p[3], ap[3], axis[3];
@@ -65,18 +68,18 @@ template <int G> __device__ __forceinline__ double group_allsum(double x) {
 
 __device__ __forceinline__ int bperm_i(int src4, int x) { return __builtin_amdgcn_ds_bpermute(src4, x); }
 
-// Local transform of a chain joint (selects, not branches: no divergence
-// inside a group before the scans).
-__device__ __forceinline__ Xf local_xf(const ChainRec &J, float q, bool act) {
+// Local transform of a chain joint from its staged record (no branches and no
+// selects: the record's zero entries drop the other joint kind's terms, and
+// the identity record gives the identity for q = 0).
+__device__ __forceinline__ Xf local_xf(const ChainRec &J, float q) {
   float s, c;
   joint_sincos(q, &s, &c);
-  const bool pri = J.kind == KIND_PRISMATIC;
-  const float sr = pri ? 0.f : s, omc = pri ? 0.f : 1.f - c, qp = pri ? q : 0.f;
+  const float omc = 1.f - c;
   Xf X;
 #pragma unroll
-  for (int k = 0; k < 9; ++k) X.R[k] = J.A[k] + sr * J.Bq[k] + omc * J.Cq[k];
-  X.p = load3(J.p) + qp * load3(J.ap);
-  return act ? X : identity_xf();
+  for (int k = 0; k < 9; ++k) X.R[k] = J.A[k] + s * J.Bq[k] + omc * J.Cq[k];
+  X.p = load3(J.p) + q * load3(J.ap);
+  return X;
 }
 
 // Per-group problem state (the same in every lane of the group, except the
@@ -94,24 +97,36 @@ enum : int { kIdle = 0, kRunning = 1, kRetired = 2 };
 
 template <int G, int C> __global__ __launch_bounds__(256) void k_clik_lanes(HeteroClikArgs A) {
   __shared__ __align__(16) int s_raw[sizeof(ClikSegment) * kMaxSegments / 4];
-  __shared__ ChainRec s_chain[kMaxSegments][kClikMaxChain];
+  __shared__ ChainRec s_chain[kMaxSegments][kClikMaxChain + 1];
   {
     const int *src = reinterpret_cast<const int *>(A.seg);
     for (int i = threadIdx.x; i < (int)(sizeof(s_raw) / 4); i += blockDim.x) s_raw[i] = src[i];
-    for (int i = threadIdx.x; i < A.nseg * kClikMaxChain; i += blockDim.x) {
-      const ClikSegment &S = A.seg[i / kClikMaxChain];
-      const int k = i % kClikMaxChain;
+    constexpr int kSlots = kClikMaxChain + 1;
+    for (int i = threadIdx.x; i < A.nseg * kSlots; i += blockDim.x) {
+      const ClikSegment &S = A.seg[i / kSlots];
+      const int k = i % kSlots;
+      ChainRec &R = s_chain[i / kSlots][k];
+      if (k == kClikMaxChain) {  // identity record
+        for (int t = 0; t < 9; ++t) {
+          R.A[t] = (t % 4 == 0) ? 1.f : 0.f;
+          R.Bq[t] = R.Cq[t] = 0.f;
+        }
+        for (int t = 0; t < 3; ++t) R.p[t] = R.ap[t] = R.axis[t] = 0.f;
+        R.kind = KIND_REVOLUTE;
+        R.vuser = 0;
+        continue;
+      }
       if (k >= S.P.nchain) continue;
       const JointRec<float> &J = joints_of(S.tab)[S.P.chain[k]];
-      ChainRec &R = s_chain[i / kClikMaxChain][k];
+      const bool pri = J.kind == KIND_PRISMATIC;
       for (int t = 0; t < 9; ++t) {
         R.A[t] = J.A[t];
-        R.Bq[t] = J.Bq[t];
-        R.Cq[t] = J.Cq[t];
+        R.Bq[t] = pri ? 0.f : J.Bq[t];
+        R.Cq[t] = pri ? 0.f : J.Cq[t];
       }
       for (int t = 0; t < 3; ++t) {
         R.p[t] = J.p[t];
-        R.ap[t] = J.ap[t];
+        R.ap[t] = pri ? J.ap[t] : 0.f;
         R.axis[t] = J.axis[t];
       }
       R.kind = J.kind;
@@ -224,7 +239,7 @@ template <int G, int C> __global__ __launch_bounds__(256) void k_clik_lanes(Hete
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const int j = g * C + c;
-      W[c] = local_xf(chain[j < st.nc ? j : 0], st.q[c], j < st.nc);
+      W[c] = local_xf(chain[j < st.nc ? j : kClikMaxChain], st.q[c]);  // q = 0 past the chain end
       part = c == 0 ? W[0] : compose(part, W[c]);
     }
     Xf incl = part;
@@ -262,10 +277,9 @@ template <int G, int C> __global__ __launch_bounds__(256) void k_clik_lanes(Hete
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const int j = g * C + c;
-      const ChainRec &J = chain[j < st.nc ? j : 0];
+      const ChainRec &J = chain[j < st.nc ? j : kClikMaxChain];  // axis 0 past the chain end: column 0
       const V3<float> aw = mv(W[c].R, load3(J.axis));
-      S6<float> cw = J.kind == KIND_PRISMATIC ? S6<float>{aw, {0.f, 0.f, 0.f}} : S6<float>{cross(W[c].p, aw), aw};
-      if (j >= st.nc) cw = zero6<float>();
+      const S6<float> cw = J.kind == KIND_PRISMATIC ? S6<float>{aw, {0.f, 0.f, 0.f}} : S6<float>{cross(W[c].p, aw), aw};
       float cv[6];
       vec_of(to_local(E.R, E.p, cw), cv);
 #pragma unroll

@@ -62,10 +62,11 @@ template <class T> __device__ __forceinline__ void log_coeffs_dev(T t, T &alpha,
   } else {
     T st, ct;
     sincos_t(t, &st, &ct);
-    T omc = T(1) - ct;
-    alpha = t * st / (T(2) * omc);
-    beta = T(1) / t2 - st / (T(2) * t * omc);
-    dbeta = T(-2) / (t2 * t2) + (T(1) + st / t) / (T(2) * t2 * omc);
+    // two reciprocals (1/t, 1/(1 - cos t)) instead of one division per term
+    const T r = T(1) / t, u = T(1) / (T(1) - ct), r2 = r * r, hsu = T(0.5) * st * u;
+    alpha = t * hsu;
+    beta = r2 - r * hsu;
+    dbeta = T(-2) * r2 * r2 + (T(1) + st * r) * (T(0.5) * u) * r2;
   }
 }
 

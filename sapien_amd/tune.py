@@ -64,8 +64,11 @@ def _save_cache(d):
 def autotune(model, batch_sizes=(16384,), iters=100, use_cache=True, report=None):
     """Pick and install the fastest chain-kernel shape for each batch size.
 
-    Returns {B: (G, C, microseconds)}. Models that are not serial chains of
-    <= 16 joints (where the thread-per-environment kernels run) return {}.
+    Each candidate is timed with HIP events over launches that rotate through
+    enough input/output sets to miss the Infinity Cache, three rounds over all
+    candidates, best round each. Returns {B: (G, C, microseconds)}. Models that
+    are not serial chains of <= 16 joints (where the thread-per-environment
+    kernels run) return {}.
     """
     import torch
 
@@ -84,37 +87,44 @@ def autotune(model, batch_sizes=(16384,), iters=100, use_cache=True, report=None
     lo, hi = model.joint_limits()
     out = {}
     for B in sorted(batch_sizes):
-        key = f"{gpu}|chain{nj}|L{L}|B{B}"
+        key = f"{gpu}|chain{nj}|L{L}|B{B}|rotated"
         if key in cache:
             G, C, us = cache[key]
             set_shape(model, B, G, C)
             out[B] = (G, C, us)
             continue
+        # input/output sets rotated so the working set (>= 512 MB) exceeds the
+        # 256 MB Infinity Cache, as in a stream of fresh batches
+        per_env = 4 * (3 * nj + 7 * L + 6 * nj + nj)
+        nsets = max(2, min(64, -(-(512 << 20) // (per_env * B))))
         gen = torch.Generator(device="cpu").manual_seed(B)
-        q = (torch.rand(B, nj, generator=gen, dtype=torch.float64) * torch.tensor(hi - lo) + torch.tensor(lo)).float()
-        q = q.to(dev)
-        qd = torch.randn(B, nj, generator=gen).to(dev)
-        qdd = torch.randn(B, nj, generator=gen).to(dev)
-        P = torch.empty(B, L, 7, device=dev)
-        J = torch.empty(B, 6, nj, device=dev)
-        T = torch.empty(B, nj, device=dev)
-        timings = []
-        for G, C in shapes():
-            if G * C < nj or (G * C) // 2 >= nj and G * C > 8:
-                continue  # does not cover the chain, or wastes more than half of the lanes
-            set_shape(model, B, G, C)
-            for _ in range(10):
-                launch(B, q, qd, qdd, P, J, T, stream.cuda_stream)
-            a = torch.cuda.Event(enable_timing=True)
-            b = torch.cuda.Event(enable_timing=True)
-            a.record(stream)
-            for _ in range(iters):
-                launch(B, q, qd, qdd, P, J, T, stream.cuda_stream)
-            b.record(stream)
-            b.synchronize()
-            timings.append((a.elapsed_time(b) / iters * 1e3, G, C))
-            if report is not None:
-                report.append({"B": B, "G": G, "C": C, "us": timings[-1][0]})
+        sets = []
+        for _ in range(nsets):
+            q = (torch.rand(B, nj, generator=gen, dtype=torch.float64) * torch.tensor(hi - lo) + torch.tensor(lo))
+            sets.append((q.float().to(dev), torch.randn(B, nj, generator=gen).to(dev),
+                         torch.randn(B, nj, generator=gen).to(dev), torch.empty(B, L, 7, device=dev),
+                         torch.empty(B, 6, nj, device=dev), torch.empty(B, nj, device=dev)))
+        cands = [(G, C) for G, C in shapes()
+                 if not (G * C < nj or (G * C) // 2 >= nj and G * C > 8)]  # covers the chain, wastes < half the lanes
+        best = {}
+        for _ in range(3):  # rounds over all candidates; the best round of each counts
+            for G, C in cands:
+                set_shape(model, B, G, C)
+                for i in range(10):
+                    launch(B, *sets[i % nsets], stream.cuda_stream)
+                a = torch.cuda.Event(enable_timing=True)
+                b = torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+                for i in range(iters):
+                    launch(B, *sets[i % nsets], stream.cuda_stream)
+                b.record(stream)
+                b.synchronize()
+                t = a.elapsed_time(b) / iters * 1e3
+                best[(G, C)] = min(best.get((G, C), t), t)
+        del sets
+        timings = [(t, G, C) for (G, C), t in best.items()]
+        if report is not None:
+            report.extend({"B": B, "G": G, "C": C, "us": t} for t, G, C in timings)
         us, G, C = min(timings)
         set_shape(model, B, G, C)
         out[B] = (G, C, us)
