@@ -120,7 +120,9 @@ struct sapamd_model {
   void *tab32 = nullptr, *tab64 = nullptr;
   int32_t *d_chain = nullptr;   // every link's support chain, concatenated
   std::vector<int> chain_off, chain_len;
-  int32_t *d_counter = nullptr;
+  int32_t *d_counter = nullptr;  // CLIK [0] work queue, [1], [3] hand-over counts, [2] second-phase queue
+  ClikCont *d_cont = nullptr;    // CLIK hand-over records (grown on demand)
+  int64_t cont_cap = 0;
   hipStream_t stream = nullptr;  // the device's shared per-call stream (not owned)
   double *h_buf = nullptr, *d_buf = nullptr;
   size_t buf_doubles = 0;
@@ -149,12 +151,32 @@ struct sapamd_model {
     if (tab64) (void)hipFree(tab64);
     if (d_chain) (void)hipFree(d_chain);
     if (d_counter) (void)hipFree(d_counter);
+    if (d_cont) (void)hipFree(d_cont);
     if (d_buf) (void)hipFree(d_buf);
     if (h_buf) (void)hipHostFree(h_buf);
   }
 
   int nv() const { return host->nv(); }
   int L() const { return host->nlinks(); }
+
+  // The lanes-per-joint CLIK's queue counters and hand-over records for n
+  // problems; overlapping CLIK launches on one model race on them.
+  void clik_workspace(HeteroClikArgs &A, int64_t n, hipStream_t s) {
+    if (n > cont_cap) {
+      DeviceGuard g(device);
+      if (d_cont) HIP_OK(hipFree(d_cont));
+      d_cont = nullptr;
+      cont_cap = 0;
+      HIP_OK(hipMalloc(&d_cont, sizeof(ClikCont) * n));
+      cont_cap = n;
+    }
+    A.counter = d_counter;
+    A.cont_count = d_counter + 1;
+    A.cont_queue = d_counter + 2;
+    A.cont_easy = d_counter + 3;
+    A.cont = d_cont;
+    HIP_OK(launch_fill_i32(d_counter, 0, 4, s));
+  }
 
   template <class T> std::vector<char> build_table() const {
     const auto &J = host->joints();
@@ -1028,14 +1050,6 @@ int sapamd_hetero_batched(const sapamd_model *const *models, int32_t nmodels, co
   });
 }
 
-namespace {
-// Persistent blocks of the lanes-per-joint CLIK: two 256-thread blocks per CU
-// (two wavefronts per SIMD at its register count), fewer for small batches.
-int clik_lanes_blocks(int64_t total, int max_chain, int cus) {
-  const int per_block = 256 / clik_lanes_group(max_chain);
-  return (int)std::max<int64_t>(1, std::min<int64_t>(2 * cus, (total + per_block - 1) / per_block));
-}
-}  // namespace
 
 int sapamd_clik_hetero_batched(const sapamd_model *const *models, int32_t nmodels, const int64_t *B,
                                const int32_t *links, const float *const *targets, const float *const *q0,
@@ -1088,18 +1102,20 @@ int sapamd_clik_hetero_batched(const sapamd_model *const *models, int32_t nmodel
       max_chain = std::max(max_chain, g.P.nchain);
     }
     if (A.total == 0) return;
-    // the queue counter of the first model: overlapping launches on it race
+    // the queue counter (and hand-over records) of the first model:
+    // overlapping launches on it race
     A.counter = models[0]->d_counter;
     int cus = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, models[0]->device) == hipSuccess) cus = prop.multiProcessorCount;
-    HIP_OK(launch_fill_i32(A.counter, 0, 1, S(stream)));
     bool lanes = clik_lanes_group(max_chain) != 0 && A.total <= INT32_MAX;
     for (int i = 0; i < nmodels; ++i) lanes = lanes && models[i]->variant != SAPAMD_KERNEL_THREAD_PER_ENV;
     if (lanes) {
-      HIP_OK(launch_clik_lanes(A, max_chain, clik_lanes_blocks(A.total, max_chain, cus), S(stream)));
+      const_cast<sapamd_model *>(models[0])->clik_workspace(A, A.total, S(stream));
+      HIP_OK(launch_clik_lanes(A, max_chain, cus, S(stream)));
       return;
     }
+    HIP_OK(launch_fill_i32(A.counter, 0, 1, S(stream)));
     const int64_t waves = std::min<int64_t>(4 * cus, (A.total + 63) / 64);
     HIP_OK(launch_clik_hetero(A, max_chain, (int)waves, S(stream)));
   });
@@ -1186,10 +1202,9 @@ void clik_launch(const sapamd_model *m, int64_t B, int64_t nstart, int32_t link,
     g.first = 0;
     A.nseg = 1;
     A.total = total;
-    A.counter = m->d_counter;
-    HIP_OK(launch_fill_i32(m->d_counter, 0, 1, S(stream)));
+    const_cast<sapamd_model *>(m)->clik_workspace(A, total, S(stream));
     if (solved) HIP_OK(launch_fill_i32(solved, INT32_MAX, B, S(stream)));
-    HIP_OK(launch_clik_lanes(A, P.nchain, clik_lanes_blocks(total, P.nchain, dev_cus), S(stream)));
+    HIP_OK(launch_clik_lanes(A, P.nchain, dev_cus, S(stream)));
     return;
   }
   const int64_t waves = (total + 63) / 64;

@@ -91,11 +91,15 @@ template <int C> struct ClikState {
   V3<float> pd;
   float q[C], qbest[C], msk[C];
   float minErr, bestErr[6];
+  float midErr;  // error at iteration cap / 2 (first phase)
 };
 
 enum : int { kIdle = 0, kRunning = 1, kRetired = 2 };
 
-template <int G, int C> __global__ __launch_bounds__(256) void k_clik_lanes(HeteroClikArgs A) {
+// CONT = false: the first phase (problems from A.counter in index order; those
+// still running after A.cap iterations are handed to A.cont). CONT = true: the
+// second phase (the handed-over problems, from A.cont_queue).
+template <int G, int C, bool CONT> __global__ __launch_bounds__(256) void k_clik_lanes(HeteroClikArgs A) {
   __shared__ __align__(16) int s_raw[sizeof(ClikSegment) * kMaxSegments / 4];
   __shared__ ChainRec s_chain[kMaxSegments][kClikMaxChain + 1];
   {
@@ -153,21 +157,30 @@ template <int G, int C> __global__ __launch_bounds__(256) void k_clik_lanes(Hete
 #pragma unroll
   for (int c = 0; c < C; ++c) st.q[c] = st.qbest[c] = st.msk[c] = 0.f;
   st.minErr = 1e10f;
+  st.midErr = 0.f;
 #pragma unroll
   for (int r = 0; r < 6; ++r) st.bestErr[r] = 0.f;
   int state = kIdle;
+  // problems in this phase's queue (the second phase reads the first's counts)
+  const int64_t nhard =
+      CONT ? (int64_t)__hip_atomic_load(A.cont_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+  const int64_t total =
+      CONT ? nhard + __hip_atomic_load(A.cont_easy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : A.total;
 
   for (;;) {
     // ---- refill: the groups without a problem take the next indices
     const uint64_t want = __ballot(state == kIdle && g == 0);
     if (want) {  // uniform
       const int nwant = __popcll(want);
-      const int old = atomicAdd(A.counter, lane == 0 ? nwant : 0);
+      const int old = atomicAdd(CONT ? A.cont_queue : A.counter, lane == 0 ? nwant : 0);
       const int first = __builtin_amdgcn_readlane(old, 0);
       const int64_t idx = (int64_t)first + __popcll(want & below);
-      // multi-start skip test: every lane loads (from a valid address in any
-      // case) and the group takes its first lane's reading
-      const int64_t ic = idx < A.total ? idx : 0;
+      // second phase: the front records first, then the back ones
+      const ClikCont *rec =
+          CONT ? A.cont + (idx < nhard ? idx : idx < total ? A.total - 1 - (idx - nhard) : 0) : nullptr;
+      // multi-start skip test (first phase): every lane loads (from a valid
+      // address in any case) and the group takes its first lane's reading
+      const int64_t ic = CONT ? (idx < total ? rec->prob : 0) : (idx < total ? idx : 0);
       int s = 0;
       while (s + 1 < A.nseg && ic >= s_seg[s + 1].first) ++s;
       const ClikSegment &S = s_seg[s];
@@ -175,13 +188,15 @@ template <int G, int C> __global__ __launch_bounds__(256) void k_clik_lanes(Hete
       const int64_t prob = ic - S.first;
       const int64_t tgt = P.ntargets ? prob % P.ntargets : prob;
       const int32_t start = P.ntargets ? (int32_t)(prob / P.ntargets) : 0;
-      int lowest =
-          __hip_atomic_load(P.solved ? P.solved + tgt : A.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      lowest = bperm_i(base4, lowest);
+      int lowest = INT32_MAX;
+      if constexpr (!CONT) {
+        lowest = __hip_atomic_load(P.solved ? P.solved + tgt : A.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        lowest = bperm_i(base4, lowest);
+      }
       if (state == kIdle) {  // per group; no cross-lane operations inside
-        if (idx >= A.total) {
+        if (idx >= total) {
           state = kRetired;
-        } else if (P.solved && lowest < start) {
+        } else if (!CONT && P.solved && lowest < start) {
           // a lower start already converged for this target: skip
           const int nv = S.tab->nv;
           float *qo = S.q_out + prob * nv;
@@ -199,7 +214,7 @@ template <int G, int C> __global__ __launch_bounds__(256) void k_clik_lanes(Hete
           st.prob = prob;
           st.seg = s;
           st.nc = P.nchain;
-          st.it = 0;
+          st.it = CONT ? rec->it : 0;
           const LinkRec<float> &Lk = links_of(M)[P.link];
           float Rt[9];
           V3<float> pt;
@@ -217,13 +232,19 @@ template <int G, int C> __global__ __launch_bounds__(256) void k_clik_lanes(Hete
             const int j = g * C + c;
             const bool act = j < st.nc;
             const int vu = s_chain[s][act ? j : 0].vuser;
-            st.q[c] = act && q0 ? q0[vu] : 0.f;
-            st.qbest[c] = st.q[c];
+            if constexpr (CONT) {
+              st.q[c] = act ? rec->q[j] : 0.f;
+              st.qbest[c] = act ? rec->qbest[j] : 0.f;
+            } else {
+              st.q[c] = act && q0 ? q0[vu] : 0.f;
+              st.qbest[c] = st.q[c];
+            }
             st.msk[c] = act ? (P.mask ? float(P.mask[prob * P.mask_ld + vu]) : 1.f) : 0.f;
           }
-          st.minErr = 1e10f;
+          st.minErr = CONT ? rec->minErr : 1e10f;
+          st.midErr = 0.f;
 #pragma unroll
-          for (int r = 0; r < 6; ++r) st.bestErr[r] = 0.f;
+          for (int r = 0; r < 6; ++r) st.bestErr[r] = CONT ? rec->bestErr[r] : 0.f;
           state = kRunning;
         }
       }
@@ -260,9 +281,9 @@ template <int G, int C> __global__ __launch_bounds__(256) void k_clik_lanes(Hete
       for (int j = 0; j < 3; ++j)
         Ri[3 * i + j] = E.R[i] * st.Rd[j] + E.R[3 + i] * st.Rd[3 + j] + E.R[6 + i] * st.Rd[6 + j];
     const V3<float> pi = mtv(E.R, st.pd - E.p);
-    // (with Jlog6(iMd^-1) for the step from the same rotation log)
-    float Jl[36];
-    const S6<float> e = log6_jlog6_inv(Ri, pi, Jl);
+    // (with Jlog6(iMd^-1) for the step from the same rotation log, as its blocks)
+    float Ja[9], Jc[9];
+    const S6<float> e = log6_jlog6_inv_blocks(Ri, pi, Ja, Jc);
     float ev[6];
     vec_of(e, ev);
     const float en = sqrtf(dot6(e, e));
@@ -280,15 +301,11 @@ template <int G, int C> __global__ __launch_bounds__(256) void k_clik_lanes(Hete
       const ChainRec &J = chain[j < st.nc ? j : kClikMaxChain];  // axis 0 past the chain end: column 0
       const V3<float> aw = mv(W[c].R, load3(J.axis));
       const S6<float> cw = J.kind == KIND_PRISMATIC ? S6<float>{aw, {0.f, 0.f, 0.f}} : S6<float>{cross(W[c].p, aw), aw};
-      float cv[6];
-      vec_of(to_local(E.R, E.p, cw), cv);
+      const S6<float> cl = to_local(E.R, E.p, cw);
+      const V3<float> u = mv(Ja, cl.a), top = mv(Ja, cl.l) + mv(Jc, u);
+      const float jv[6] = {top.x, top.y, top.z, u.x, u.y, u.z};
 #pragma unroll
-      for (int r = 0; r < 6; ++r) {
-        float acc = 0.f;
-#pragma unroll
-        for (int m = 0; m < 6; ++m) acc += Jl[r * 6 + m] * cv[m];
-        Jk[c][r] = -acc * st.msk[c];
-      }
+      for (int r = 0; r < 6; ++r) Jk[c][r] = -jv[r] * st.msk[c];
       int t = 0;
 #pragma unroll
       for (int r = 0; r < 6; ++r)
@@ -312,7 +329,34 @@ template <int G, int C> __global__ __launch_bounds__(256) void k_clik_lanes(Hete
     }
     ldlt6_solve(JJt, y);
 
+    // first phase: a problem that takes another step and reaches A.cap
+    // iterations with it is handed over (slots taken like the refill, in code
+    // every lane runs)
+    bool cont = false;
+    int64_t slot = 0;
+    if constexpr (!CONT) {
+      const bool fin = en < float(P.eps) || st.it >= P.max_iter;
+      cont = state == kRunning && !fin && st.it + 1 == A.cap;
+      // A problem on its way converges geometrically (about 0.9 per iteration
+      // at dt = 0.1, 0.04 over the second half of the phase); one whose error
+      // fell by less than 5x over that half is likely stuck, runs to the
+      // iteration limit, and goes to the front.
+      const bool far = en > 0.2f * st.midErr;
+      const uint64_t wh = __ballot(cont && far && g == 0), we = __ballot(cont && !far && g == 0);
+      int base_h = 0, base_e = 0;  // read from lane 0 with every lane active
+      if (wh) {                    // uniform
+        const int old = atomicAdd(A.cont_count, lane == 0 ? __popcll(wh) : 0);
+        base_h = __builtin_amdgcn_readlane(old, 0);
+      }
+      if (we) {  // uniform
+        const int old = atomicAdd(A.cont_easy, lane == 0 ? __popcll(we) : 0);
+        base_e = __builtin_amdgcn_readlane(old, 0);
+      }
+      slot = far ? (int64_t)base_h + __popcll(wh & below) : A.total - 1 - ((int64_t)base_e + __popcll(we & below));
+    }
+
     if (state == kRunning) {  // per group; no cross-lane operations inside
+      if (!CONT && st.it == A.cap / 2) st.midErr = en;
       if (en < st.minErr) {
         st.minErr = en;
 #pragma unroll
@@ -365,6 +409,22 @@ template <int G, int C> __global__ __launch_bounds__(256) void k_clik_lanes(Hete
           st.q[c] += v * float(P.dt);
         }
         ++st.it;
+        if (cont && slot >= 0 && slot < A.total) {  // always in range: at most A.total hand-overs
+          ClikCont &R = A.cont[slot];
+#pragma unroll
+          for (int c = 0; c < C; ++c) {
+            R.q[g * C + c] = st.q[c];
+            R.qbest[g * C + c] = st.qbest[c];
+          }
+          if (g == 0) {
+            R.prob = S.first + st.prob;
+            R.it = st.it;
+            R.minErr = st.minErr;
+#pragma unroll
+            for (int r = 0; r < 6; ++r) R.bestErr[r] = st.bestErr[r];
+          }
+          state = kIdle;
+        }
       }
     }
   }
@@ -374,12 +434,26 @@ template <int G, int C> __global__ __launch_bounds__(256) void k_clik_lanes(Hete
 
 int clik_lanes_group(int max_chain) { return max_chain <= kClikMaxChain ? 4 : 0; }
 
-hipError_t launch_clik_lanes(const HeteroClikArgs &A, int max_chain, int blocks, hipStream_t s) {
+hipError_t launch_clik_lanes(const HeteroClikArgs &A, int max_chain, int cus, hipStream_t s) {
   if (A.total <= 0) return hipSuccess;
-  if (!clik_lanes_group(max_chain) || A.total > INT32_MAX) return hipErrorNotSupported;
-  // one shape for every chain length, so a problem gives the same bits in a
-  // per-model launch and in a mixed-tree launch
-  hipLaunchKernelGGL((k_clik_lanes<4, 4>), dim3((unsigned)blocks), dim3(256), 0, s, A);
+  if (!clik_lanes_group(max_chain) || A.total > INT32_MAX || !A.cont || !A.cont_count || !A.cont_queue ||
+      !A.cont_easy)
+    return hipErrorNotSupported;
+  // The same two shapes for every chain length, so a problem gives the same
+  // bits in a per-model launch and in a mixed-tree launch. The first phase
+  // packs 16 problems per wavefront (throughput); the problems left after
+  // kClikPhase1Iters iterations, the slow and the unreachable ones, run 8 per
+  // wavefront with half the joints per lane (a shorter iteration).
+  // Persistent grids, fewer blocks for small batches: the first phase two
+  // blocks per CU (two wavefronts per SIMD), the second one block per CU, so
+  // the long problems it starts first keep a SIMD each.
+  HeteroClikArgs A1 = A;
+  A1.cap = kClikPhase1Iters;
+  const auto grid = [&](int per_block, int per_cu) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(per_cu * cus, (A.total + per_block - 1) / per_block));
+  };
+  hipLaunchKernelGGL((k_clik_lanes<4, 4, false>), dim3(grid(256 / 4, 2)), dim3(256), 0, s, A1);
+  hipLaunchKernelGGL((k_clik_lanes<8, 2, true>), dim3(grid(256 / 8, 1)), dim3(256), 0, s, A1);
   return hipGetLastError();
 }
 

@@ -110,20 +110,45 @@ struct ClikSegment {
   int32_t *success, *iters;
   int64_t B, first;
 };
+// State of a problem handed from the first CLIK phase to the second: the
+// iterate about to be evaluated, the iteration count and the best iterate so
+// far, per chain joint.
+struct ClikCont {
+  int64_t prob;  // problem index over the launch's segments
+  int32_t it;
+  float minErr;
+  float bestErr[6];
+  float q[16], qbest[16];
+};
 struct HeteroClikArgs {
   ClikSegment seg[kMaxSegments];
   int32_t nseg;
   int64_t total;
   int32_t *counter;
+  // Two-phase lanes-per-joint CLIK: problems still running after `cap`
+  // iterations go to `cont` and finish in a second launch that takes them
+  // from cont_queue. Those still far from their target (likely to run to the
+  // iteration limit) fill `cont` from the front (count in cont_count), the
+  // others from the back (count in cont_easy), so the second launch starts
+  // the long ones first. cap = 0: one phase.
+  ClikCont *cont = nullptr;
+  int32_t *cont_count = nullptr, *cont_queue = nullptr, *cont_easy = nullptr;
+  int32_t cap = 0;
 };
 hipError_t launch_clik_hetero(const HeteroClikArgs &A, int max_chain, int waves, hipStream_t s);
-// Lanes-per-joint CLIK (FP32, chains of <= 16 joints): one problem per group of
-// 4 lanes x 4 chain joints, persistent blocks of 256 threads pulling problems
-// from A.counter (zeroed before the launch). Serves the per-model batched and
-// the mixed-tree launches; clik_lanes_group returns the group size, 0 above
-// 16 chain joints.
+// Lanes-per-joint CLIK (FP32, chains of <= 16 joints) in two phases:
+// persistent blocks of 256 threads pulling problems from A.counter, first one
+// problem per 4 lanes x 4 chain joints for at most kClikPhase1Iters
+// iterations, then the problems still running one per 8 lanes x 2 joints
+// (shorter iterations for the long tail). A.counter, A.cont_count,
+// A.cont_queue and A.cont_easy are zeroed before the launch; A.cont holds
+// A.total records.
+// Serves the per-model batched and the mixed-tree launches;
+// clik_lanes_group returns the first phase's group size, 0 above 16 chain
+// joints.
+constexpr int kClikPhase1Iters = 128;
 int clik_lanes_group(int max_chain);
-hipError_t launch_clik_lanes(const HeteroClikArgs &A, int max_chain, int blocks, hipStream_t s);
+hipError_t launch_clik_lanes(const HeteroClikArgs &A, int max_chain, int cus, hipStream_t s);
 
 // Lanes-per-joint fused FK + Jacobian + RNEA for serial chains of <= 16 joints
 // (FP32), G lanes per environment x C joints per lane. Returns

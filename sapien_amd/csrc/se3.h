@@ -81,10 +81,10 @@ __device__ __forceinline__ S6<T> log6_dev(const T *R, V3<T> p) {
 
 // Jlog6 (6x6 row-major): [[A, C A], [0, A]], A = alpha I + beta w w^T + [w]x / 2,
 // C = dbeta (w.p) w w^T - (t^2 dbeta + 2 beta) p w^T + beta w p^T + beta (w.p) I + [p]x / 2.
+// jlog6_blocks gives the 3x3 blocks A and C; jlog6_core the assembled matrix.
 template <class T>
-__device__ __forceinline__ void jlog6_core(V3<T> w, T t, T al, T be, T db, V3<T> p, T *Jl) {
+__device__ __forceinline__ void jlog6_blocks(V3<T> w, T t, T al, T be, T db, V3<T> p, T *A, T *C) {
   T wv[3] = {w.x, w.y, w.z}, pv[3] = {p.x, p.y, p.z};
-  T A[9], C[9];
   T wp = dot(w, p);
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
@@ -96,6 +96,11 @@ __device__ __forceinline__ void jlog6_core(V3<T> w, T t, T al, T be, T db, V3<T>
   A[3] += T(0.5) * w.z; A[6] -= T(0.5) * w.y; A[7] += T(0.5) * w.x;
   C[1] -= T(0.5) * p.z; C[2] += T(0.5) * p.y; C[5] -= T(0.5) * p.x;
   C[3] += T(0.5) * p.z; C[6] -= T(0.5) * p.y; C[7] += T(0.5) * p.x;
+}
+template <class T>
+__device__ __forceinline__ void jlog6_core(V3<T> w, T t, T al, T be, T db, V3<T> p, T *Jl) {
+  T A[9], C[9];
+  jlog6_blocks(w, t, al, be, db, p, A, C);
   T CA[9];
   mm(C, A, CA);
   for (int i = 0; i < 3; ++i)
@@ -125,6 +130,16 @@ template <class T> __device__ __forceinline__ S6<T> log6_jlog6_inv(const T *R, V
   T al, be, db;
   log_coeffs_dev(t, al, be, db);
   jlog6_core(T(-1) * w, t, al, be, db, T(-1) * mtv(R, p), Jl);
+  return {al * p - T(0.5) * cross(w, p) + (be * dot(w, p)) * w, w};
+}
+// The same with Jlog6(M^-1) as its blocks A, C: Jlog6 v = [A v_l + C (A v_a); A v_a]
+// costs 27 products per column instead of 36, and C A is never formed.
+template <class T> __device__ __forceinline__ S6<T> log6_jlog6_inv_blocks(const T *R, V3<T> p, T *A, T *C) {
+  T t;
+  V3<T> w = log3_dev(R, t);
+  T al, be, db;
+  log_coeffs_dev(t, al, be, db);
+  jlog6_blocks(T(-1) * w, t, al, be, db, T(-1) * mtv(R, p), A, C);
   return {al * p - T(0.5) * cross(w, p) + (be * dot(w, p)) * w, w};
 }
 
