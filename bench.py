@@ -254,7 +254,8 @@ def mixed_workloads(sa, S, np, torch, dev):
     iters = float(sum(o[3].sum().item() for o in outs)) / 65536
     res["clik_mixed"] = {"problems": nc, "problems_s": 65536 / sec, "ms": sec * 1e3, "success_rate": succ,
                          "mean_iterations": iters, "max_iterations": 1000,
-                         "kernel": "lanes per joint (4 lanes x 4 chain joints per problem), persistent refill queue"}
+                         "kernel": "lanes per joint, two persistent launches: 4 lanes x 4 chain joints per problem "
+                                   "for the first 128 iterations, then the problems still running at 8 lanes x 2 joints"}
 
     # rollouts: 4096 split 4:3:2:1, 128 steps, tau ~ N(0, 5^2), forward and backward
     nr = _split(4096)

@@ -147,16 +147,14 @@ template <int CTRL> __device__ __forceinline__ float dpp_old(float old, float x)
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old), __builtin_bit_cast(int, x),
                                                                CTRL, 0xF, 0xF, false));
 }
-// Zero past the row edge (bound_ctrl): no `old` operand to materialize first.
-template <int CTRL> __device__ __forceinline__ float dpp_zero(float x) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), CTRL, 0xF, 0xF, true));
-}
+// (The zero entries could use bound_ctrl DPP moves, which need no `old`
+// value first: 27 fewer instructions per wavefront, but measured twice as slow
+// at 262144 environments and more on gfx950, so the explicit `old` stays.)
 template <int D> __device__ __forceinline__ Xf below_xf_il(const Xf &x) {  // joint j - D, identity before joint 0
   Xf o;
 #pragma unroll
-  for (int k = 0; k < 9; ++k)
-    o.R[k] = (k % 4 == 0) ? dpp_old<0x110 + 2 * D>(1.f, x.R[k]) : dpp_zero<0x110 + 2 * D>(x.R[k]);
-  o.p = {dpp_zero<0x110 + 2 * D>(x.p.x), dpp_zero<0x110 + 2 * D>(x.p.y), dpp_zero<0x110 + 2 * D>(x.p.z)};
+  for (int k = 0; k < 9; ++k) o.R[k] = dpp_old<0x110 + 2 * D>((k % 4 == 0) ? 1.f : 0.f, x.R[k]);
+  o.p = {dpp_old<0x110 + 2 * D>(0.f, x.p.x), dpp_old<0x110 + 2 * D>(0.f, x.p.y), dpp_old<0x110 + 2 * D>(0.f, x.p.z)};
   return o;
 }
 template <int D> __device__ __forceinline__ S6<float> below6_il(S6<float> v) {
