@@ -194,14 +194,15 @@ int sapamd_clik_multistart_batched(const sapamd_model *m, int64_t B, int32_t nst
                                    int32_t max_iterations, double dt, double damp, float *q_out, int32_t *success,
                                    float *err, int32_t *iterations, int32_t *solved, void *stream);
 
-/* CLIK problems of several models in ONE persistent launch (at most 8
- * models): model i solves B[i] problems for its link links[i], with targets[i]
+/* CLIK problems of several models in one batch (at most 8 models; one
+ * persistent launch per phase over all of them): model i solves B[i] problems for its link links[i], with targets[i]
  * (B[i], 7), optional q0[i] (B[i], nv_i) and masks[i] (nv_i), writing
  * q_out[i], success[i], err[i], iterations[i] (may be NULL) like
  * sapamd_clik_batched. Lanes pull problems from one queue, so early-exiting
  * problems of any model free their lane for the next one (no reference
  * counterpart: the reference solves one problem per call,
- * pinocchio_model.cpp:228-300). The queue counter belongs to models[0]. */
+ * pinocchio_model.cpp:228-300). The queue counters and hand-over records
+ * belong to models[0]. */
 int sapamd_clik_hetero_batched(const sapamd_model *const *models, int32_t nmodels, const int64_t *B,
                                const int32_t *links, const float *const *targets, const float *const *q0,
                                const int32_t *const *masks, double eps, int32_t max_iterations, double dt,
