@@ -237,6 +237,32 @@ def test_batched_clik(variant):
     assert agree >= 28
 
 
+def test_batched_clik_results_do_not_depend_on_the_batch():
+    """The lanes-per-joint CLIK runs every problem for up to 128 iterations in
+    one launch and hands the ones still running to a second launch through a
+    record buffer (stuck-looking ones first). A problem's result must not
+    depend on what else is in the batch or where its record lands: the same
+    512 reachable problems solved alone and interleaved with 512 unreachable
+    ones (targets moved 5 m away, which run to the iteration limit) give the
+    same bits; the unreachable ones report failure after max_iterations."""
+    rob = S.chain7()
+    m, o = _pair(rob)
+    n = 512
+    tgt = o.link_poses(S.samples(rob, n, 1009)[0])[:, 7]
+    far = o.link_poses(S.samples(rob, n, 1010)[0])[:, 7].copy()
+    far[:, :3] += 5.0
+    mixed = np.empty((2 * n, 7))
+    mixed[0::2], mixed[1::2] = tgt, far
+    alone = [x.cpu().numpy() for x in m.compute_inverse_kinematics_batched(7, _t(tgt))]
+    both = [x.cpu().numpy() for x in m.compute_inverse_kinematics_batched(7, _t(mixed))]
+    assert (alone[3] > 128).any()  # some reachable problems went through the hand-over
+    for a, b in zip(alone, both):
+        assert np.array_equal(a, b[0::2])
+    Q, ok, err, it = (x[1::2] for x in both)
+    assert not ok.any() and (it == 1000).all()
+    assert (np.linalg.norm(err, axis=1) > 1.0).all()
+
+
 # ----------------------------------------------------------------- kernel families agree
 @pytest.mark.parametrize("n", [3, 7, 12, 16])
 def test_lanes_per_joint_matches_oracle_and_generic(n):
