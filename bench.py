@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--kernel", default="auto", help="auto | generic | fast (kernel variant of the fused path)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
     ap.add_argument("--no-workloads", action="store_true", help="skip the mixed-tree CLIK / rollout / hetero sections")
+    ap.add_argument("--workloads-only", action="store_true",
+                    help="run only the mixed-tree CLIK / rollout / hetero sections (for profiling them)")
     ap.add_argument("--shape", type=int, nargs=2, default=None, metavar=("G", "C"),
                     help="lanes-per-joint shape instead of autotuning (profiling runs, where tracing skews the tuner)")
     ap.add_argument("--probe-traffic", action="store_true", help=argparse.SUPPRESS)
@@ -326,6 +328,9 @@ def main():
     from sapien_amd import synthetic as S
 
     dev = torch.device("cuda:0")
+    if args.workloads_only:
+        print(json.dumps({"workloads": mixed_workloads(sa, S, np, torch, dev)}))
+        return
     rob = S.chain7()
     m = sa.PinocchioModel(rob.urdf, [0.0, 0.0, -9.81])
     m.set_joint_order(rob.joint_names)
