@@ -14,7 +14,7 @@ Beside the headline number it reports:
                 8 TB/s HBM3E; traffic from rocprofv3 FETCH_SIZE + WRITE_SIZE;
                 the FP32 VALU work (rocprofv3 SQ_INSTS_VALU_* counters) against
                 the 157.3 TFLOP/s vector peak; the bandwidth an elementwise
-                copy reaches on the box;
+                copy reaches on this GPU;
   cpu_baseline  the FP64 C++ oracle (same algorithms) timed on this host's cores
                 on a bounded sample, on one core, and through its per-call
                 Python API (one configuration per call);
@@ -422,7 +422,7 @@ def main():
     achieved = BYTES_PER_EVAL * B / (kern_ms * 1e-3) / 1e9
     traffic = None if args.no_traffic else measure_traffic(tuned[B][:2] if tuned else None)
 
-    # achievable HBM bandwidth on this box: an elementwise pass over 256 MiB
+    # achievable HBM bandwidth on this GPU: an elementwise pass over 256 MiB
     # (read + write), the calibration kernel of the traffic counters
     csrc = torch.ones(COPY_BYTES // 4, device=dev)
     cdst = torch.empty_like(csrc)
@@ -466,7 +466,7 @@ def main():
                      "kernel_ms": kern_ms, "bytes_per_launch": BYTES_PER_EVAL * B,
                      "copy_gbs": {"value": copy_gbs, "what": "torch.neg over 256 MiB (read + write), 10 launches, "
                                                              "HIP events: the bandwidth an elementwise kernel reaches "
-                                                             "on this box"}},
+                                                             "on this GPU"}},
         "launch_modes": launch_modes,
     }
     if traffic and "VALU" in traffic:
