@@ -155,8 +155,10 @@ __device__ __forceinline__ void tree_body(const ModelTable<float> *M, const Mode
     cf[g * 12 + 9] = Fj.a.x; cf[g * 12 + 10] = Fj.a.y; cf[g * 12 + 11] = Fj.a.z;
     wave_sync();
     // branch-free: every load is issued up front, both products computed and
-    // selected
-    float *Mrow = io.M + (e * nv + u) * nv;
+    // selected. M is symmetric (both triangles come from the same dot6(S_a,
+    // F_b) with the ancestor's column first), so lane u stores entry (i, u)
+    // instead of (u, i): at each i the group writes one contiguous row.
+    float *Mcol = io.M + e * nv * nv + u;
     const bool out = live && valid;
 #pragma unroll 4
     for (int i = 0; i < nj; ++i) {
@@ -168,7 +170,7 @@ __device__ __forceinline__ void tree_body(const ModelTable<float> *M, const Mode
       const float dij = dot6(ci, Fj), dji = dot6(col, Fi);
       const bool i_anc_j = i <= g && g < i + sub_i;
       const bool j_anc_i = g < i && i < g + T.size;
-      if (out) st<false>(i_anc_j ? dij : j_anc_i ? dji : 0.f, Mrow + vu_i);
+      if (out) st<false>(i_anc_j ? dij : j_anc_i ? dji : 0.f, Mcol + (int64_t)vu_i * nv);
     }
   }
 }
