@@ -121,8 +121,8 @@ struct sapamd_model {
   int32_t *d_chain = nullptr;   // every link's support chain, concatenated
   std::vector<int> chain_off, chain_len;
   int32_t *d_counter = nullptr;  // CLIK [0] work queue, [1], [3] hand-over counts, [2] second-phase queue
-  ClikCont *d_cont = nullptr;    // CLIK hand-over records (grown on demand)
-  int64_t cont_cap = 0;
+  mutable ClikCont *d_cont = nullptr;  // CLIK hand-over records (grown on demand)
+  mutable int64_t cont_cap = 0;
   hipStream_t stream = nullptr;  // the device's shared per-call stream (not owned)
   double *h_buf = nullptr, *d_buf = nullptr;
   size_t buf_doubles = 0;
@@ -161,7 +161,7 @@ struct sapamd_model {
 
   // The lanes-per-joint CLIK's queue counters and hand-over records for n
   // problems; overlapping CLIK launches on one model race on them.
-  void clik_workspace(HeteroClikArgs &A, int64_t n, hipStream_t s) {
+  void clik_workspace(HeteroClikArgs &A, int64_t n, hipStream_t s) const {
     if (n > cont_cap) {
       DeviceGuard g(device);
       if (d_cont) HIP_OK(hipFree(d_cont));
@@ -1111,7 +1111,7 @@ int sapamd_clik_hetero_batched(const sapamd_model *const *models, int32_t nmodel
     bool lanes = clik_lanes_group(max_chain) != 0 && A.total <= INT32_MAX;
     for (int i = 0; i < nmodels; ++i) lanes = lanes && models[i]->variant != SAPAMD_KERNEL_THREAD_PER_ENV;
     if (lanes) {
-      const_cast<sapamd_model *>(models[0])->clik_workspace(A, A.total, S(stream));
+      models[0]->clik_workspace(A, A.total, S(stream));
       HIP_OK(launch_clik_lanes(A, max_chain, cus, S(stream)));
       return;
     }
@@ -1202,7 +1202,7 @@ void clik_launch(const sapamd_model *m, int64_t B, int64_t nstart, int32_t link,
     g.first = 0;
     A.nseg = 1;
     A.total = total;
-    const_cast<sapamd_model *>(m)->clik_workspace(A, total, S(stream));
+    m->clik_workspace(A, total, S(stream));
     if (solved) HIP_OK(launch_fill_i32(solved, INT32_MAX, B, S(stream)));
     HIP_OK(launch_clik_lanes(A, P.nchain, dev_cus, S(stream)));
     return;
