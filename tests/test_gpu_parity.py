@@ -178,6 +178,30 @@ def test_batched_chain7_metric_size():
     assert rel_err(A, Ar) <= 1e-3
 
 
+def test_batched_chain7_large_batches_are_tiled_copies():
+    """Batch sizes of the throughput sweep (262,144 and 1,048,576 environments)
+    cannot be compared against the FP64 oracle in seconds; an environment's
+    result does not depend on its position in the batch, so the 16384-sample
+    batch tiled 16x and 64x must give the 16384 results tiled, bit for bit, in
+    the interleaved 8 x 1 and the 4 x 2 shapes."""
+    from sapien_amd import tune
+
+    rob = S.chain7()
+    m, o = _pair(rob)
+    q, qd, qdd, _ = (_t(x) for x in S.samples(rob, 16384, 1001))
+    try:
+        for G, C in ((8, 1), (4, 2)):
+            tune.set_shape(m, 1 << 40, G, C)
+            ref = m.compute_fk_jacobian_rnea_batched(q, qd, qdd)
+            for reps in (16, 64):
+                out = m.compute_fk_jacobian_rnea_batched(q.repeat(reps, 1), qd.repeat(reps, 1), qdd.repeat(reps, 1))
+                for a, r in zip(out, ref):
+                    assert torch.equal(a, r.repeat(reps, *([1] * (r.dim() - 1)))), (G, C, reps)
+                del out
+    finally:
+        tune.set_shape(m, 0, 0, 0)
+
+
 @pytest.mark.parametrize("tree", TREES)
 def test_batched_mixed_trees(tree):
     rob = S.robot(tree)
