@@ -302,7 +302,10 @@ def mixed_workloads(sa, S, np, torch, dev):
         trees[name] = {"rollouts": nr[j], "forward_ms": tf * 1e3, "backward_ms": tb * 1e3,
                        "precision": sa.dynamics_precision(models[j])[0],
                        "kernel": "lanes per joint" if sa.kernel_variant(models[j]) == "lanes_per_joint"
-                       and robs[j].nv <= 32 else "thread per env"}
+                       and robs[j].nv <= 32 else "thread per env",
+                       # share of rollouts whose final state is finite (explicit Euler at this
+                       # step is unstable for quad18's light feet, in FP64 as well)
+                       "final_state_finite": float(torch.isfinite(ck[j][0]).all(dim=1).float().mean())}
     res["rollouts_mixed"] = {"rollouts": nr, "steps": T, "dt": dt, "checkpoint_every": K,
                              "forward_env_steps_s": 4096 * T / cf, "forward_ms": cf * 1e3,
                              "backward_env_steps_s": 4096 * T / cb, "backward_ms": cb * 1e3,

@@ -711,3 +711,36 @@ def test_real_robot_parity(fname):
             tau = o.rnea(q, qd, qdd)
             A = m.compute_forward_dynamics_batched(_t(q), _t(qd), _t(tau)).cpu().numpy()
             assert rel_err(A, qdd) <= 1e-3, (v, sa.dynamics_precision(m))
+
+
+def test_clik_mixed_workload_size():
+    """The benchmark's CLIK workload at its size: 65,536 problems of the four
+    trees (4:3:2:1), targets = FK of seeded configurations, q0 = neutral. The
+    mixed launch equals each model's own batched CLIK bit for bit, every
+    problem reports at most max_iterations, a converged problem's error is
+    below eps, and a sample of converged results reaches its target in the
+    FP64 oracle."""
+    import sapien_amd as sa
+
+    counts = [26214, 19661, 13107, 6554]
+    models, links, tgts = [], [], []
+    for i, (tree, nprob) in enumerate(zip(["chain7", "arm9", "quad18", "torso30"], counts)):
+        rob = S.robot(tree)
+        m, o = _pair(rob)
+        q = _t(S.samples(rob, nprob, 1004 + i)[0])
+        models.append((m, o, rob))
+        links.append(rob.ee_link)
+        tgts.append(m.compute_forward_kinematics_batched(q)[:, rob.ee_link].contiguous())
+    outs = sa.clik_hetero_batched([x[0] for x in models], links, tgts)
+    total_ok = 0
+    for (m, o, rob), link, t, (Q, ok, err, it) in zip(models, links, tgts, outs):
+        Qr, okr, errr, itr = m.compute_inverse_kinematics_batched(link, t)
+        assert torch.equal(Q, Qr) and torch.equal(ok, okr) and torch.equal(err, errr) and torch.equal(it, itr), rob.name
+        okn, itn, errn = ok.cpu().numpy(), it.cpu().numpy(), err.cpu().numpy()
+        assert ((itn >= 0) & (itn <= 1000)).all() and (itn[~okn] == 1000).all()
+        assert (np.linalg.norm(errn[okn], axis=1) < 1e-4).all()
+        idx = np.flatnonzero(okn)[:256]
+        reached = o.link_poses(Q.cpu().numpy()[idx])[:, link, :3]
+        assert np.abs(reached - t.cpu().numpy()[idx, :3]).max() < 2e-4, rob.name
+        total_ok += okn.sum()
+    assert total_ok / sum(counts) > 0.9

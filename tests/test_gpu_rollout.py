@@ -287,3 +287,33 @@ def test_ill_conditioned_tree_switches_to_fp64(variant):
         rhs = aq[b] @ vq + aqd[b] @ vqd + (atau[b] * vt).sum()
         scale = np.linalg.norm(np.concatenate([gq[b], gqd[b]])) * np.linalg.norm(np.concatenate([jq, jqd]))
         assert abs(lhs - rhs) <= 1e-3 * max(scale, 1e-12), (b, lhs, rhs)
+
+
+@pytest.mark.parametrize("tree", ["chain7", "arm9", "quad18", "torso30"])
+def test_rollout_workload_size_fused_matches_per_step(tree):
+    """The benchmark's rollout workload at its size (4096 rollouts split
+    4:3:2:1, 128 steps, checkpoint every 16): the one-launch rollout and its
+    adjoint equal one launch per step (the same step arithmetic) bit for bit:
+    final state, checkpoints and gradients. (quad18's states leave the FP32
+    range under these torques within about 15 steps, as the FP64 oracle's do:
+    explicit Euler at 1/240 s is unstable for its light feet, so for quad18
+    the equality includes the positions of the non-finite values.)"""
+    import sapien_amd as sa
+
+    rob = S.robot(tree)
+    m, _ = _pair(rob)
+    B = {"chain7": 1638, "arm9": 1229, "quad18": 819, "torso30": 410}[tree]
+    T, K = 128, 16
+    q0, qd0, tau = _problem(rob, B, T, 1520)
+    g = _t(np.random.default_rng(12).normal(size=(B, rob.nv)))
+    res = {}
+    for mode in ("per_step", "fused"):
+        sa.set_rollout_launch(m, mode)
+        qT, qdT, ck = m.rollout_batched(_t(q0), _t(qd0), _t(tau), DT, K)
+        grads = m.rollout_backward_batched(_t(tau), ck, DT, g, g, K)
+        res[mode] = [x.cpu().numpy() for x in (qT, qdT, ck, *grads)]
+    sa.set_rollout_launch(m, "fused")
+    for name, a, b in zip(("qT", "qdT", "ckpt", "dq0", "dqd0", "dtau"), res["per_step"], res["fused"]):
+        if tree != "quad18":
+            assert np.isfinite(b).all(), name
+        assert np.array_equal(a, b, equal_nan=True), (tree, name)
