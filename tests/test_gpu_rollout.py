@@ -294,10 +294,10 @@ def test_rollout_workload_size_fused_matches_per_step(tree):
     """The benchmark's rollout workload at its size (4096 rollouts split
     4:3:2:1, 128 steps, checkpoint every 16): the one-launch rollout and its
     adjoint equal one launch per step (the same step arithmetic) bit for bit:
-    final state, checkpoints and gradients. (quad18's states leave the FP32
-    range under these torques within about 15 steps, as the FP64 oracle's do:
-    explicit Euler at 1/240 s is unstable for its light feet, so for quad18
-    the equality includes the positions of the non-finite values.)"""
+    final state, checkpoints and gradients. (Some quad18 states leave the
+    FP32 range under these torques, as the FP64 oracle's do: explicit Euler at
+    1/240 s is unstable for its light feet at some configurations, so for
+    quad18 the equality includes the positions of the non-finite values.)"""
     import sapien_amd as sa
 
     rob = S.robot(tree)
