@@ -4,6 +4,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <initializer_list>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -505,16 +506,25 @@ struct sapamd_model {
     if (len != nv()) throw std::runtime_error(std::string(what) + " must have length nv");
   }
 
+  // stage arrays of n doubles, back to back from `off`, with one copy
+  double *stage_seq(std::initializer_list<const double *> srcs, size_t n, size_t off) {
+    size_t k = 0;
+    for (const double *src : srcs) std::memcpy(h_buf + off + n * k++, src, n * sizeof(double));
+    HIP_OK(hipMemcpyAsync(d_buf + off, h_buf + off, k * n * sizeof(double), hipMemcpyHostToDevice, stream));
+    return d_buf + off;
+  }
   // stage host doubles to the device buffer at `off` (in doubles)
   double *stage(const double *src, size_t n, size_t off) {
     std::memcpy(h_buf + off, src, n * sizeof(double));
     HIP_OK(hipMemcpyAsync(d_buf + off, h_buf + off, n * sizeof(double), hipMemcpyHostToDevice, stream));
     return d_buf + off;
   }
+  // device buffer at `off` -> the pinned host buffer at the same offset, then
+  // to dst (dst = nullptr: the caller reads h_buf)
   void fetch(double *dst, size_t n, size_t off) {
     HIP_OK(hipMemcpyAsync(h_buf + off, d_buf + off, n * sizeof(double), hipMemcpyDeviceToHost, stream));
     HIP_OK(hipStreamSynchronize(stream));
-    std::memcpy(dst, h_buf + off, n * sizeof(double));
+    if (dst) std::memcpy(dst, h_buf + off, n * sizeof(double));
   }
   // serial chain of <= 16 joints: the G x C chain kernels apply
   bool chain_kernel() const { return host->njoints() >= 1 && host->is_chain() && host->njoints() <= 16; }
@@ -668,9 +678,12 @@ int sapamd_compute_full_jacobian(sapamd_model *m, const double *qpos, int64_t n)
     o.jl_links = o.jw_links + 6 * nv * L;
     HIP_OK(launch_kinematics<double>(m->t64(), m->host->njoints(), 1, KF_POSES | KF_JLINKS, dq, nullptr, nullptr, o,
                                      m->stream));
-    m->fetch(m->pose_cache.data(), 7 * L, nv);
-    m->fetch(m->jw_cache.data(), 6 * nv * L, nv + 7 * L);
-    m->fetch(m->jl_cache.data(), 6 * nv * L, nv + 7 * L + 6 * nv * L);
+    // poses and both link Jacobian sets are contiguous in the device buffer:
+    // one copy and one wait for the three caches
+    m->fetch(nullptr, 7 * L + 12 * nv * L, nv);
+    std::memcpy(m->pose_cache.data(), m->h_buf + nv, 7 * L * sizeof(double));
+    std::memcpy(m->jw_cache.data(), m->h_buf + nv + 7 * L, 6 * nv * L * sizeof(double));
+    std::memcpy(m->jl_cache.data(), m->h_buf + nv + 7 * L + 6 * nv * L, 6 * nv * L * sizeof(double));
   });
 }
 
@@ -716,8 +729,7 @@ int sapamd_compute_coriolis_matrix(sapamd_model *m, const double *qpos, const do
     m->require_tables();
     DeviceGuard g(m->device);
     size_t nv = m->nv();
-    double *dq = m->stage(qpos, nv, 0);
-    double *dv = m->stage(qvel, nv, nv);
+    double *dq = m->stage_seq({qpos, qvel}, nv, 0), *dv = dq + nv;
     HIP_OK(launch_coriolis<double>(m->t64(), m->host->njoints(), 1, dq, dv, m->d_buf + 2 * nv, m->stream));
     m->fetch(out, nv * nv, 2 * nv);
   });
@@ -730,9 +742,7 @@ int sapamd_compute_inverse_dynamics(sapamd_model *m, const double *qpos, const d
     m->require_tables();
     DeviceGuard g(m->device);
     size_t nv = m->nv();
-    double *dq = m->stage(qpos, nv, 0);
-    double *dv = m->stage(qvel, nv, nv);
-    double *da = m->stage(qacc, nv, 2 * nv);
+    double *dq = m->stage_seq({qpos, qvel, qacc}, nv, 0), *dv = dq + nv, *da = dq + 2 * nv;
     KinOut<double> o;
     o.tau = m->d_buf + 3 * nv;
     HIP_OK(launch_kinematics<double>(m->t64(), m->host->njoints(), 1, KF_RNEA, dq, dv, da, o, m->stream));
@@ -747,9 +757,7 @@ int sapamd_compute_forward_dynamics(sapamd_model *m, const double *qpos, const d
     m->require_tables();
     DeviceGuard g(m->device);
     size_t nv = m->nv();
-    double *dq = m->stage(qpos, nv, 0);
-    double *dv = m->stage(qvel, nv, nv);
-    double *df = m->stage(qf, nv, 2 * nv);
+    double *dq = m->stage_seq({qpos, qvel, qf}, nv, 0), *dv = dq + nv, *df = dq + 2 * nv;
     HIP_OK(launch_aba<double>(m->t64(), m->host->njoints(), 1, dq, dv, df, m->d_buf + 3 * nv, m->stream));
     m->fetch(out, nv, 3 * nv);
   });
