@@ -2,11 +2,12 @@
 B = 16384 environments of the seeded 7-dof chain, FP32, one MI355X.
 
 One step = one launch of the fused kernel over one batch of 16384 environments,
-inputs already resident in HBM. 64 distinct input/output buffer sets (~530 MB)
-are rotated so the working set exceeds the 256 MB Infinity Cache and the
-measured bandwidth is HBM bandwidth. Steps run either eagerly (one C-ABI call
-per step) or as replays of a HIP graph holding the 64 launches; `value` is the
-faster of the two. Prints ONE JSON line.
+inputs already resident in HBM. Step i uses input/output buffer set i % 64
+(64 sets, ~530 MB, more than the 256 MB Infinity Cache). The `steps` launches
+are captured once in a HIP graph; after `warmup` eager launches and a 512 MiB
+write that evicts the Infinity Cache, the timed region is one replay of that
+graph, bracketed by HIP events on the launch stream and by the host clock.
+`value` = 16384 x steps / host time. Prints ONE JSON line.
 
 Beside the headline number it reports:
   roofline      algorithmic bytes per launch / mean launch duration (HIP events
@@ -162,23 +163,13 @@ def measure_traffic(shape=None, timeout=120):
     return res
 
 
-MIX = (("chain7", 4), ("arm9", 3), ("quad18", 2), ("torso30", 1))
-
-
-def _split(total):
-    """total split 4:3:2:1 over the four trees (rounded, summing to total)."""
-    w = [k for _, k in MIX]
-    n = [round(total * k / sum(w)) for k in w]
-    n[0] += total - sum(n)
-    return n
-
-
 def mixed_workloads(sa, S, np, torch, dev):
     """The mixed batch of the four trees: FK+J+RNEA+CRBA of 8192 environments
     in one launch, 65,536 CLIK problems in one persistent launch, and 4096 ABA
     rollouts x 128 steps (dt = 1/240, checkpoint every 16) forward and backward."""
     import time as _t
 
+    MIX, _split = S.MIX, S.split_mix
     models, robs = [], []
     for name, _ in MIX:
         rob = S.robot(name)
@@ -259,13 +250,12 @@ def mixed_workloads(sa, S, np, torch, dev):
                          "kernel": "lanes per joint, two persistent launches: 4 lanes x 4 chain joints per problem "
                                    "for the first 128 iterations, then the problems still running at 8 lanes x 2 joints"}
 
-    # rollouts: 4096 split 4:3:2:1, 128 steps, tau ~ N(0, 5^2), forward and backward
+    # rollouts: 4096 split 4:3:2:1, 128 steps, tau ~ N(0, var 5), forward and backward
     nr = _split(4096)
-    T, K, dt = 128, 16, 1.0 / 240.0
+    T, K, dt = S.ROLLOUT_STEPS, S.ROLLOUT_CKPT, S.ROLLOUT_DT
     per = []
     for i, (m, rob, b) in enumerate(zip(models, robs, nr)):
-        q0, qd0, _, _ = S.samples(rob, b, 1005 + i, dtype=np.float32)
-        tau = np.random.default_rng(1005 + i).normal(scale=5.0, size=(b, T, rob.nv)).astype(np.float32)
+        q0, qd0, tau = S.rollout_workload(rob, b, 1005 + i)
         per.append((m, torch.from_numpy(q0).to(dev), torch.from_numpy(qd0).to(dev), torch.from_numpy(tau).to(dev)))
     ck = {}
     # one default-priority stream per tree: the trees' step launches overlap
@@ -360,69 +350,69 @@ def main():
     stream = torch.cuda.current_stream(dev)
     launch = sa.native_fk_jac_rnea(m)  # C-ABI launcher over raw device pointers
 
-    def step(i):
+    def step(i, st=None):
         s = sets[i % NSETS]
-        launch(B, s["q"], s["qd"], s["qdd"], s["P"], s["J"], s["T"], stream.cuda_stream)
+        launch(B, s["q"], s["qd"], s["qdd"], s["P"], s["J"], s["T"], st or stream.cuda_stream)
 
-    for i in range(args.warmup):
-        step(i)
-    torch.cuda.synchronize()
-    # (1) eager: one C-ABI launch per step from Python, with and without
-    # per-launch events in the stream
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        ev[i][0].record(stream)
-        step(i)
-        ev[i][1].record(stream)
-    torch.cuda.synchronize()
-    eager_ms_events = float(np.mean([a.elapsed_time(b) for a, b in ev]))
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step(i)
-    torch.cuda.synchronize()
-    elapsed_eager = time.perf_counter() - t0
+    # 512 MiB written between the warm-up and the timed region: the 256 MB
+    # Infinity Cache then holds none of the timed launches' buffers
+    flush = torch.empty(512 << 20, dtype=torch.uint8, device=dev)
 
-    # (2) graph: the 64 launches (one per buffer set) captured once in a HIP
-    # graph; K steps = K // 64 replays + K % 64 eager launches, bracketed by
-    # events on the launch stream. Host launch cost leaves the loop.
+    # The timed region: exactly `steps` launches, step i over buffer set
+    # i % 64, captured once in a HIP graph and replayed once. Set 0's outputs
+    # from an eager launch are the check that the replay computes the same bits.
+    step(0)
+    torch.cuda.synchronize()
     ref0 = [t.clone() for t in (sets[0]["P"], sets[0]["J"], sets[0]["T"])]
     graph = torch.cuda.CUDAGraph()
     cap = torch.cuda.Stream(dev)
     cap.wait_stream(stream)
     with torch.cuda.graph(graph, stream=cap):
         cs = torch.cuda.current_stream(dev).cuda_stream
-        for k in range(NSETS):
-            sk = sets[k]
-            launch(B, sk["q"], sk["qd"], sk["qdd"], sk["P"], sk["J"], sk["T"], cs)
+        for i in range(args.steps):
+            step(i, cs)
     stream.wait_stream(cap)
+    torch.cuda.synchronize()
+    # warm-up: `warmup` eager launches over the sets counted down from the
+    # last one (with few steps, not the timed sets)
+    for i in range(args.warmup):
+        step(NSETS - 1 - (i % NSETS))
     for t in (sets[0]["P"], sets[0]["J"], sets[0]["T"]):
         t.fill_(float("nan"))
-    graph.replay()
-    torch.cuda.synchronize()
-    graph_ok = all(torch.equal(a, b) for a, b in zip(ref0, (sets[0]["P"], sets[0]["J"], sets[0]["T"])))
-    full, rem = divmod(args.steps, NSETS)
-    for _ in range(2):
-        graph.replay()
+    flush.fill_(1)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     e0.record(stream)
-    for _ in range(full):
-        graph.replay()
-    for i in range(rem):
-        step(i)
+    graph.replay()
     e1.record(stream)
     torch.cuda.synchronize()
-    elapsed_graph = time.perf_counter() - t0
+    elapsed = time.perf_counter() - t0
     graph_ms = e0.elapsed_time(e1) / args.steps
-
-    elapsed = min(elapsed_eager, elapsed_graph)
+    graph_ok = all(torch.equal(a, b) for a, b in zip(ref0, (sets[0]["P"], sets[0]["J"], sets[0]["T"])))
+    if not graph_ok:
+        raise RuntimeError("the graph replay differs from the eager launch on buffer set 0")
     value = B * args.steps / elapsed
     # average launch duration on the launch stream over the timed region
-    kern_ms = graph_ms if graph_ok else eager_ms_events
+    kern_ms = graph_ms
     achieved = BYTES_PER_EVAL * B / (kern_ms * 1e-3) / 1e9
+
+    # not the metric: the same graph replayed a second time, and `steps` eager
+    # launches (one C-ABI call per step from Python), each after the same
+    # cache flush, HIP events around each sequence
+    def timed_seq(fn):
+        flush.fill_(2)
+        torch.cuda.synchronize()
+        s0 = time.perf_counter()
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - s0) / args.steps, e0.elapsed_time(e1) / args.steps
+
+    second_wall, second_ms = timed_seq(graph.replay)
+    eager_wall, eager_ms = timed_seq(lambda: [step(i) for i in range(args.steps)])
+    del flush
     traffic = None if args.no_traffic else measure_traffic(tuned[B][:2] if tuned else None)
 
     # achievable HBM bandwidth on this GPU: an elementwise pass over 256 MiB
@@ -441,9 +431,12 @@ def main():
     del csrc, cdst
 
     launch_modes = {
-        "graph": {"ms_per_step": elapsed_graph / args.steps * 1e3, "events_ms_per_launch": graph_ms,
-                  "replay_matches_eager": graph_ok},
-        "eager": {"ms_per_step": elapsed_eager / args.steps * 1e3, "events_ms_per_launch": eager_ms_events},
+        "graph": {"ms_per_step": elapsed / args.steps * 1e3, "events_ms_per_launch": graph_ms,
+                  "replay_matches_eager": graph_ok,
+                  "what": f"the timed region: one replay of a HIP graph of {args.steps} launches"},
+        "graph_second_replay": {"ms_per_step": second_wall * 1e3, "events_ms_per_launch": second_ms},
+        "eager": {"ms_per_step": eager_wall * 1e3, "events_ms_per_launch": eager_ms,
+                  "what": "one C-ABI call per step from Python"},
     }
     out = {
         "metric": METRIC,
@@ -457,7 +450,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
-        "data": "synthetic: seeded 7-dof chain (seed 0), q~U(limits), qd~N(0,1), qdd~N(0,var 4), 64 rotating sets",
+        "data": (f"synthetic: seeded 7-dof chain (seed 0), q~U(limits), qd~N(0,1), qdd~N(0,var 4); step i reads "
+                 f"buffer set i % {NSETS} ({min(args.steps, NSETS)} distinct sets in the timed region), "
+                 f"Infinity Cache flushed before it"),
         "config": {"workload": "chain7 FK (8 link poses) + WORLD Jacobian (6x7) + RNEA, B=16384, FP32",
                    "envs": B, "dof": n, "links": L, "kernel": sa.kernel_variant(m),
                    "shape": ({"lanes_per_env": tuned[B][0], "joints_per_lane": tuned[B][1],
@@ -543,8 +538,7 @@ def main():
             gm.set_joint_order(grob.joint_names)
             gm.set_link_order(grob.link_names)
         GB, gn, gL = 8192, grob.nv, grob.nlinks
-        gq, gqd, gqdd, gts = [torch.from_numpy(x).to(dev) for x in S.samples(grob, GB, 1003, dtype=np.float32)]
-        gt = torch.from_numpy(S.samples(grob, GB, 1013, dtype=np.float32)[0]).to(dev)
+        gq, gqd, gqdd, gts, gt = [torch.from_numpy(x).to(dev) for x in S.gradient_workload(grob, GB)]
         gt = gm.compute_forward_kinematics_batched(gt)[:, grob.ee_link, :3].contiguous()
         bufs = dict(P=torch.empty(GB, gL, 7, device=dev), T=torch.empty(GB, gn, device=dev),
                     loss=torch.empty(GB, device=dev), dq=torch.empty(GB, gn, device=dev),
@@ -619,7 +613,20 @@ def main():
         o = OracleModel(rob.urdf)
         o.set_joint_order(rob.joint_names)
         o.set_link_order(rob.link_names)
-        threads = min(16, os.cpu_count() or 1)
+        # host cores: the GPU box allots 16 host cores to each GPU, so the
+        # multi-core baseline uses at most 16 threads (and no more than this
+        # process may run on)
+        try:
+            allowed = len(os.sched_getaffinity(0))
+        except (AttributeError, OSError):
+            allowed = os.cpu_count() or 1
+        threads = max(1, min(16, allowed))
+        cpu_model = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                cpu_model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+        except OSError:
+            pass
         # one seeded chunk of 1M samples, evaluated repeatedly until the sample
         # holds about cpu_seconds of CPU work
         chunk = 1 << 20
@@ -634,6 +641,9 @@ def main():
             reps += 1
         nsamp = chunk * reps
         out["cpu_baseline"] = {"value": nsamp / secs, "unit": "evals/s", "cores": threads, "kind": "port",
+                               "host": {"cpu_model": cpu_model, "os_cpu_count": os.cpu_count(),
+                                        "affinity_cores": allowed,
+                                        "threads_used": threads, "cap": "16 host cores per GPU on the GPU box"},
                                "sample": f"{reps} x {chunk} seeded chain7 samples, FP64 FK + 8 link poses + WORLD "
                                          f"Jacobian + RNEA per sample, C++ oracle statically split over {threads} "
                                          f"threads ({secs:.1f} s)"}
@@ -652,12 +662,15 @@ def main():
         qa = S.samples(rob, 1024, 1000)[0]
         t0 = time.perf_counter()
         for i in range(1024):
-            o.link_poses(qa[i:i + 1])
-            o.jacobian_world(qa[i:i + 1])
+            fk = o.link_poses(qa[i:i + 1])[0]  # compute_forward_kinematics (every link pose cached)
+            for l in range(L):  # get_link_pose(l): one call per link, a (p, q) pair each
+                _ = (fk[l, :3].copy(), fk[l, 3:].copy())
+            o.jacobian_world(qa[i:i + 1])  # compute_full_jacobian
         out["cpu_baseline"]["per_call_api"] = {
             "value": 1024 / (time.perf_counter() - t0), "unit": "evals/s", "cores": 1,
-            "sample": "1024 seeded chain7 samples, one call each for the link poses and the WORLD Jacobian through "
-                      "the oracle's Python API (FK + 8 link poses + full Jacobian, FP64)"}
+            "sample": "1024 seeded chain7 samples through the oracle's Python API, per sample one FK call, 8 "
+                      "get_link_pose-style per-link reads and one full-Jacobian call (FP64), as the plain CPU "
+                      "reference workload calls PinocchioModel"}
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out))
 

@@ -108,8 +108,9 @@ int sapamd_compute_inverse_kinematics(sapamd_model *m, int32_t link, const float
 /* ---------------------------------------------------------------- batched device path (FP32) */
 /* Kernel family used by the batched kinematics entry points (no reference
  * counterpart; the reference has no GPU path). 0 = auto (lanes-per-joint for
- * serial chains of <= 16 joints, else thread-per-environment), 1 =
- * thread-per-environment, 2 = lanes-per-joint (fails for unsupported trees).
+ * serial chains of <= 16 joints and for trees of <= 32 joints, else
+ * thread-per-environment), 1 = thread-per-environment, 2 = lanes-per-joint
+ * (fails for larger trees).
  * Not synchronized with launches in flight on other threads. */
 enum { SAPAMD_KERNEL_AUTO = 0, SAPAMD_KERNEL_THREAD_PER_ENV = 1, SAPAMD_KERNEL_LANES_PER_JOINT = 2 };
 int sapamd_set_kernel_variant(sapamd_model *m, int32_t variant);
@@ -132,14 +133,32 @@ int sapamd_fk_jac_rnea_batched(const sapamd_model *m, int64_t B, const float *q,
 /* The same over padded buffers, without copies: q/qd/qdd/tau rows every *_ld
  * floats (0 = nv), link poses every pose_link_stride floats (0 = 7) within
  * rows of pose_env_stride floats (0 = L * 7); J stays (B, 6, nv). This reads a
- * GPU simulator's joint state [n_art, max_dof] in place and writes p | q wxyz
- * into the first 7 floats per link of a [n_art, max_links, 13] body-state
- * buffer (src/physx/physx_system.cpp:1048-1101 lay these out for SAPIEN's GPU
- * PhysX system). */
+ * GPU simulator's joint state [n_art, max_dof] in place
+ * (src/physx/physx_system.cpp:1076-1101). Poses are in the model's root frame,
+ * as get_link_pose returns them; sapamd_link_state_strided writes world-frame
+ * link poses and velocities into a simulator's body-state buffer. */
 int sapamd_fk_jac_rnea_strided(const sapamd_model *m, int64_t B, const float *q, int64_t q_ld, const float *qd,
                                int64_t qd_ld, const float *qdd, int64_t qdd_ld, float *poses,
                                int64_t pose_env_stride, int64_t pose_link_stride, float *J, float *tau,
                                int64_t tau_ld, void *stream);
+/* Link poses and velocities of B articulations written into a GPU
+ * simulator's body-state buffer, the work of PhysxSystemGpu::
+ * gpuUpdateArticulationKinematics followed by gpuFetchArticulationLinkPose /
+ * gpuFetchArticulationLinkVel (src/physx/physx_system.cpp:585-618,711-721):
+ * body rows of 13 floats p xyz | q wxyz | v xyz | w xyz (SapienBodyData,
+ * src/physx/physx_system.cuh:89-94), link l of articulation b at
+ * body + b * body_env_stride + l * body_link_stride (0 = L * 13 and 13; the
+ * [n_art, max_links, 13] layout of src/physx/physx_system.cpp:1048-1051).
+ * Row 0 of each articulation holds user link 0 (the simulator's root link)
+ * and is read first as its world pose (world minus the scene offset,
+ * src/physx/physx_system.cu:85-99) and, with qd, its velocity, which place the
+ * articulation; then every user link's row receives its world pose and, with
+ * qd, the linear velocity of its centre of mass and its angular velocity (world
+ * frame), row 0 included (rewritten to the same values up to rounding). q / qd rows every q_ld / qd_ld floats (0 = nv); qd may be NULL
+ * (poses only). Trees of 1 to 32 movable joints. */
+int sapamd_link_state_strided(const sapamd_model *m, int64_t B, const float *q, int64_t q_ld, const float *qd,
+                              int64_t qd_ld, float *body, int64_t body_env_stride, int64_t body_link_stride,
+                              void *stream);
 /* Passive joint forces, as PhysxArticulation::computePassiveForce
  * (src/physx/articulation.cpp:400-418): gravity compensation g(q) and/or
  * Coriolis and centrifugal forces C(q, qd) qd, i.e. RNEA(q, qd, 0) with the

@@ -18,11 +18,10 @@ template <class S> void pose7(const SE3<S> &M, S *out) {
 }
 
 SE3<double> se3_from_pose7(const double *pq) {
-  // pose given as p(3) + quaternion wxyz(4); the quaternion is normalized as
-  // Eigen::Quaterniond(...).toRotationMatrix() would (pinocchio_model.cpp:255).
-  double w = pq[3], x = pq[4], y = pq[5], z = pq[6];
-  double n = std::sqrt(w * w + x * x + y * y + z * z);
-  w /= n; x /= n; y /= n; z /= n;
+  // pose given as p(3) + quaternion wxyz(4), converted as
+  // Eigen::Quaterniond(w, x, y, z).toRotationMatrix() does (pinocchio_model.cpp:255):
+  // the unit-quaternion formula applied as is, without normalizing q.
+  const double w = pq[3], x = pq[4], y = pq[5], z = pq[6];
   SE3<double> M;
   M.R(0, 0) = 1 - 2 * (y * y + z * z); M.R(0, 1) = 2 * (x * y - w * z); M.R(0, 2) = 2 * (x * z + w * y);
   M.R(1, 0) = 2 * (x * y + w * z); M.R(1, 1) = 1 - 2 * (x * x + z * z); M.R(1, 2) = 2 * (y * z - w * x);

@@ -290,6 +290,7 @@ struct sapamd_model {
         lr[l].p[i] = T(f.place.p[i]);
         if (f.place.p[i] != 0.0) ident = false;
       }
+      for (int i = 0; i < 3; ++i) lr[l].c[i] = T(f.com[i]);
       lr[l].joint = f.joint;
       lr[l].ident = ident ? 1 : 0;
     }
@@ -948,6 +949,29 @@ int sapamd_fk_jac_rnea_strided(const sapamd_model *m, int64_t B, const float *q,
   });
 }
 
+int sapamd_link_state_strided(const sapamd_model *m, int64_t B, const float *q, int64_t q_ld, const float *qd,
+                              int64_t qd_ld, float *body, int64_t body_env_stride, int64_t body_link_stride,
+                              void *stream) {
+  return guarded([&] {
+    check_batched(m, B);
+    if (B == 0) return;
+    need(q, "q");
+    need(body, "body");
+    const int nv = m->nv(), L = m->L();
+    if (L == 0) throw std::runtime_error("link index out of bound");
+    if (!tree_group(m->host->njoints()))
+      throw std::invalid_argument("link state: trees of 1 to 32 movable joints only");
+    check_ld(q_ld, nv, "q");
+    check_ld(qd_ld, nv, "qd");
+    check_ld(body_link_stride, 13, "body link");
+    const int64_t ls = body_link_stride ? body_link_stride : 13;
+    check_ld(body_env_stride, (int64_t)(L - 1) * ls + 13, "body");
+    const int64_t es = body_env_stride ? body_env_stride : (int64_t)L * ls;
+    HIP_OK(launch_link_state(m->t32(), m->host->njoints(), B, q, q_ld ? q_ld : nv, qd, qd_ld ? qd_ld : nv, body, es,
+                             ls, S(stream)));
+  });
+}
+
 int sapamd_passive_force_batched(const sapamd_model *m, int64_t B, const float *q, int64_t q_ld, const float *qd,
                                  int64_t qd_ld, int32_t gravity, int32_t coriolis_and_centrifugal, float *tau,
                                  int64_t tau_ld, void *stream) {
@@ -1059,6 +1083,28 @@ int sapamd_hetero_batched(const sapamd_model *const *models, int32_t nmodels, co
 }
 
 
+namespace {
+constexpr int kMaxDevices = 64;
+
+// Compute units of a device (queried once per device).
+int device_cus(int device) {
+  static std::mutex mu;
+  static int cus[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) return 256;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!cus[device]) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n <= 0) {
+      (void)hipGetLastError();
+      n = 256;
+    }
+    cus[device] = n;
+  }
+  return cus[device];
+}
+
+}  // namespace
+
 int sapamd_clik_hetero_batched(const sapamd_model *const *models, int32_t nmodels, const int64_t *B,
                                const int32_t *links, const float *const *targets, const float *const *q0,
                                const int32_t *const *masks, double eps, int32_t max_iterations, double dt,
@@ -1113,9 +1159,7 @@ int sapamd_clik_hetero_batched(const sapamd_model *const *models, int32_t nmodel
     // the queue counter (and hand-over records) of the first model:
     // overlapping launches on it race
     A.counter = models[0]->d_counter;
-    int cus = 256;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, models[0]->device) == hipSuccess) cus = prop.multiProcessorCount;
+    const int cus = device_cus(models[0]->device);
     bool lanes = clik_lanes_group(max_chain) != 0 && A.total <= INT32_MAX;
     for (int i = 0; i < nmodels; ++i) lanes = lanes && models[i]->variant != SAPAMD_KERNEL_THREAD_PER_ENV;
     if (lanes) {
@@ -1176,6 +1220,7 @@ namespace {
 void clik_launch(const sapamd_model *m, int64_t B, int64_t nstart, int32_t link, const float *target,
                  const float *q0, const int32_t *mask, double eps, int32_t max_iterations, double dt, double damp,
                  float *q_out, int32_t *success, float *err, int32_t *iterations, int32_t *solved, void *stream) {
+  int32_t *counters = m->d_counter;
   ClikParams P;
   P.link = link;
   P.chain = m->d_chain + m->chain_off[link];
@@ -1186,13 +1231,11 @@ void clik_launch(const sapamd_model *m, int64_t B, int64_t nstart, int32_t link,
   P.max_iter = max_iterations;
   P.mask = mask;
   P.mask_ld = 0;
-  P.counter = m->d_counter;
+  P.counter = counters;
   P.ntargets = solved ? B : 0;
   P.solved = solved;
   const int64_t total = B * nstart;
-  int dev_cus = 256;
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, m->device) == hipSuccess) dev_cus = prop.multiProcessorCount;
+  const int dev_cus = device_cus(m->device);
   if (clik_lanes_group(P.nchain) && total <= INT32_MAX && m->variant != SAPAMD_KERNEL_THREAD_PER_ENV) {
     // one segment of the lanes-per-joint CLIK (problems taken in index order,
     // so multi-start problems run start-major)
@@ -1221,7 +1264,7 @@ void clik_launch(const sapamd_model *m, int64_t B, int64_t nstart, int32_t link,
   // order): finished lanes pull the next problem. The queue counter belongs to
   // the model, so overlapping batched CLIK calls on one model race.
   P.persistent_waves = (waves > 4 * dev_cus || solved) ? (int)std::min<int64_t>(waves, 4 * dev_cus) : 0;
-  if (P.persistent_waves) HIP_OK(launch_fill_i32(m->d_counter, 0, 1, S(stream)));
+  if (P.persistent_waves) HIP_OK(launch_fill_i32(counters, 0, 1, S(stream)));
   else P.counter = nullptr;
   if (solved) HIP_OK(launch_fill_i32(solved, INT32_MAX, B, S(stream)));
   HIP_OK(launch_clik<float>(m->t32(), m->host->njoints(), total, P, target, q0, q_out, success, err, iterations,

@@ -366,14 +366,26 @@ template <int G, int C, bool CONT> __global__ __launch_bounds__(256) void k_clik
       }
       const bool ok = en < float(P.eps);
       if (ok || st.it >= P.max_iter) {
-        // result: chain dofs from the best iterate (continuous wrapped), the rest from q0
+        // result: chain dofs from the best iterate, the rest from q0; every
+        // continuous dof wrapped by atan2 (posP2S over the whole configuration,
+        // pinocchio_model.cpp:87,299)
         const int nv = S.tab->nv;
         float *qo = S.q_out + st.prob * nv;
         const float *q0 = S.q0 ? S.q0 + st.prob * nv : nullptr;
-        for (int u = g; u < nv; u += G) {
+        const JointRec<float> *JT = joints_of(S.tab);
+        for (int j = g; j < S.tab->nj; j += G) {
+          const int u = JT[j].vuser;
           bool chain_dof = false;
           for (int k = 0; k < st.nc; ++k) chain_dof |= chain[k].vuser == u;
-          if (!chain_dof) qo[u] = q0 ? q0[u] : 0.f;
+          if (!chain_dof) {
+            float v = q0 ? q0[u] : 0.f;
+            if (JT[j].kind == KIND_CONTINUOUS) {
+              float s, co;
+              sincosf(v, &s, &co);
+              v = atan2f(s, co);
+            }
+            qo[u] = v;
+          }
         }
 #pragma unroll
         for (int c = 0; c < C; ++c) {

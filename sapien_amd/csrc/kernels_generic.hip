@@ -477,9 +477,20 @@ __device__ void clik_problem(const ModelTable<T> *__restrict__ M, const ClikPara
         qc[k] += v * T(P.dt);
       }
     }
-    // result: chain dofs from the best iterate, the rest from q0; continuous wrapped
+    // result: chain dofs from the best iterate, the rest from q0; every
+    // continuous dof wrapped by atan2 (posP2S over the whole configuration,
+    // pinocchio_model.cpp:87,299)
     T *qo = q_out + prob * nv;
-    for (int u = 0; u < nv; ++u) qo[u] = q0 ? q0[prob * nv + u] : T(0);
+    for (int j = 0; j < M->nj; ++j) {
+      const JointRec<T> &J = JR[j];
+      T v = q0 ? q0[prob * nv + J.vuser] : T(0);
+      if (J.kind == KIND_CONTINUOUS) {
+        T s, c;
+        sincos_t(v, &s, &c);
+        v = atan2(s, c);
+      }
+      qo[J.vuser] = v;
+    }
     for (int k = 0; k < nc; ++k) {
       const JointRec<T> &J = JR[P.chain[k]];
       T v = qbest[k];

@@ -312,6 +312,137 @@ __global__ __launch_bounds__(256) void k_tree_vjp(const ModelTable<float> *__res
   tree_vjp_body<G>(reinterpret_cast<const ModelTable<float> *>(s_tab), B, env, q, qd, qdd, gsrc, A);
 }
 
+// Link poses and velocities written into a GPU simulator's body-state rows
+// (13 floats: p xyz | q wxyz | v xyz | w xyz, world frame). Row 0 of an
+// environment holds user link 0 (the simulator's root link): its pose and
+// velocity are read first and place the articulation. With (R_m, p_m) a link's
+// pose and (w_m, v_m) its angular and origin velocity in the model frame
+// (from q, qd), and (R_0, p_0, w_0) read from row 0:
+//   R_b = R_0 R_m0^T, p_b = p_0 - R_b p_m0          (the model frame in the world)
+//   w_b = w_0 - R_b w_m0, u_b = v_0 - R_b v_m0      (its motion; u_b at point p_0)
+//   pose_l = (R_b R_ml, R_b p_ml + p_b),  w_l = w_b + R_b w_ml,
+//   v_l = u_b + w_b x (p_l - p_0) + R_b v_ml + w_l x (R_l c_l),
+// v_0 being row 0's origin velocity (its stored centre-of-mass velocity minus
+// w_0 x R_0 c_0) and the last term moving each origin velocity to the link's
+// centre of mass c_l. Without qd only the poses (first 7 floats) are written.
+struct LinkStateIO {
+  const float *q, *qd;
+  int64_t ld_q, ld_qd;
+  float *body;
+  int64_t env_stride, link_stride;
+};
+
+// Model-frame pose and velocity (angular, at the link origin) of user link l
+// from the group's joint transforms X and joint velocities v (every lane calls
+// this with its own l: the permutes run in all lanes).
+template <int G>
+__device__ __forceinline__ void link_model_state(const Tree<G> &T, const LinkRec<float> &LL, const Xf &X,
+                                                 const S6<float> &v, float *Rm, V3<float> &pm, V3<float> &wm,
+                                                 V3<float> &vm) {
+  const int jl = LL.joint < 0 ? 0 : LL.joint;
+  const Xf W = bperm_xf(T.base4 + 4 * jl, X);
+  const S6<float> Vj = bperm6(T.base4 + 4 * jl, v);
+  mm(W.R, LL.R, Rm);
+  pm = mv(W.R, load3(LL.p)) + W.p;
+  wm = Vj.a;
+  vm = Vj.l + cross(Vj.a, pm);
+  if (LL.joint < 0) {  // on the universe: fixed in the model frame
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Rm[k] = LL.R[k];
+    pm = load3(LL.p);
+    wm = vm = V3<float>{0.f, 0.f, 0.f};
+  }
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void k_link_state(const ModelTable<float> *__restrict__ Mg, int64_t B,
+                                                    LinkStateIO io) {
+  __shared__ float4 s_tab[kTreeTableWords4];
+  stage_table(s_tab, Mg);
+  __syncthreads();
+  const ModelTable<float> *M = reinterpret_cast<const ModelTable<float> *>(s_tab);
+  const int64_t env = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
+  const int g = threadIdx.x & (G - 1);
+  const bool live = env < B;
+  const int64_t e = live ? env : B - 1;
+  const int nj = M->nj, nv = M->nv, L = M->nlinks;
+  const bool vel = io.qd != nullptr;
+  const bool valid = g < nj;
+  const JointRec<float> &J = joints_of(M)[valid ? g : nj - 1];
+  Tree<G> T;
+  T.g = g;
+  T.base4 = ((threadIdx.x & 63) & ~(G - 1)) * 4;
+#pragma unroll
+  for (int k = 0; k < Tree<G>::K; ++k) T.anc[k] = valid ? J.anc[k] : -1;
+  T.size = valid ? J.subtree : 0;
+
+  const int gc = g < nv ? g : nv - 1;
+  const float qu = io.q[e * io.ld_q + gc], qdu = vel ? io.qd[e * io.ld_qd + gc] : 0.f;
+  const int src = T.base4 + 4 * J.vuser;
+  float qj = bperm(src, qu), qdj = bperm(src, qdu);
+  if (!valid) qj = qdj = 0.f;
+  float s, c;
+  joint_sincos(qj, &s, &c);
+  Xf X;
+  joint_world(J, qj, s, c, static_cast<const float *>(nullptr), V3<float>{0, 0, 0}, true, X.R, X.p);
+  if (!valid) X = identity_xf();
+  T.prefix(X);
+  S6<float> v = zero6<float>();
+  if (vel) {
+    v = valid ? qdj * jcol(J, X.R, X.p) : zero6<float>();
+    T.prefix(v);  // spatial velocity of each joint body, model frame, linear part at the model origin
+  }
+
+  // row 0: user link 0's world pose and velocity (read before any row is written)
+  float *row = io.body + e * io.env_stride;
+  const V3<float> p0 = {row[0], row[1], row[2]};
+  float qw = row[3], qx = row[4], qy = row[5], qz = row[6];
+  const float rn = rsqrtf(qw * qw + qx * qx + qy * qy + qz * qz);
+  qw *= rn; qx *= rn; qy *= rn; qz *= rn;
+  const float R0[9] = {1.f - 2.f * (qy * qy + qz * qz), 2.f * (qx * qy - qw * qz), 2.f * (qx * qz + qw * qy),
+                       2.f * (qx * qy + qw * qz), 1.f - 2.f * (qx * qx + qz * qz), 2.f * (qy * qz - qw * qx),
+                       2.f * (qx * qz - qw * qy), 2.f * (qy * qz + qw * qx), 1.f - 2.f * (qx * qx + qy * qy)};
+  V3<float> vc0 = {0.f, 0.f, 0.f}, w0 = {0.f, 0.f, 0.f};
+  if (vel) {
+    vc0 = {row[7], row[8], row[9]};
+    w0 = {row[10], row[11], row[12]};
+  }
+  const LinkRec<float> &L0 = links_of(Mg)[0];
+  float Rm0[9];
+  V3<float> pm0, wm0, vm0;
+  link_model_state<G>(T, L0, X, v, Rm0, pm0, wm0, vm0);
+  float Rb[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      Rb[3 * i + k] = R0[3 * i] * Rm0[3 * k] + R0[3 * i + 1] * Rm0[3 * k + 1] + R0[3 * i + 2] * Rm0[3 * k + 2];
+  const V3<float> pb = p0 - mv(Rb, pm0);
+  const V3<float> wb = w0 - mv(Rb, wm0);
+  const V3<float> ub = vc0 - cross(w0, mv(R0, load3(L0.c))) - mv(Rb, vm0);
+
+  for (int l0 = 0; l0 < L; l0 += G) {  // every lane takes part in the permutes
+    const int l = l0 + g;
+    const LinkRec<float> &LL = links_of(Mg)[l < L ? l : L - 1];
+    float Rm[9];
+    V3<float> pm, wm, vm;
+    link_model_state<G>(T, LL, X, v, Rm, pm, wm, vm);
+    float Rw[9];
+    mm(Rb, Rm, Rw);
+    const V3<float> pw = mv(Rb, pm) + pb;
+    if (live && l < L) {
+      float *out = row + l * io.link_stride;
+      store_pose(out, Rw, pw);
+      if (vel) {
+        const V3<float> ww = wb + mv(Rb, wm);
+        const V3<float> vc = ub + cross(wb, pw - p0) + mv(Rb, vm) + cross(ww, mv(Rw, load3(LL.c)));
+        out[7] = vc.x; out[8] = vc.y; out[9] = vc.z;
+        out[10] = ww.x; out[11] = ww.y; out[12] = ww.z;
+      }
+    }
+  }
+}
+
 // Heterogeneous launch: block ranges map to segments (one tree type each).
 __global__ __launch_bounds__(256) void k_hetero(HeteroArgs args, int flags) {
   __shared__ float4 s_tab[kTreeTableWords4];
@@ -360,6 +491,21 @@ hipError_t launch_tree_fused(const ModelTable<float> *tab, int nj, int64_t B, in
   if (G == 8) hipLaunchKernelGGL(k_tree_fused<8>, grid, dim3(256), lds, s, tab, B, flags, io);
   else if (G == 16) hipLaunchKernelGGL(k_tree_fused<16>, grid, dim3(256), lds, s, tab, B, flags, io);
   else hipLaunchKernelGGL(k_tree_fused<32>, grid, dim3(256), lds, s, tab, B, flags, io);
+  return hipGetLastError();
+}
+
+hipError_t launch_link_state(const ModelTable<float> *tab, int nj, int64_t B, const float *q, int64_t ld_q,
+                             const float *qd, int64_t ld_qd, float *body, int64_t env_stride, int64_t link_stride,
+                             hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  const int G = tree_group(nj);
+  if (!G) return hipErrorNotSupported;
+  LinkStateIO io{q, qd, ld_q, ld_qd, body, env_stride, link_stride};
+  const int64_t per_block = 256 / G;
+  dim3 grid((unsigned)((B + per_block - 1) / per_block));
+  if (G == 8) hipLaunchKernelGGL(k_link_state<8>, grid, dim3(256), 0, s, tab, B, io);
+  else if (G == 16) hipLaunchKernelGGL(k_link_state<16>, grid, dim3(256), 0, s, tab, B, io);
+  else hipLaunchKernelGGL(k_link_state<32>, grid, dim3(256), 0, s, tab, B, io);
   return hipGetLastError();
 }
 

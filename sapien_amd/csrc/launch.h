@@ -47,6 +47,14 @@ hipError_t launch_tree_fused(const ModelTable<float> *tab, int nj, int64_t B, in
 template <class T> struct VjpArgs;
 // Tree version of the fused forward + reverse pass (pose gradients need one
 // user link per joint).
+// Link poses (and, with qd, centre-of-mass and angular velocities) of every
+// user link into body-state rows of 13 floats (p | q wxyz | v | w) every
+// link_stride floats within rows of env_stride floats; row 0 of each
+// environment (the root link) is read first as the articulation's world pose
+// and velocity. Trees of 1 to 32 joints; q / qd rows every ld_q / ld_qd floats.
+hipError_t launch_link_state(const ModelTable<float> *tab, int nj, int64_t B, const float *q, int64_t ld_q,
+                             const float *qd, int64_t ld_qd, float *body, int64_t env_stride, int64_t link_stride,
+                             hipStream_t s);
 hipError_t launch_tree_vjp(const ModelTable<float> *tab, int nj, int64_t B, const float *q, const float *qd,
                            const float *qdd, VjpArgs<float> a, hipStream_t s);
 // All segments in one launch (passed by value, up to kMaxSegments); the grid is

@@ -231,6 +231,7 @@ ArticulatedModel::ArticulatedModel(const std::string &urdf, const double gravity
   if (root->tag != "robot") throw std::invalid_argument("URDF root element must be <robot>");
 
   std::map<std::string, BodyInertia> link_inertia;
+  std::map<std::string, std::array<double, 3>> link_com;
   std::vector<std::string> link_names;
   for (auto &k : root->kids) {
     if (k->tag != "link") continue;
@@ -253,6 +254,9 @@ ArticulatedModel::ArticulatedModel(const std::string &urdf, const double gravity
         for (int j = 0; j < 3; ++j)
           Ir[3 * i + j] = RI[3 * i] * T.R[3 * j] + RI[3 * i + 1] * T.R[3 * j + 1] + RI[3 * i + 2] * T.R[3 * j + 2];
       bi = BodyInertia::from_com(m, T.p, Ir);
+      link_com[nm] = {T.p[0], T.p[1], T.p[2]};
+    } else {
+      link_com[nm] = {0.0, 0.0, 0.0};
     }
     link_inertia[nm] = bi;
     link_names.push_back(nm);
@@ -297,7 +301,11 @@ ArticulatedModel::ArticulatedModel(const std::string &urdf, const double gravity
     if (!is_child.count(ln)) { root_link = ln; ++nroots; }
   if (nroots != 1) throw std::invalid_argument("URDF must have exactly one root link");
 
-  frames_.push_back(Frame{root_link, -1, Xform{}});
+  const auto with_com = [&](Frame f) {
+    for (int i = 0; i < 3; ++i) f.com[i] = link_com[f.name][i];
+    return f;
+  };
+  frames_.push_back(with_com(Frame{root_link, -1, Xform{}}));
   frame_index_[root_link] = 0;
   std::function<void(const std::string &)> visit = [&](const std::string &link) {
     for (const UJ *uj : children[link]) {
@@ -305,7 +313,7 @@ ArticulatedModel::ArticulatedModel(const std::string &urdf, const double gravity
       Xform place = xmul(pf.place, uj->origin);
       if (uj->type == "fixed") {
         frame_index_[uj->child] = (int)frames_.size();
-        frames_.push_back(Frame{uj->child, pf.joint, place});
+        frames_.push_back(with_com(Frame{uj->child, pf.joint, place}));
         if (pf.joint >= 0) joints_[pf.joint].inertia.add(link_inertia[uj->child].moved(place));
       } else {
         Joint j;
@@ -334,7 +342,7 @@ ArticulatedModel::ArticulatedModel(const std::string &urdf, const double gravity
         nv_ += j.nv;
         nq_ += j.nq;
         frame_index_[uj->child] = (int)frames_.size();
-        frames_.push_back(Frame{uj->child, id, Xform{}});
+        frames_.push_back(with_com(Frame{uj->child, id, Xform{}}));
       }
       visit(uj->child);
     }

@@ -49,6 +49,7 @@ struct Frame {
   std::string name;
   int joint = -1;  // joint the frame is attached to (-1 = universe)
   Xform place;     // joint frame -> link frame
+  double com[3] = {0, 0, 0};  // the link's own centre of mass in the link frame (its inertial origin)
 };
 
 class ArticulatedModel {

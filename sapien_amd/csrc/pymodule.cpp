@@ -414,6 +414,20 @@ class PyModel {
                                      vqdd ? vqdd->strides[0] : 0, fp(vP), vP ? vP->strides[0] : 0,
                                      vP ? vP->strides[1] : 0, fp(vJ), fp(vT), vT ? vT->strides[0] : 0, s));
   }
+  void link_state_into(py::object q, py::object qd, py::object body, py::object stream) {
+    sapamd_info i = info();
+    auto vq = take_strided(q, {-1, i.nv}, i.device);
+    const int64_t B = q.attr("shape").cast<std::vector<int64_t>>()[0];
+    std::unique_ptr<DLIn> vqd;
+    if (!qd.is_none()) vqd = take_strided(qd, {B, i.nv}, i.device);
+    auto vb = take_strided(body, {B, -1, -1}, i.device);
+    auto shp = body.attr("shape").cast<std::vector<int64_t>>();
+    if (shp[1] < i.nlinks || shp[2] < 13) throw py::value_error("body must have shape (B, >= nlinks, >= 13)");
+    void *s = stream.is_none() ? current_stream(i.device) : reinterpret_cast<void *>(stream.cast<intptr_t>());
+    check(sapamd_link_state_strided(m_, B, static_cast<float *>(vq->ptr), vq->strides[0],
+                                    vqd ? static_cast<float *>(vqd->ptr) : nullptr, vqd ? vqd->strides[0] : 0,
+                                    static_cast<float *>(vb->ptr), vb->strides[0], vb->strides[1], s));
+  }
   py::object passive_force_batched(py::object q, py::object qd, bool gravity, bool coriolis, py::object out,
                                    py::object stream) {
     sapamd_info i = info();
@@ -955,6 +969,12 @@ PYBIND11_MODULE(_pinocchio, m) {
            "Fused FK / WORLD Jacobian / RNEA into caller buffers, without copies: joint arrays (B, nv) with any "
            "row stride (e.g. [:, :nv] of a padded [n_art, max_dof] state), poses (B, >= L, >= 7) with the last "
            "dimension contiguous (e.g. a [n_art, max_links, 13] body-state buffer: p | q wxyz in the first 7).")
+      .def("compute_link_state_into", &PyModel::link_state_into, py::arg("qpos"), py::arg("qvel") = py::none(),
+           py::arg("body") = py::none(), py::arg("stream") = py::none(),
+           "Link poses and velocities into a GPU simulator's body-state buffer (B, >= L, >= 13) rows "
+           "p | q wxyz | v | w, world frame: row 0 (the root link) is read as the articulation's world pose and "
+           "velocity, then every link's row is written (v = centre-of-mass velocity); without qvel only the "
+           "poses. Joint arrays (B, nv) with any row stride.")
       .def("compute_passive_force_batched", &PyModel::passive_force_batched, py::arg("qpos"),
            py::arg("qvel") = py::none(), py::arg("gravity") = true, py::arg("coriolis_and_centrifugal") = true,
            py::arg("out") = py::none(), py::arg("stream") = py::none(),

@@ -165,10 +165,11 @@ template <class T> __device__ __forceinline__ void ldlt6_solve(T *A, T *y) {
     for (int k = i + 1; k < 6; ++k) y[i] -= A[k * 6 + i] * y[k];
 }
 
+// Target pose p | q (w x y z) -> R, p as Eigen::Quaterniond(w, x, y, z)
+// .toRotationMatrix() builds it (pinocchio_model.cpp:255): the quaternion is
+// not normalized, so a non-unit q gives (1 - |q|^2) I + |q|^2 R(q / |q|).
 template <class T> __device__ __forceinline__ void pose_to_mat(const T *P, T *R, V3<T> &p) {
-  T w = P[3], x = P[4], y = P[5], z = P[6];
-  T n = T(1) / sqrt(w * w + x * x + y * y + z * z);
-  w *= n; x *= n; y *= n; z *= n;
+  const T w = P[3], x = P[4], y = P[5], z = P[6];
   R[0] = T(1) - T(2) * (y * y + z * z); R[1] = T(2) * (x * y - w * z); R[2] = T(2) * (x * z + w * y);
   R[3] = T(2) * (x * y + w * z); R[4] = T(1) - T(2) * (x * x + z * z); R[5] = T(2) * (y * z - w * x);
   R[6] = T(2) * (x * z - w * y); R[7] = T(2) * (y * z + w * x); R[8] = T(1) - T(2) * (x * x + y * y);

@@ -44,6 +44,7 @@ template <class T> struct alignas(16) JointRec {
 
 template <class T> struct alignas(16) LinkRec {
   T R[9], p[3];   // link frame relative to its joint frame
+  T c[3];         // the link's own centre of mass in the link frame
   int32_t joint;  // -1 = universe (the root link)
   int32_t ident;  // 1 when R = I and p = 0
   int32_t pad[2];

@@ -222,3 +222,36 @@ def samples(rob, B, seed, dtype=np.float64):
     qdd = rng.normal(0.0, 2.0, size=(B, n))
     tau = rng.normal(0.0, math.sqrt(10.0), size=(B, n))
     return [x.astype(dtype) for x in (q, qd, qdd, tau)]
+
+
+# ---------------------------------------------------------------- workloads
+# The mixed batch: chain7, arm9, quad18 and torso30 in the ratio 4:3:2:1.
+MIX = (("chain7", 4), ("arm9", 3), ("quad18", 2), ("torso30", 1))
+
+
+def split_mix(total):
+    """total split 4:3:2:1 over the four trees (rounded, summing to total)."""
+    w = [k for _, k in MIX]
+    n = [round(total * k / sum(w)) for k in w]
+    n[0] += total - sum(n)
+    return n
+
+
+ROLLOUT_STEPS, ROLLOUT_CKPT, ROLLOUT_DT = 128, 16, 1.0 / 240.0
+
+
+def rollout_workload(rob, B, seed, T=ROLLOUT_STEPS, dtype=np.float32):
+    """Inputs of B rollouts: q0, qd0 from samples(rob, B, seed) and torques
+    tau (B, T, nv) ~ N(0, var 5) from default_rng(seed + 100)."""
+    q0, qd0, _, _ = samples(rob, B, seed, dtype=dtype)
+    tau = np.random.default_rng(seed + 100).normal(0.0, math.sqrt(5.0), size=(B, T, rob.nv)).astype(dtype)
+    return q0, qd0, tau
+
+
+def gradient_workload(rob, B, dtype=np.float32):
+    """Inputs of the pose + torque loss over B environments: (q, qd, qdd, tau*)
+    from samples(rob, B, 1003) and the configurations q* (seed 1013) whose
+    end-effector positions FK(q*) are the targets."""
+    q, qd, qdd, tau_star = samples(rob, B, 1003, dtype=dtype)
+    q_star = samples(rob, B, 1013, dtype=dtype)[0]
+    return q, qd, qdd, tau_star, q_star

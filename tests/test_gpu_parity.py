@@ -19,7 +19,8 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 from oracle.urdf_model import OracleModel  # noqa: E402
-from parity import POSE_TOL_M, POSE_TOL_RAD, REL_TOL, pose_errors, rel_err  # noqa: E402
+from parity import (POSE_TOL_M, POSE_TOL_RAD, REL_TOL, link_state_reference, pose_errors, quat_to_R,  # noqa: E402
+                    rel_err)
 from sapien_amd import synthetic as S  # noqa: E402
 
 TREES = list(S.ROBOTS)
@@ -614,6 +615,49 @@ def test_padded_simulator_buffers(tree):
         assert pos <= POSE_TOL_M and ang <= POSE_TOL_RAD
 
 
+@pytest.mark.parametrize("tree", TREES)
+def test_simulator_link_state(tree):
+    """sapamd_link_state_strided on a padded [n_art, max_links, 13] body-state
+    buffer (src/physx/physx_system.cpp:1048-1051) whose row 0 holds a moving,
+    non-identity root pose and velocity: every link's row receives the world
+    pose root * (model pose) and the centre-of-mass and angular velocities,
+    against the oracle's root-frame link poses and WORLD link Jacobians J q_dot
+    composed with the root motion; padding rows and a padded joint state are
+    left alone; without qvel only the poses change."""
+    rob = S.robot(tree)
+    m, o = _pair(rob)
+    B, n, L = 600, rob.nv, rob.nlinks
+    q, qd, _, _ = S.samples(rob, B, 1952)
+    rng = np.random.default_rng(1953)
+    max_dof, max_links = n + 3, L + 2
+    state = torch.full((2, B, max_dof), -3.0, device=DEV)
+    state[0, :, :n], state[1, :, :n] = _t(q), _t(qd)
+    root_q = rng.normal(size=(B, 4))
+    root_q /= np.linalg.norm(root_q, axis=1, keepdims=True)
+    root = np.concatenate([rng.uniform(-2, 2, (B, 3)), root_q, rng.normal(size=(B, 3)), rng.normal(size=(B, 3))], 1)
+    body = torch.full((B, max_links, 13), -5.0, device=DEV)
+    body[:, 0] = _t(root)
+    m.compute_link_state_into(state[0, :, :n], state[1, :, :n], body)
+    got = body.cpu().numpy().astype(np.float64)
+    assert bool((body[:, L:] == -5.0).all()) and bool((state[:, :, n:] == -3.0).all())
+    pw, Rw, vc, ww = link_state_reference(o, rob.urdf, rob.link_names, q, qd, root)
+    # row 0 keeps the root's pose and velocity
+    assert np.allclose(ww[:, 0], root[:, 10:13], atol=1e-12) and np.allclose(vc[:, 0], root[:, 7:10], atol=1e-12)
+    gq = got[:, :L, 3:7] / np.linalg.norm(got[:, :L, 3:7], axis=-1, keepdims=True)
+    assert np.abs(got[:, :L, :3] - pw).max() <= 4 * POSE_TOL_M
+    assert np.abs(quat_to_R(gq) - Rw).max() <= 4 * POSE_TOL_RAD
+    scale = np.abs(np.concatenate([vc, ww], -1)).reshape(B, -1).max(1)
+    verr = np.abs(np.concatenate([got[:, :L, 7:10] - vc, got[:, :L, 10:13] - ww], -1)).reshape(B, -1).max(1)
+    print(f"{tree}: link velocity rel err {(verr / scale).max():.2e}")
+    assert (verr / scale).max() <= REL_TOL
+    # poses only: the velocity slots keep their contents
+    body2 = torch.full((B, max_links, 13), -5.0, device=DEV)
+    body2[:, 0] = _t(root)
+    m.compute_link_state_into(state[0, :, :n], None, body2)
+    assert torch.equal(body2[:, :L, :7], body[:, :L, :7])
+    assert bool((body2[:, 1:L, 7:] == -5.0).all()) and torch.equal(body2[:, 0, 7:], _t(root)[:, 7:])
+
+
 def test_multistart_ik():
     """Multi-start batched IK (mplib's Planner.IK(target, init_qpos,
     n_init_qpos)): the answer is the lowest start that converges, identical to
@@ -650,6 +694,28 @@ def test_multistart_ik():
     assert not ok.cpu().numpy()[unreachable].any()
     assert ok.cpu().numpy()[~unreachable].mean() > 0.9
     assert oks[0].float().mean() <= ok.float().mean()  # extra starts never hurt
+    # against the FP64 oracle: every converged answer reaches its target in the
+    # oracle's FK, and on a 32-target sample the starts run in order through
+    # oracle.clik pick the same start with the same outcome
+    Qn, okn, startn = Q.cpu().numpy(), ok.cpu().numpy(), start.cpu().numpy()
+    reached = o.link_poses(Qn[okn].astype(np.float64))[:, link]
+    assert np.abs(reached[:, :3] - t[okn, :3]).max() < 2e-4
+    dots = np.abs((reached[:, 3:] * t[okn, 3:]).sum(1)).clip(0, 1)
+    assert (2 * np.arccos(dots)).max() < 2e-4
+    st = starts.cpu().numpy().astype(np.float64)
+    tt32 = tt.cpu().numpy().astype(np.float64)
+    for b in np.random.default_rng(1961).choice(B, 32, replace=False):
+        chosen, errs = None, []
+        for k in range(1 + n_init):
+            qr, okr, errr, _ = o.clik(link, tt32[b], q0=st[k, b], max_iterations=300)
+            errs.append(np.linalg.norm(errr))
+            if okr:
+                chosen = k
+                break
+        if chosen is None:
+            assert not okn[b], b
+        else:
+            assert okn[b] and startn[b] == chosen, (b, startn[b], chosen)
 
 
 def _robot_file(fname):

@@ -250,13 +250,26 @@ def test_rnea_matches_lagrangian(tree):
 
 
 def test_exported_layout_gives_identical_dynamics():
-    rob = S.torso30()
-    m1 = _model(rob)
-    m2 = _model(rob, rewrite(rob.urdf, seed=9))
-    q, qd, qdd, tau = S.samples(rob, 3, 18)
-    assert np.allclose(m1.rnea(q, qd, qdd), m2.rnea(q, qd, qdd), atol=1e-11)
-    assert np.allclose(m1.crba(q), m2.crba(q), atol=1e-12)
-    assert np.allclose(m1.jacobian_world(q), m2.jacobian_world(q), atol=1e-12)
+    """The exporter's layout (dummy links, x-axis joint frames, principal-axis
+    inertials) is the same mechanism: identical outputs to rounding when the
+    values are printed in full, and within the FP32 quantization of the
+    geometry (about 6e-8 relative per value) as the exporter prints them."""
+    from parity import rel_err
+
+    for tree in ("torso30", "arm9"):
+        rob = S.robot(tree)
+        m1 = _model(rob)
+        q, qd, qdd, tau = S.samples(rob, 3, 18)
+        m2 = _model(rob, rewrite(rob.urdf, seed=9, fp32=False))
+        assert np.allclose(m1.rnea(q, qd, qdd), m2.rnea(q, qd, qdd), atol=1e-11)
+        assert np.allclose(m1.crba(q), m2.crba(q), atol=1e-12)
+        assert np.allclose(m1.jacobian_world(q), m2.jacobian_world(q), atol=1e-12)
+        assert np.allclose(m1.link_poses(q)[..., :3], m2.link_poses(q)[..., :3], atol=1e-12)
+        m3 = _model(rob, rewrite(rob.urdf, seed=9))
+        assert rel_err(m3.rnea(q, qd, qdd), m1.rnea(q, qd, qdd)) < 1e-5
+        assert rel_err(m3.crba(q), m1.crba(q)) < 1e-5
+        assert rel_err(m3.jacobian_world(q), m1.jacobian_world(q)) < 1e-5
+        assert np.abs(m3.link_poses(q)[..., :3] - m1.link_poses(q)[..., :3]).max() < 1e-6
 
 
 def test_pendulum_closed_form():
@@ -367,6 +380,27 @@ def test_log6_inverts_exp6_and_jlog6_is_its_derivative():
             assert np.allclose((lp - lm) / (2 * h), J[:, k], atol=2e-6 if t > 3 else 1e-7), (t, k)
 
 
+def test_target_quaternion_is_not_normalized():
+    """The CLIK target is built as Eigen::Quaterniond(w, x, y, z).toRotationMatrix()
+    (pinocchio_model.cpp:255), which applies the unit-quaternion formula to q as
+    given: q = s q_hat gives R = (1 - s^2) I + s^2 R(q_hat). For q_hat a rotation
+    by phi about k that matrix has trace 3 - 2 s^2 (1 - cos phi) and skew part
+    s^2 sin(phi) [k]x, so the trace-based log3 returns
+    theta / sin(theta) * s^2 sin(phi) k with cos(theta) = 1 - s^2 (1 - cos phi).
+    A normalizing conversion would return phi k instead."""
+    k = np.array([0.3, -0.5, 0.81])
+    k /= np.linalg.norm(k)
+    for phi, s in [(0.7, 0.9975), (2.0, 1.02), (0.3, 1.0)]:
+        qv = s * np.r_[math.cos(phi / 2), math.sin(phi / 2) * k]
+        l, _ = log6(np.r_[0.0, 0.0, 0.0, qv], with_jac=False)
+        theta = math.acos(1 - s * s * (1 - math.cos(phi)))
+        ref = theta / math.sin(theta) * s * s * math.sin(phi) * k
+        assert np.allclose(l[3:], ref, atol=1e-13), (phi, s)
+        assert np.allclose(l[:3], 0, atol=1e-15)
+    # the reference's own panda target (manualtest/urdf.py:32-33) has |q|^2 = 0.9951
+    assert abs(np.linalg.norm([0.21, 0.73, 0.41, 0.5]) ** 2 - 0.9951) < 1e-12
+
+
 @pytest.mark.parametrize("tree", ["chain7", "arm9"])
 def test_clik_converges_to_fk_targets(tree):
     rob = S.robot(tree)
@@ -417,6 +451,55 @@ def test_rollout_jvp_matches_finite_differences():
     assert np.allclose((a[1] - b[1])[0] / (2 * h), jqd, atol=1e-6)
 
 
+def _slider_urdf(mass, axis="1 0 0"):
+    # one body on a prismatic joint; its axis is orthogonal to gravity (0, 0, -9.81)
+    return ('<robot name="slider"><link name="base"/>'
+            f'<link name="body"><inertial><origin xyz="0.1 0.2 0.3"/><mass value="{mass!r}"/>'
+            '<inertia ixx="0.01" iyy="0.02" izz="0.03" ixy="0" ixz="0" iyz="0"/></inertial></link>'
+            '<joint name="slide" type="prismatic"><parent link="base"/><child link="body"/>'
+            f'<origin xyz="0.5 -0.2 1.0"/><axis xyz="{axis}"/><limit lower="-10" upper="10"/></joint></robot>')
+
+
+def test_rollout_is_semi_implicit_euler_closed_form():
+    """A 1-dof prismatic body of mass m under a constant force F along an axis
+    orthogonal to gravity has qdd = F/m exactly, so T steps of semi-implicit
+    Euler (qd += dt qdd; then q += dt qd) give
+        qd_T = qd0 + T dt F/m,   q_T = q0 + T dt qd0 + dt^2 (F/m) T (T + 1) / 2.
+    Explicit Euler (q updated with the old qd) would give T (T - 1) / 2, and a
+    dropped or doubled dt changes both lines."""
+    mass, F, T, dt = 1.7, 3.1, 37, 1.0 / 240
+    for axis in ("1 0 0", "0 1 0", "0.6 0.8 0"):
+        m = OracleModel(_slider_urdf(mass, axis))
+        m.set_joint_order(["slide"])
+        m.set_link_order(["body"])
+        q0, qd0 = np.array([[0.25], [-0.4]]), np.array([[0.5], [-1.25]])
+        tau = np.full((2, T, 1), F)
+        qT, qdT = m.rollout(q0, qd0, tau, dt)
+        a = F / mass
+        assert np.allclose(qdT, qd0 + T * dt * a, rtol=0, atol=1e-13)
+        assert np.allclose(qT, q0 + T * dt * qd0 + dt * dt * a * T * (T + 1) / 2, rtol=0, atol=1e-13)
+        # qdd from the oracle's forward dynamics is exactly F/m (no gravity along the axis)
+        assert np.allclose(m.aba(q0, qd0, tau[:, 0]), a, rtol=1e-14)
+
+
+def test_rollout_equals_step_loop_over_forward_dynamics():
+    """oracle.rollout is T applications of qd += dt ABA(q, qd, tau_t), q += dt qd
+    over the oracle's ABA, which the ABA tests pin (RNEA round trip, dense solve)."""
+    for tree in ("chain7", "quad18"):
+        rob = S.robot(tree)
+        m = _model(rob)
+        q0, qd0, _, _ = S.samples(rob, 3, 31)
+        T, dt = 9, 1.0 / 240
+        tau = np.random.default_rng(32).normal(0, 5, size=(3, T, rob.nv))
+        qT, qdT = m.rollout(q0, qd0, tau, dt)
+        q, qd = q0.copy(), qd0.copy()
+        for t in range(T):
+            qd = qd + dt * m.aba(q, qd, tau[:, t])
+            q = q + dt * qd
+        assert np.allclose(qT, q, rtol=1e-12, atol=1e-12)
+        assert np.allclose(qdT, qd, rtol=1e-12, atol=1e-12)
+
+
 def test_error_semantics():
     rob = S.chain7()
     m = OracleModel(rob.urdf)
@@ -462,3 +545,44 @@ def test_real_robots_match_bruteforce(fname):
         Mb = br.mass_matrix(q[b])
         assert np.allclose(m.crba(q[b:b + 1])[0], Mb, rtol=1e-10, atol=1e-12)
         assert np.allclose(m.rnea(q[b:b + 1], z, z)[0], br.gravity_torque(q[b]), rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("tree", ["arm9", "quad18"])
+def test_link_state_reference_is_the_time_derivative_of_the_poses(tree):
+    """The link-state reference the simulator-adapter test uses: its
+    centre-of-mass and angular velocities are the time derivatives (central
+    differences) of its own world poses when q moves with qd and link 0 moves
+    rigidly with the given centre-of-mass and angular velocity."""
+    from scipy.spatial.transform import Rotation
+
+    from parity import link_coms, link_state_reference
+
+    rob = S.robot(tree)
+    o = _model(rob)
+    rng = np.random.default_rng(40)
+    B = 4
+    q, qd, _, _ = S.samples(rob, B, 41)
+    R0 = Rotation.random(B, random_state=42).as_matrix()
+    p0, vc0, w0 = rng.uniform(-1, 1, (B, 3)), rng.normal(size=(B, 3)), rng.normal(size=(B, 3))
+    c = link_coms(rob.urdf, rob.link_names)
+
+    def root_at(t):
+        Rt = Rotation.from_rotvec(t * w0).as_matrix() @ R0
+        com = p0 + np.einsum("bij,j->bi", R0, c[0]) + t * vc0
+        pt = com - np.einsum("bij,j->bi", Rt, c[0])
+        xyzw = Rotation.from_matrix(Rt).as_quat()
+        return np.concatenate([pt, xyzw[:, 3:], xyzw[:, :3], vc0, w0], 1)
+
+    pw, Rw, vc, ww = link_state_reference(o, rob.urdf, rob.link_names, q, qd, root_at(0.0))
+    h = 1e-6
+    pp, Rp, _, _ = link_state_reference(o, rob.urdf, rob.link_names, q + h * qd, qd, root_at(h))
+    pm, Rm, _, _ = link_state_reference(o, rob.urdf, rob.link_names, q - h * qd, qd, root_at(-h))
+    com_p = pp + np.einsum("blij,lj->bli", Rp, c)
+    com_m = pm + np.einsum("blij,lj->bli", Rm, c)
+    assert np.allclose((com_p - com_m) / (2 * h), vc, atol=1e-6)
+    dR = Rp @ np.swapaxes(Rm, -1, -2)  # exp(2 h [w])
+    w_fd = Rotation.from_matrix(dR.reshape(-1, 3, 3)).as_rotvec().reshape(B, -1, 3) / (2 * h)
+    assert np.allclose(w_fd, ww, atol=1e-6)
+    # and the world poses are root * (model pose of link 0)^-1 * model pose
+    assert np.allclose(pw[:, 0], root_at(0.0)[:, :3], atol=1e-12)
+    assert np.allclose(Rw[:, 0], R0, atol=1e-12)
