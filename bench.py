@@ -345,8 +345,8 @@ def main():
         q, qd, qdd, _ = S.samples(rob, B, 1001 + k, dtype=np.float32)
         sets.append(dict(q=torch.from_numpy(q).to(dev), qd=torch.from_numpy(qd).to(dev),
                          qdd=torch.from_numpy(qdd).to(dev),
-                         P=torch.empty(B, L, 7, device=dev), J=torch.empty(B, 6, n, device=dev),
-                         T=torch.empty(B, n, device=dev)))
+                         P=torch.zeros(B, L, 7, device=dev), J=torch.zeros(B, 6, n, device=dev),
+                         T=torch.zeros(B, n, device=dev)))  # outputs written once at set-up
     stream = torch.cuda.current_stream(dev)
     launch = sa.native_fk_jac_rnea(m)  # C-ABI launcher over raw device pointers
 

@@ -505,7 +505,8 @@ template <class S> V3<S> log3(const M3<S> &R, S &theta) {
       w(i, 0) = sval(x) > 0 ? sgn[i] * sqrt(x) : S(0.0);
     }
   } else {
-    S h = sval(theta) > 1e-12 ? theta / sin(theta) / S(2.0) : S(0.5);
+    // factor 1/2 below theta = eps^(1/4) = 2^-13, as Pinocchio 2.6's log3
+    S h = sval(theta) > 1.220703125e-4 ? theta / sin(theta) / S(2.0) : S(0.5);
     w = v3<S>(h * (R(2, 1) - R(1, 2)), h * (R(0, 2) - R(2, 0)), h * (R(1, 0) - R(0, 1)));
   }
   return w;
@@ -515,7 +516,7 @@ template <class S> V3<S> log3(const M3<S> &R, S &theta) {
 // dbeta = (d beta / dt) / t, with their Taylor expansions near zero.
 template <class S> void log_coeffs(S t, S &alpha, S &beta, S &dbeta) {
   S t2 = t * t;
-  if (sval(t) < 1e-4) {
+  if (sval(t) < 1.220703125e-4) {  // series below eps^(1/4) = 2^-13, as Pinocchio 2.6's log6
     alpha = S(1.0) - t2 / S(12.0) - t2 * t2 / S(720.0);
     beta = S(1.0) / S(12.0) + t2 / S(720.0);
     dbeta = S(1.0) / S(360.0);

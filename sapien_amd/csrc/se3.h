@@ -20,36 +20,32 @@ template <class T> __device__ __forceinline__ void vec_of(S6<T> x, T *v) {
 }
 
 // ---------------------------------------------------------------- SE(3) log for CLIK
-// Rotation log with theta = atan2(|axial|, (tr - 1) / 2), well conditioned at
-// every angle; near theta = pi the axis comes from the largest diagonal entry.
+// Rotation log in the form Pinocchio 2.6 evaluates it (the reference's
+// log6 / Jlog6 inside compute_inverse_kinematics): theta = acos((tr R - 1) / 2)
+// clipped to [0, pi]; below pi - 1e-2 the axial vector
+// theta / (2 sin theta) (R - R^T)^v, with the factor 1 below theta = 2^-13
+// (eps^(1/4)) in FP64 and 1 + theta^2 / 6 below 1e-3 in FP32; above, the axis
+// magnitudes from the diagonal, R_ii = cos theta + (1 - cos theta) k_i^2, with
+// the signs of the skew part. For a rotation any correct log agrees; this
+// form also gives the reference's error vector when R is not orthonormal (a
+// target built from a quaternion that is not unit length).
 template <class T> __device__ __forceinline__ V3<T> log3_dev(const T *R, T &theta) {
   const T PI = T(3.14159265358979323846);
-  T c = (R[0] + R[4] + R[8] - T(1)) * T(0.5);
-  V3<T> ax = {T(0.5) * (R[7] - R[5]), T(0.5) * (R[2] - R[6]), T(0.5) * (R[3] - R[1])};
-  T s = sqrt(dot(ax, ax));
-  theta = atan2(s, c);
+  const T tr = R[0] + R[4] + R[8];
+  theta = tr >= T(3) ? T(0) : tr <= T(-1) ? PI : acos((tr - T(1)) * T(0.5));
+  const V3<T> ax = {T(0.5) * (R[7] - R[5]), T(0.5) * (R[2] - R[6]), T(0.5) * (R[3] - R[1])};
   if (theta < PI - T(1e-2)) {
-    T k = s > T(1e-7) ? theta / s : T(1) + theta * theta / T(6);
+    const bool fp64 = sizeof(T) == 8;
+    const T k = theta > (fp64 ? T(1.220703125e-4) : T(1e-3)) ? theta / sin(theta)
+                : fp64 ? T(1) : T(1) + theta * theta / T(6);
     return k * ax;
   }
-  T omc = T(1) - c;
-  T d0 = R[0] - c, d1 = R[4] - c, d2 = R[8] - c;
-  V3<T> k;
-  if (d0 >= d1 && d0 >= d2) {
-    k.x = sqrt(fmax(d0 / omc, T(0)));
-    k.y = (R[1] + R[3]) / (T(2) * omc * k.x);
-    k.z = (R[2] + R[6]) / (T(2) * omc * k.x);
-  } else if (d1 >= d2) {
-    k.y = sqrt(fmax(d1 / omc, T(0)));
-    k.x = (R[1] + R[3]) / (T(2) * omc * k.y);
-    k.z = (R[5] + R[7]) / (T(2) * omc * k.y);
-  } else {
-    k.z = sqrt(fmax(d2 / omc, T(0)));
-    k.x = (R[2] + R[6]) / (T(2) * omc * k.z);
-    k.y = (R[5] + R[7]) / (T(2) * omc * k.z);
-  }
-  if (dot(k, ax) < T(0)) k = T(-1) * k;
-  return theta * k;
+  const T c = cos(theta);
+  const T b = theta * theta / (T(1) - c);
+  const T x0 = (R[0] - c) * b, x1 = (R[4] - c) * b, x2 = (R[8] - c) * b;
+  return {(R[7] > R[5] ? T(1) : T(-1)) * (x0 > T(0) ? sqrt(x0) : T(0)),
+          (R[2] > R[6] ? T(1) : T(-1)) * (x1 > T(0) ? sqrt(x1) : T(0)),
+          (R[3] > R[1] ? T(1) : T(-1)) * (x2 > T(0) ? sqrt(x2) : T(0))};
 }
 
 // alpha = (t/2) cot(t/2), beta = (1 - alpha) / t^2, dbeta = beta'(t) / t; series below 0.5.
@@ -122,8 +118,9 @@ template <class T> __device__ __forceinline__ void jlog6_dev(const T *R, V3<T> p
 
 // log6 of (R, p) and Jlog6 of its inverse (R^T, -R^T p) from one rotation log:
 // log3(R^T) = -log3(R) (the axial vector changes sign exactly, the angle and
-// the coefficients are the same), so this equals log6_dev(R, p) and
-// jlog6_dev(R^T, -R^T p) bit for bit.
+// the coefficients are the same; in the branch near pi the signs come from the
+// skew part, which also flips, unless two of its entries tie exactly), so this
+// equals log6_dev(R, p) and jlog6_dev(R^T, -R^T p) bit for bit.
 template <class T> __device__ __forceinline__ S6<T> log6_jlog6_inv(const T *R, V3<T> p, T *Jl) {
   T t;
   V3<T> w = log3_dev(R, t);
