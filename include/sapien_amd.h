@@ -192,6 +192,21 @@ int sapamd_clik_batched(const sapamd_model *m, int64_t B, int32_t link, const fl
                         const int32_t *mask, double eps, int32_t max_iterations, double dt, double damp,
                         float *q_out, int32_t *success, float *err, int32_t *iterations, void *stream);
 
+/* Allocation-free sapamd_clik_batched: the same arguments, kernels and results,
+ * but the queue counters and hand-over records come from a fixed scratch array
+ * in the library's device memory (about 11 MB per GPU, reserved when the
+ * library loads and held until the process ends) instead of memory the model
+ * allocates: the call makes no hipMalloc / hipFree and allocates nothing
+ * itself. No counterpart in the reference: for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops). At most 65536 problems per call
+ * (more fail with SAPAMD_INVALID_ARGUMENT). Every call on one GPU shares that
+ * GPU's scratch, so calls that overlap on one GPU (other streams, other
+ * threads) race; the counters are cleared by a fill kernel at each call. */
+int sapamd_clik_batched_noalloc(const sapamd_model *m, int64_t B, int32_t link, const float *target,
+                                const float *q0, const int32_t *mask, double eps, int32_t max_iterations, double dt,
+                                double damp, float *q_out, int32_t *success, float *err, int32_t *iterations,
+                                void *stream);
+
 /* Heterogeneous batch in ONE launch: models[i] (trees of 1 to 32 joints, at
  * most 8 models) each with B[i] environments and its own device buffers; per
  * model any of poses/J/tau/M may be NULL (arrays of pointers may be NULL to

@@ -1083,8 +1083,34 @@ int sapamd_hetero_batched(const sapamd_model *const *models, int32_t nmodels, co
 }
 
 
+// ---------------------------------------------------------------- allocation-free CLIK
+// Scratch memory of sapamd_clik_batched_noalloc: the four queue counters of a
+// CLIK launch, then one hand-over record per problem, for at most
+// kNoAllocClikProblems problems (about 11 MB per GPU, held until the process
+// ends). Host code never passes `scratch` itself to a kernel or copy: it is a
+// host-side stand-in; each device's copy is reached through the address that
+// hipGetSymbolAddress returns with that device current (noalloc_scratch).
+constexpr int64_t kNoAllocClikProblems = 65536;
+constexpr size_t kNoAllocCounterBytes = 256;
+static __device__ __align__(256) char scratch[kNoAllocCounterBytes + sizeof(ClikCont) * kNoAllocClikProblems];
+
 namespace {
 constexpr int kMaxDevices = 64;
+
+// This device's copy of `scratch`, looked up once per device.
+char *noalloc_scratch(int device) {
+  static std::mutex mu;
+  static char *ptr[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) throw std::runtime_error("device index out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[device]) {
+    DeviceGuard g(device);
+    void *p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ptr[device] = static_cast<char *>(p);
+  }
+  return ptr[device];
+}
 
 // Compute units of a device (queried once per device).
 int device_cus(int device) {
@@ -1216,11 +1242,14 @@ int sapamd_aba_batched(const sapamd_model *m, int64_t B, const float *q, const f
 
 namespace {
 // Batched CLIK over nstart x B problems (start-major); solved != nullptr turns
-// on the multi-start skip.
+// on the multi-start skip. With ws != nullptr the queue counters and hand-over
+// records are taken from ws (the allocation-free entry point's scratch: four
+// counters, then records from byte kNoAllocCounterBytes on) instead of the model.
 void clik_launch(const sapamd_model *m, int64_t B, int64_t nstart, int32_t link, const float *target,
                  const float *q0, const int32_t *mask, double eps, int32_t max_iterations, double dt, double damp,
-                 float *q_out, int32_t *success, float *err, int32_t *iterations, int32_t *solved, void *stream) {
-  int32_t *counters = m->d_counter;
+                 float *q_out, int32_t *success, float *err, int32_t *iterations, int32_t *solved, void *stream,
+                 char *ws = nullptr) {
+  int32_t *counters = ws ? reinterpret_cast<int32_t *>(ws) : m->d_counter;
   ClikParams P;
   P.link = link;
   P.chain = m->d_chain + m->chain_off[link];
@@ -1253,7 +1282,16 @@ void clik_launch(const sapamd_model *m, int64_t B, int64_t nstart, int32_t link,
     g.first = 0;
     A.nseg = 1;
     A.total = total;
-    m->clik_workspace(A, total, S(stream));
+    if (ws) {
+      A.counter = counters;
+      A.cont_count = counters + 1;
+      A.cont_queue = counters + 2;
+      A.cont_easy = counters + 3;
+      A.cont = reinterpret_cast<ClikCont *>(ws + kNoAllocCounterBytes);
+      HIP_OK(launch_fill_i32(counters, 0, 4, S(stream)));
+    } else {
+      m->clik_workspace(A, total, S(stream));
+    }
     if (solved) HIP_OK(launch_fill_i32(solved, INT32_MAX, B, S(stream)));
     HIP_OK(launch_clik_lanes(A, P.nchain, dev_cus, S(stream)));
     return;
@@ -1285,6 +1323,26 @@ int sapamd_clik_batched(const sapamd_model *m, int64_t B, int32_t link, const fl
     need(err, "err");
     clik_launch(m, B, 1, link, target, q0, mask, eps, max_iterations, dt, damp, q_out, success, err, iterations,
                 nullptr, stream);
+  });
+}
+
+int sapamd_clik_batched_noalloc(const sapamd_model *m, int64_t B, int32_t link, const float *target,
+                                const float *q0, const int32_t *mask, double eps, int32_t max_iterations, double dt,
+                                double damp, float *q_out, int32_t *success, float *err, int32_t *iterations,
+                                void *stream) {
+  return guarded([&] {
+    check_batched(m, B);
+    m->check_link(link);
+    if (B > kNoAllocClikProblems)
+      throw std::invalid_argument("sapamd_clik_batched_noalloc: at most 65536 problems per call");
+    if (B == 0) return;
+    need(target, "target");
+    need(q_out, "q_out");
+    need(success, "success");
+    need(err, "err");
+    char *ws = noalloc_scratch(m->device);
+    clik_launch(m, B, 1, link, target, q0, mask, eps, max_iterations, dt, damp, q_out, success, err, iterations,
+                nullptr, stream, ws);
   });
 }
 
