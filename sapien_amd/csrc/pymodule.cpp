@@ -299,6 +299,7 @@ class PyModel {
   // ---------------------------------------------------------------- batched (DLPack, FP32)
   py::object fk_jac_rnea_batched(py::object q, py::object qd, py::object qdd, bool poses, bool jac, bool rnea,
                                  py::object stream) {
+    StreamScope scope_(stream, info().device);
     Batch b(*this, q);
     py::object qdT, qddT, P = py::none(), J = py::none(), T = py::none();
     std::unique_ptr<DLIn> vqd, vqdd, vP, vJ, vT;
@@ -334,6 +335,7 @@ class PyModel {
     return fk_jac_rnea_batched(q, qd, qdd, false, false, true, stream).cast<py::tuple>()[2];
   }
   py::object link_jacobians_batched(py::object q, py::object stream) {
+    StreamScope scope_(stream, info().device);
     Batch b(*this, q);
     int L = info().nlinks;
     py::object Jw = b.empty({b.B, L, 6, b.n}), Jl = b.empty({b.B, L, 6, b.n});
@@ -343,6 +345,7 @@ class PyModel {
     return py::make_tuple(Jw, Jl);
   }
   py::object crba_batched(py::object q, py::object stream) {
+    StreamScope scope_(stream, info().device);
     Batch b(*this, q);
     py::object M = b.empty({b.B, b.n, b.n});
     auto vq = b.view(b.q);
@@ -351,6 +354,7 @@ class PyModel {
     return M;
   }
   py::object coriolis_batched(py::object q, py::object qd, py::object stream) {
+    StreamScope scope_(stream, info().device);
     Batch b(*this, q);
     py::object qdT = b.input(qd), C = b.empty({b.B, b.n, b.n});
     auto vq = b.view(b.q), vqd = b.view(qdT), vC = b.view(C);
@@ -358,6 +362,7 @@ class PyModel {
     return C;
   }
   py::object aba_batched(py::object q, py::object qd, py::object tau, py::object stream) {
+    StreamScope scope_(stream, info().device);
     Batch b(*this, q);
     py::object qdT = b.input(qd), tT = b.input(tau), A = b.empty({b.B, b.n});
     auto vq = b.view(b.q), vqd = b.view(qdT), vt = b.view(tT), vA = b.view(A);
@@ -366,6 +371,7 @@ class PyModel {
   }
   py::object clik_batched(int64_t link, py::object target, py::object q0, py::object mask, double eps,
                           int max_iterations, double dt, double damp, py::object stream) {
+    StreamScope scope_(stream, info().device);
     Batch b(*this, target, 7);
     py::object q0T = q0.is_none() ? py::none() : b.input(q0);
     py::object Q = b.empty({b.B, b.n}), E = b.empty({b.B, 6});
@@ -393,6 +399,7 @@ class PyModel {
   // stride, e.g. [:, :nv] of a [n_art, max_dof] buffer.
   void fk_jac_rnea_into(py::object q, py::object qd, py::object qdd, py::object poses, py::object J, py::object tau,
                         py::object stream) {
+    StreamScope scope_(stream, info().device);
     sapamd_info i = info();
     auto vq = take_strided(q, {-1, i.nv}, i.device);
     const int64_t B = q.attr("shape").cast<std::vector<int64_t>>()[0];
@@ -415,6 +422,7 @@ class PyModel {
                                      vP ? vP->strides[1] : 0, fp(vJ), fp(vT), vT ? vT->strides[0] : 0, s));
   }
   void link_state_into(py::object q, py::object qd, py::object body, py::object stream) {
+    StreamScope scope_(stream, info().device);
     sapamd_info i = info();
     auto vq = take_strided(q, {-1, i.nv}, i.device);
     const int64_t B = q.attr("shape").cast<std::vector<int64_t>>()[0];
@@ -430,6 +438,7 @@ class PyModel {
   }
   py::object passive_force_batched(py::object q, py::object qd, bool gravity, bool coriolis, py::object out,
                                    py::object stream) {
+    StreamScope scope_(stream, info().device);
     sapamd_info i = info();
     py::module_ torch = py::module_::import("torch");
     py::object dev = torch.attr("device")("cuda", i.device);
@@ -468,6 +477,7 @@ class PyModel {
   // (q, success, error, iterations) and the lowest converged start per target.
   py::tuple clik_multistart_raw(int64_t link, py::object target, py::object q0, int nstart, py::object mask,
                                 double eps, int max_iterations, double dt, double damp, py::object stream) {
+    StreamScope scope_(stream, info().device);
     Batch b(*this, target, 7);
     const int64_t total = b.B * nstart;
     py::object Q0 = b.torch.attr("as_tensor")(q0).attr("to")(py::arg("device") = b.dev,
@@ -496,6 +506,7 @@ class PyModel {
   // ---------------------------------------------------------------- adjoints (DLPack, FP32)
   py::tuple kin_vjp_batched(py::object q, py::object qd, py::object qdd, py::object gposes, py::object gtau,
                             py::object stream) {
+    StreamScope scope_(stream, info().device);
     Batch b(*this, q);
     const bool rnea = !gtau.is_none();
     py::object qdT, qddT, gT, gP;
@@ -529,6 +540,7 @@ class PyModel {
   }
   py::tuple pose_torque_loss_batched(py::object q, py::object qd, py::object qdd, py::object tau_star,
                                      int64_t ee_link, py::object p_target, py::object stream) {
+    StreamScope scope_(stream, info().device);
     Batch b(*this, q);
     const bool rnea = !tau_star.is_none();
     py::object qdT, qddT, tsT, ptT;
@@ -567,6 +579,7 @@ class PyModel {
   // (T, K) of a (B, T, nv) torque tensor and a checkpoint interval
   static int32_t ckpt_count(int64_t T, int64_t K) { return (int32_t)((T + K - 1) / K); }
   py::tuple rollout_batched(py::object q0, py::object qd0, py::object tau, double dt, int64_t K, py::object stream) {
+    StreamScope scope_(stream, info().device);
     if (K < 1) throw py::value_error("checkpoint_every must be >= 1");
     Batch b(*this, q0);
     py::object qdT = b.input(qd0);
@@ -584,6 +597,7 @@ class PyModel {
   }
   py::tuple rollout_backward_batched(py::object tau, py::object ckpt, double dt, py::object gq, py::object gqd,
                                      int64_t K, py::object stream) {
+    StreamScope scope_(stream, info().device);
     if (K < 1) throw py::value_error("checkpoint_every must be >= 1");
     Batch b(*this, gq);
     py::object aq = b.q.attr("clone")(), aqd = b.input(gqd).attr("clone")();
@@ -605,6 +619,7 @@ class PyModel {
   // In place on q, qd (contiguous float32 (B, nv) tensors on the model's GPU):
   // no copies, no allocation, so a caller can capture it in a CUDA/HIP graph.
   void rollout_step_(py::object q, py::object qd, py::object tau_t, double dt, py::object stream) {
+    StreamScope scope_(stream, info().device);
     Batch b(*this, q);
     for (const py::object *t : {&q, &qd, &tau_t}) {
       if (!py::isinstance(*t, b.torch.attr("Tensor")) || !t->attr("is_contiguous")().cast<bool>() ||
@@ -620,6 +635,7 @@ class PyModel {
 
   // Heterogeneous batch: several models, one launch (FK + J + RNEA + CRBA).
   static py::list hetero_batched(py::list models, py::list qs, py::object qds, py::object qdds, py::object stream) {
+    StreamScope scope_(stream, models.size() ? models[0].cast<PyModel &>().info().device : 0);
     const size_t n = models.size();
     if (qs.size() != n) throw py::value_error("one q batch per model");
     std::vector<std::unique_ptr<Batch>> batches;
@@ -673,6 +689,7 @@ class PyModel {
   static py::list clik_hetero_batched(py::list models, py::list links, py::list poses, py::object q0s,
                                       py::object masks, double eps, int max_iterations, double dt, double damp,
                                       py::object stream) {
+    StreamScope scope_(stream, models.size() ? models[0].cast<PyModel &>().info().device : 0);
     const size_t n = models.size();
     if (links.size() != n || poses.size() != n) throw py::value_error("one link and one pose batch per model");
     std::vector<std::unique_ptr<Batch>> batches;
@@ -751,6 +768,31 @@ class PyModel {
 
   // Normalizes batched inputs: any tensor / array-like becomes a contiguous
   // float32 tensor on the model's GPU, shape (B, ncols).
+  // With a stream given, a batched call stages its inputs (device and dtype
+  // conversion) and allocates its outputs with that stream as torch's current
+  // stream, so those copies precede the kernel on the same stream and the
+  // caching allocator ties the temporaries to it.
+  struct StreamScope {
+    py::object ctx;
+    StreamScope(const py::object &stream, int device) {
+      if (stream.is_none()) return;
+      py::module_ torch = py::module_::import("torch");
+      py::object ext = torch.attr("cuda").attr("ExternalStream")(
+          stream.cast<intptr_t>(), py::arg("device") = torch.attr("device")("cuda", device));
+      ctx = torch.attr("cuda").attr("stream")(ext);
+      ctx.attr("__enter__")();
+    }
+    ~StreamScope() {
+      if (!ctx) return;
+      try {
+        ctx.attr("__exit__")(py::none(), py::none(), py::none());
+      } catch (...) {
+      }
+    }
+    StreamScope(const StreamScope &) = delete;
+    StreamScope &operator=(const StreamScope &) = delete;
+  };
+
   struct Batch {
     PyModel &self;
     py::module_ torch;

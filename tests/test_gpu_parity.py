@@ -663,6 +663,32 @@ def test_simulator_link_state(tree):
     assert bool((body2[:, 1:L, 7:] == -5.0).all()) and torch.equal(body2[:, 0, 7:], _t(root)[:, 7:])
 
 
+def test_batched_calls_on_a_given_stream():
+    """With stream= given, a batched call converts its inputs and allocates its
+    outputs on that stream: while the current stream is held busy, FP64 device
+    inputs (which need a conversion) give the same bits as the plain call."""
+    rob = S.chain7()
+    m, o = _pair(rob)
+    q, qd, qdd, _ = S.samples(rob, 4096, 2300)
+    ref = m.compute_fk_jacobian_rnea_batched(_t(q), _t(qd), _t(qdd))
+    tgt = _t(o.link_poses(q)[:, 7])
+    ref_ik = m.compute_inverse_kinematics_batched(7, tgt, _t(q + 0.05))
+    q64, qd64, qdd64 = (torch.tensor(x, dtype=torch.float64, device=DEV) for x in (q, qd, qdd))
+    q064 = torch.tensor(q + 0.05, dtype=torch.float64, device=DEV)
+    tgt64 = tgt.double()
+    torch.cuda.synchronize()  # the caller's inputs are ready before the current stream is held
+    side = torch.cuda.Stream()
+    torch.cuda._sleep(20_000_000)  # a bounded spin keeping the current stream busy (milliseconds)
+    out = m.compute_fk_jacobian_rnea_batched(q64, qd64, qdd64, stream=side.cuda_stream)
+    out_ik = m.compute_inverse_kinematics_batched(7, tgt64, q064, stream=side.cuda_stream)
+    side.synchronize()
+    torch.cuda.synchronize()
+    for a, b in zip(out, ref):
+        assert torch.equal(a, b)
+    for a, b in zip(out_ik, ref_ik):
+        assert torch.equal(a, b)
+
+
 def test_multistart_ik():
     """Multi-start batched IK (mplib's Planner.IK(target, init_qpos,
     n_init_qpos)): the answer is the lowest start that converges, identical to
