@@ -97,6 +97,11 @@ int sapamd_compute_inverse_dynamics(sapamd_model *m, const double *qpos, const d
 /* compute_forward_dynamics: pinocchio_model.cpp:221-226 (ABA). */
 int sapamd_compute_forward_dynamics(sapamd_model *m, const double *qpos, const double *qvel, const double *qf,
                                     int64_t n, double *out);
+/* get_random_qpos: pinocchio_model.cpp:142-144 (getRandomConfiguration, user
+ * order): out (nv) uniform within the joint limits (continuous joints in
+ * [-pi, pi)), drawn on the GPU by the generator of sapamd_random_qpos_batched
+ * (row 0 of the batch for this seed). */
+int sapamd_get_random_qpos(sapamd_model *m, uint64_t seed, double *out);
 /* compute_inverse_kinematics: pinocchio_model.cpp:228-300. pose7 = p xyz | q wxyz;
  * initial_qpos may be NULL (n0 = 0: neutral); active_qmask may be NULL (nm = 0:
  * all active). Outputs q (nv), success, best error (6). */
@@ -159,6 +164,12 @@ int sapamd_fk_jac_rnea_strided(const sapamd_model *m, int64_t B, const float *q,
 int sapamd_link_state_strided(const sapamd_model *m, int64_t B, const float *q, int64_t q_ld, const float *qd,
                               int64_t qd_ld, float *body, int64_t body_env_stride, int64_t body_link_stride,
                               void *stream);
+/* Batched get_random_qpos (pinocchio_model.cpp:142-144,
+ * python/py_package/wrapper/pinocchio_model.py:113-114): out (B, nv), entry
+ * (b, u) uniform within the joint limits of user dof u (continuous joints in
+ * [-pi, pi)), from Philox4x32-10 at counter (b, u, 0, 0) under the key
+ * (seed low, seed high 32 bits): the same seed gives the same batch. */
+int sapamd_random_qpos_batched(const sapamd_model *m, int64_t B, uint64_t seed, float *out, void *stream);
 /* Passive joint forces, as PhysxArticulation::computePassiveForce
  * (src/physx/articulation.cpp:400-418): gravity compensation g(q) and/or
  * Coriolis and centrifugal forces C(q, qd) qd, i.e. RNEA(q, qd, 0) with the

@@ -972,6 +972,35 @@ int sapamd_link_state_strided(const sapamd_model *m, int64_t B, const float *q, 
   });
 }
 
+int sapamd_random_qpos_batched(const sapamd_model *m, int64_t B, uint64_t seed, float *out, void *stream) {
+  return guarded([&] {
+    check_batched(m, B);
+    if (B == 0) return;
+    need(out, "out");
+    const int nv = m->nv();
+    if (nv > kRandomMaxDofs) throw std::invalid_argument("random configurations: at most 64 dofs");
+    RandomLimits lim{};
+    if (sapamd_joint_limits(m, lim.lo, lim.hi) != SAPAMD_OK) throw std::runtime_error(sapamd_last_error());
+    HIP_OK(launch_random_qpos<float>(B, nv, seed, lim, out, S(stream)));
+  });
+}
+
+int sapamd_get_random_qpos(sapamd_model *m, uint64_t seed, double *out) {
+  return guarded([&] {
+    if (!m || !out) throw std::invalid_argument("null argument");
+    m->require_tables();
+    const int nv = m->nv();
+    if (nv > kRandomMaxDofs) throw std::invalid_argument("random configurations: at most 64 dofs");
+    RandomLimits lim{};
+    if (sapamd_joint_limits(m, lim.lo, lim.hi) != SAPAMD_OK) throw std::runtime_error(sapamd_last_error());
+    DeviceGuard g(m->device);
+    HIP_OK(launch_random_qpos<double>(1, nv, seed, lim, m->d_buf, m->stream));
+    HIP_OK(hipMemcpyAsync(m->h_buf, m->d_buf, sizeof(double) * nv, hipMemcpyDeviceToHost, m->stream));
+    HIP_OK(hipStreamSynchronize(m->stream));
+    for (int u = 0; u < nv; ++u) out[u] = m->h_buf[u];
+  });
+}
+
 int sapamd_passive_force_batched(const sapamd_model *m, int64_t B, const float *q, int64_t q_ld, const float *qd,
                                  int64_t qd_ld, int32_t gravity, int32_t coriolis_and_centrifugal, float *tau,
                                  int64_t tau_ld, void *stream) {

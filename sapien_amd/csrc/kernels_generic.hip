@@ -582,6 +582,49 @@ inline dim3 grid_for(int64_t B) { return dim3((unsigned)((B + 63) / 64)); }
 
 }  // namespace
 
+namespace {
+// Philox4x32-10 (Salmon, Moraes, Dror and Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC 2011): ten rounds of two 32 x 32 -> 64-bit products,
+// the key bumped by the Weyl constants between rounds.
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c.x, p1 = (uint64_t)0xCD9E8D57u * c.z;
+    c = make_uint4((uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1, (uint32_t)p0);
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// Entry (b, u) of a batch of random configurations: uniform in [lo_u, hi_u)
+// from the 53-bit fraction of Philox4x32-10 at counter (b, u) under the seed.
+template <class T>
+__global__ void k_random_qpos(int64_t B, int nv, uint64_t seed, RandomLimits lim, T *__restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= B * nv) return;
+  const int64_t b = i / nv;
+  const int u = (int)(i - b * nv);
+  const uint4 r = philox4x32_10(make_uint4((uint32_t)b, (uint32_t)((uint64_t)b >> 32), (uint32_t)u, 0u),
+                                (uint32_t)seed, (uint32_t)(seed >> 32));
+  const double f = ((double)(r.x >> 5) * 67108864.0 + (double)(r.y >> 6)) * (1.0 / 9007199254740992.0);
+  // separately rounded product and sum (no contraction into an FMA), so the
+  // value is reproducible by any IEEE double evaluation of the same formula
+  out[i] = (T)__dadd_rn(lim.lo[u], __dmul_rn(lim.hi[u] - lim.lo[u], f));
+}
+}  // namespace
+
+template <class T>
+hipError_t launch_random_qpos(int64_t B, int nv, uint64_t seed, const RandomLimits &lim, T *out, hipStream_t s) {
+  if (B <= 0 || nv <= 0) return hipSuccess;
+  if (nv > kRandomMaxDofs) return hipErrorInvalidValue;
+  const int64_t n = B * nv;
+  hipLaunchKernelGGL(k_random_qpos<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, B, nv, seed, lim, out);
+  return hipGetLastError();
+}
+template hipError_t launch_random_qpos<float>(int64_t, int, uint64_t, const RandomLimits &, float *, hipStream_t);
+template hipError_t launch_random_qpos<double>(int64_t, int, uint64_t, const RandomLimits &, double *, hipStream_t);
+
 hipError_t launch_fill_i32(int32_t *dst, int32_t value, int64_t n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, value, n);

@@ -235,5 +235,13 @@ hipError_t launch_restore_state(int64_t B, int nv, const float *src, int64_t str
 
 // dst[0:n] = value (a kernel, not hipMemsetAsync, so it is safe under graph capture)
 hipError_t launch_fill_i32(int32_t *dst, int32_t value, int64_t n, hipStream_t s);
+// B random configurations (B, nv), entry (b, u) uniform in [lo[u], hi[u]) from
+// Philox4x32-10 at counter (b, u) under the 64-bit seed.
+constexpr int kRandomMaxDofs = 64;
+struct RandomLimits {
+  double lo[kRandomMaxDofs], hi[kRandomMaxDofs];
+};
+template <class T>
+hipError_t launch_random_qpos(int64_t B, int nv, uint64_t seed, const RandomLimits &lim, T *out, hipStream_t s);
 
 }  // namespace sapamd

@@ -281,12 +281,25 @@ class PyModel {
                                             max_iterations, dt, damp, q.mutable_data(), &ok, err.mutable_data()));
     return py::make_tuple(q, bool(ok), err);
   }
+  // A fresh seed per call from the model's own generator (seeded from the
+  // system's entropy, as the reference's unseeded randomConfiguration draws),
+  // the values drawn on the GPU.
   py::array_t<double> get_random_qpos() {
-    int n = nv();
-    std::vector<double> lo(n), hi(n);
-    check(sapamd_joint_limits(m_, lo.data(), hi.data()));
-    py::array_t<double> out(n);
-    for (int i = 0; i < n; ++i) out.mutable_data()[i] = std::uniform_real_distribution<double>(lo[i], hi[i])(rng_);
+    py::array_t<double> out(nv());
+    check(sapamd_get_random_qpos(m_, rng_(), out.mutable_data()));
+    return out;
+  }
+  py::object random_qpos_batched(int64_t B, py::object seed, py::object stream) {
+    StreamScope scope_(stream, info().device);
+    if (B < 0) throw py::value_error("negative batch size");
+    py::module_ torch = py::module_::import("torch");
+    sapamd_info i = info();
+    py::object out = torch.attr("empty")(py::make_tuple(B, i.nv), py::arg("device") = torch.attr("device")("cuda", i.device),
+                                         py::arg("dtype") = torch.attr("float32"));
+    const uint64_t sd = seed.is_none() ? rng_() : seed.cast<uint64_t>();
+    auto vo = take(out, dlabi::kFloat, {B, i.nv}, i.device);
+    void *s = stream.is_none() ? current_stream(i.device) : reinterpret_cast<void *>(stream.cast<intptr_t>());
+    check(sapamd_random_qpos_batched(m_, B, sd, static_cast<float *>(vo->ptr), s));
     return out;
   }
   py::tuple joint_limits() {
@@ -965,6 +978,10 @@ PYBIND11_MODULE(_pinocchio, m) {
       .def("compute_single_link_local_jacobian", &PyModel::compute_single_link_local_jacobian,
            "Compute the link(body) Jacobian for a single link.", py::arg("qpos"), py::arg("link_index"))
       .def("get_random_qpos", &PyModel::get_random_qpos)
+      .def("get_random_qpos_batched", &PyModel::random_qpos_batched, py::arg("batch_size"),
+           py::arg("seed") = py::none(), py::arg("stream") = py::none(),
+           "(B, nv) FP32 configurations uniform within the joint limits (continuous joints in [-pi, pi)), "
+           "Philox4x32-10 at counter (b, u) under the seed: the same seed gives the same batch.")
       .def("joint_limits", &PyModel::joint_limits)
       .def_property_readonly("nv", [](const PyModel &s) { return s.info().nv; })
       .def_property_readonly("nq", [](const PyModel &s) { return s.info().nq; })

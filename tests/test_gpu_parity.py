@@ -852,3 +852,30 @@ def test_clik_mixed_workload_size():
         assert np.abs(reached - t.cpu().numpy()[idx, :3]).max() < 2e-4, rob.name
         total_ok += okn.sum()
     assert total_ok / sum(counts) > 0.9
+
+
+@pytest.mark.parametrize("tree", ["chain7", "torso30"])
+def test_random_qpos(tree):
+    """get_random_qpos_batched: the numpy Philox4x32-10 restatement's values
+    (FP32-rounded, bit for bit), the same batch for the same seed, every
+    entry within the joint limits (torso30's continuous wheels in [-pi, pi)),
+    moments of a uniform distribution; get_random_qpos (per call, FP64 on the
+    GPU) within the limits. The reference's own values come from Pinocchio's
+    generator and are not reproducible without it."""
+    from philox import random_qpos
+
+    rob = S.robot(tree)
+    m, _ = _pair(rob)
+    lo, hi = m.joint_limits()
+    B = 20000
+    a = m.get_random_qpos_batched(B, seed=12345).cpu().numpy()
+    ref = random_qpos(B, lo, hi, 12345).astype(np.float32)
+    assert np.array_equal(a, ref)
+    assert torch.equal(m.get_random_qpos_batched(B, seed=12345), m.get_random_qpos_batched(B, seed=12345))
+    assert not np.array_equal(m.get_random_qpos_batched(16, seed=7).cpu().numpy(), a[:16])
+    assert ((a >= lo.astype(np.float32)) & (a <= hi.astype(np.float32))).all()
+    u = (a - lo) / (hi - lo)  # uniform on [0, 1): mean 1/2, variance 1/12
+    assert np.abs(u.mean(0) - 0.5).max() < 0.01 and np.abs(u.var(0) - 1 / 12).max() < 0.005
+    for _ in range(5):
+        q = m.get_random_qpos()
+        assert q.dtype == np.float64 and q.shape == (rob.nv,) and ((q >= lo) & (q <= hi)).all()

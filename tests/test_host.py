@@ -43,8 +43,6 @@ def test_model_sizes_match_oracle(name):
     m.set_link_order(rob.link_names)
     lo, hi = m.joint_limits()
     assert np.allclose(lo, rob.lower) and np.allclose(hi, rob.upper)
-    q = m.get_random_qpos()
-    assert ((q >= lo) & (q <= hi)).all()
 
 
 def test_fixtures_parse():
@@ -156,3 +154,16 @@ def test_condition_estimate_matches_oracle_mass_matrices():
     assert sa.dynamics_precision(m)[0] == "fp32"
     with pytest.raises(ValueError):
         sa.set_dynamics_precision(m, "fp16")
+
+
+def test_philox_known_answers():
+    """The numpy Philox4x32-10 the random-configuration tests check against,
+    pinned by the generator's published known-answer vectors (Random123)."""
+    from philox import philox4x32_10
+
+    kats = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+            ((0xffffffff,) * 4, (0xffffffff, 0xffffffff), (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+            ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+             (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+    for c, k, ref in kats:
+        assert tuple(int(x) for x in philox4x32_10(*c, *k)) == ref
