@@ -169,6 +169,8 @@ def mixed_workloads(sa, S, np, torch, dev):
     rollouts x 128 steps (dt = 1/240, checkpoint every 16) forward and backward."""
     import time as _t
 
+    from sapien_amd import tune
+
     MIX, _split = S.MIX, S.split_mix
     models, robs = [], []
     for name, _ in MIX:
@@ -197,6 +199,9 @@ def mixed_workloads(sa, S, np, torch, dev):
         qs.append(torch.from_numpy(q).to(dev))
         qds.append(torch.from_numpy(qd).to(dev))
         qdds.append(torch.from_numpy(qdd).to(dev))
+    # lanes per environment of every tree's segment, autotuned over this launch
+    hrep = []
+    hshape, _ = tune.autotune_hetero(models, nb, use_cache=False, report=hrep)
     sec = timed(lambda: sa.hetero_batched(models, qs, qds, qdds), 50)
     # the same launch through the C ABI over preallocated outputs (poses, J,
     # tau, full symmetric M), HIP events on the launch stream
@@ -227,6 +232,9 @@ def mixed_workloads(sa, S, np, torch, dev):
         "bytes_per_launch": hbytes, "achieved_gbs": hbytes / (kms * 1e-3) / 1e9,
         "frac": hbytes / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "python_api_ms": sec * 1e3,
+        "shapes": {name: {"lanes_per_env": hshape[i], "envs_per_wavefront": 64 // hshape[i]}
+                   for i, (name, _) in enumerate(MIX)},
+        "autotuned": True, "tuning": hrep,
         "note": "one launch through the C ABI over preallocated outputs (M written in full, the byte count takes "
                 "its upper triangle); python_api_ms = the Python method, outputs allocated per call"}
 
@@ -237,6 +245,9 @@ def mixed_workloads(sa, S, np, torch, dev):
         q = torch.from_numpy(S.samples(rob, b, 1004 + i, dtype=np.float32)[0]).to(dev)
         tg.append(m.compute_forward_kinematics_batched(q)[:, rob.ee_link].contiguous())
     links = [rob.ee_link for rob in robs]
+    crep = []
+    sched, _ = tune.autotune_clik(lambda: sa.clik_hetero_batched(models, links, tg), "mixed65536",
+                                  use_cache=False, report=crep)
     outs = sa.clik_hetero_batched(models, links, tg)
     torch.cuda.synchronize()
     t0 = _t.perf_counter()
@@ -247,8 +258,13 @@ def mixed_workloads(sa, S, np, torch, dev):
     iters = float(sum(o[3].sum().item() for o in outs)) / 65536
     res["clik_mixed"] = {"problems": nc, "problems_s": 65536 / sec, "ms": sec * 1e3, "success_rate": succ,
                          "mean_iterations": iters, "max_iterations": 1000,
-                         "kernel": "lanes per joint, two persistent launches: 4 lanes x 4 chain joints per problem "
-                                   "for the first 128 iterations, then the problems still running at 8 lanes x 2 joints"}
+                         "schedule": {"phase1_iterations": sched[0], "phase1_lanes_per_problem": sched[1],
+                                      "phase1_problems_per_wavefront": 64 // sched[1],
+                                      "phase2_lanes_per_problem": sched[2] if sched[0] else None,
+                                      "autotuned": True},
+                         "tuning": crep,
+                         "kernel": "lanes per joint, persistent launches with per-problem early exit: the first runs "
+                                   "the phase-1 iterations, the second the problems still running"}
 
     # rollouts: 4096 split 4:3:2:1, 128 steps, tau ~ N(0, var 5), forward and backward
     nr = _split(4096)

@@ -131,6 +131,22 @@ int sapamd_set_chain_shape(sapamd_model *m, int64_t max_batch, int32_t G, int32_
 /* The shape a batch of B environments would launch with. */
 int sapamd_get_chain_shape(const sapamd_model *m, int64_t B, int32_t *G, int32_t *C);
 
+/* Lanes per environment of the lanes-per-joint tree kernels (batched
+ * kinematics of trees that are not short chains, and this model's segment of
+ * a heterogeneous launch): 8, 16 or 32, at least the joint count; 0 = the
+ * smallest that holds the joints (the autotuner's setting; no reference
+ * counterpart). */
+int sapamd_set_tree_group(sapamd_model *m, int32_t G);
+int sapamd_get_tree_group(const sapamd_model *m, int32_t *G);
+/* Schedule of the lanes-per-joint CLIK launches, library-wide so that a
+ * problem gives the same bits in a per-model and a mixed-tree launch: the
+ * first launch runs iterations [0, cap) at g1 lanes x (16 / g1) chain joints
+ * per problem, the second the problems still running at g2 lanes; cap = 0 runs
+ * every problem to its end in the first (g1, g2 in {4, 8, 16}; default 128, 4,
+ * 8). The autotuner's setting; no reference counterpart. */
+int sapamd_set_clik_schedule(int32_t cap, int32_t g1, int32_t g2);
+int sapamd_get_clik_schedule(int32_t *cap, int32_t *g1, int32_t *g2);
+
 /* Fused FK (every link pose) + full WORLD Jacobian + RNEA over B environments;
  * the throughput-critical entry point. Any of poses/J/tau may be NULL to skip it. */
 int sapamd_fk_jac_rnea_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd,

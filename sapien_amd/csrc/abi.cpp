@@ -145,6 +145,10 @@ struct sapamd_model {
   // Rollouts under the lanes-per-joint dynamics: one launch for all steps
   // (default), or one launch per step (the path graph-captured step loops use).
   bool per_step_rollouts = false;
+  // Lanes per environment of the lanes-per-joint tree kernels (0 = the
+  // smallest of 8, 16, 32 that holds the joints; the autotuner may pick more).
+  int tree_G = 0;
+  int tree_lanes() const { return tree_G ? tree_G : tree_group(host->njoints()); }
 
   ~sapamd_model() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -880,6 +884,23 @@ int sapamd_set_chain_shape(sapamd_model *m, int64_t max_batch, int32_t G, int32_
   });
 }
 
+int sapamd_set_tree_group(sapamd_model *m, int32_t G) {
+  return guarded([&] {
+    if (!m) throw std::invalid_argument("null model");
+    const int nj = m->host->njoints();
+    if (G != 0 && ((G != 8 && G != 16 && G != 32) || G < nj))
+      throw std::invalid_argument("tree group: 8, 16 or 32 lanes, at least the joint count (0 = default)");
+    m->tree_G = G;
+  });
+}
+
+int sapamd_get_tree_group(const sapamd_model *m, int32_t *G) {
+  return guarded([&] {
+    if (!m || !G) throw std::invalid_argument("null argument");
+    *G = m->tree_lanes();
+  });
+}
+
 int sapamd_get_chain_shape(const sapamd_model *m, int64_t B, int32_t *G, int32_t *C) {
   return guarded([&] {
     int g, c;
@@ -900,7 +921,7 @@ void fused_launch(const sapamd_model *m, int64_t B, int flags, const float *q, c
       m->shape_for(B, &G, &C);
       HIP_OK(launch_chain_fused(m->t32(), m->host->njoints(), B, flags, q, qd, qdd, o, S(stream), G, C));
     } else {
-      HIP_OK(launch_tree_fused(m->t32(), m->host->njoints(), B, flags, q, qd, qdd, o, S(stream)));
+      HIP_OK(launch_tree_fused(m->t32(), m->host->njoints(), B, flags, q, qd, qdd, o, S(stream), m->tree_lanes()));
     }
   } else {
     // the generic kernel reads qd / qdd whenever RNEA runs: alias the unused ones to q
@@ -1083,8 +1104,9 @@ int sapamd_hetero_batched(const sapamd_model *const *models, int32_t nmodels, co
     for (int i = 0; i < nmodels; ++i) {
       const sapamd_model *m = models[i];
       check_batched(m, B[i]);
-      const int nj = m->host->njoints(), G = tree_group(nj);
-      if (!G) throw std::runtime_error("heterogeneous launches need trees of 1 to 32 joints");
+      const int nj = m->host->njoints();
+      if (!tree_group(nj)) throw std::runtime_error("heterogeneous launches need trees of 1 to 32 joints");
+      const int G = m->tree_lanes();
       if (B[i] == 0) continue;
       need(q[i], "q");
       if (tau && tau[i]) {
@@ -1111,6 +1133,40 @@ int sapamd_hetero_batched(const sapamd_model *const *models, int32_t nmodels, co
   });
 }
 
+
+// ---------------------------------------------------------------- CLIK schedule
+// Library-wide (not per model), so a problem gives the same bits in a
+// per-model and in a mixed-tree launch.
+namespace {
+std::mutex g_clik_mu;
+ClikSchedule g_clik_schedule;
+ClikSchedule clik_schedule() {
+  std::lock_guard<std::mutex> lock(g_clik_mu);
+  return g_clik_schedule;
+}
+}  // namespace
+
+int sapamd_set_clik_schedule(int32_t cap, int32_t g1, int32_t g2) {
+  return guarded([&] {
+    ClikSchedule sch;
+    sch.cap = cap;
+    sch.g1 = g1;
+    sch.g2 = g2;
+    if (cap == 0 && (g2 != 4 && g2 != 8 && g2 != 16)) sch.g2 = 8;
+    if (!clik_schedule_valid(sch)) throw std::invalid_argument("CLIK schedule: cap >= 0, g1 and g2 in {4, 8, 16}");
+    std::lock_guard<std::mutex> lock(g_clik_mu);
+    g_clik_schedule = sch;
+  });
+}
+
+int sapamd_get_clik_schedule(int32_t *cap, int32_t *g1, int32_t *g2) {
+  return guarded([&] {
+    const ClikSchedule sch = clik_schedule();
+    if (cap) *cap = sch.cap;
+    if (g1) *g1 = sch.g1;
+    if (g2) *g2 = sch.g2;
+  });
+}
 
 // ---------------------------------------------------------------- allocation-free CLIK
 // Scratch memory of sapamd_clik_batched_noalloc: the four queue counters of a
@@ -1219,7 +1275,7 @@ int sapamd_clik_hetero_batched(const sapamd_model *const *models, int32_t nmodel
     for (int i = 0; i < nmodels; ++i) lanes = lanes && models[i]->variant != SAPAMD_KERNEL_THREAD_PER_ENV;
     if (lanes) {
       models[0]->clik_workspace(A, A.total, S(stream));
-      HIP_OK(launch_clik_lanes(A, max_chain, cus, S(stream)));
+      HIP_OK(launch_clik_lanes(A, max_chain, cus, clik_schedule(), S(stream)));
       return;
     }
     HIP_OK(launch_fill_i32(A.counter, 0, 1, S(stream)));
@@ -1322,7 +1378,7 @@ void clik_launch(const sapamd_model *m, int64_t B, int64_t nstart, int32_t link,
       m->clik_workspace(A, total, S(stream));
     }
     if (solved) HIP_OK(launch_fill_i32(solved, INT32_MAX, B, S(stream)));
-    HIP_OK(launch_clik_lanes(A, P.nchain, dev_cus, S(stream)));
+    HIP_OK(launch_clik_lanes(A, P.nchain, dev_cus, clik_schedule(), S(stream)));
     return;
   }
   const int64_t waves = (total + 63) / 64;

@@ -446,26 +446,42 @@ template <int G, int C, bool CONT> __global__ __launch_bounds__(256) void k_clik
 
 int clik_lanes_group(int max_chain) { return max_chain <= kClikMaxChain ? 4 : 0; }
 
-hipError_t launch_clik_lanes(const HeteroClikArgs &A, int max_chain, int cus, hipStream_t s) {
+bool clik_schedule_valid(const ClikSchedule &sch) {
+  const auto ok = [](int g) { return g == 4 || g == 8 || g == 16; };
+  return sch.cap >= 0 && ok(sch.g1) && (sch.cap == 0 || ok(sch.g2));
+}
+
+hipError_t launch_clik_lanes(const HeteroClikArgs &A, int max_chain, int cus, const ClikSchedule &sch,
+                             hipStream_t s) {
   if (A.total <= 0) return hipSuccess;
   if (!clik_lanes_group(max_chain) || A.total > INT32_MAX || !A.cont || !A.cont_count || !A.cont_queue ||
-      !A.cont_easy)
+      !A.cont_easy || !clik_schedule_valid(sch))
     return hipErrorNotSupported;
-  // The same two shapes for every chain length, so a problem gives the same
-  // bits in a per-model launch and in a mixed-tree launch. The first phase
-  // packs 16 problems per wavefront (throughput); the problems left after
-  // kClikPhase1Iters iterations, the slow and the unreachable ones, run 8 per
+  // The same schedule for every chain length, so a problem gives the same bits
+  // in a per-model launch and in a mixed-tree launch. By default the first
+  // phase packs 16 problems per wavefront (throughput); the problems left
+  // after its cap of iterations, the slow and the unreachable ones, run 8 per
   // wavefront with half the joints per lane (a shorter iteration).
   // Persistent grids, fewer blocks for small batches: the first phase two
   // blocks per CU (two wavefronts per SIMD), the second one block per CU, so
   // the long problems it starts first keep a SIMD each.
   HeteroClikArgs A1 = A;
-  A1.cap = kClikPhase1Iters;
+  A1.cap = sch.cap;
   const auto grid = [&](int per_block, int per_cu) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(per_cu * cus, (A.total + per_block - 1) / per_block));
   };
-  hipLaunchKernelGGL((k_clik_lanes<4, 4, false>), dim3(grid(256 / 4, 2)), dim3(256), 0, s, A1);
-  hipLaunchKernelGGL((k_clik_lanes<8, 2, true>), dim3(grid(256 / 8, 1)), dim3(256), 0, s, A1);
+  switch (sch.g1) {
+    case 4: hipLaunchKernelGGL((k_clik_lanes<4, 4, false>), dim3(grid(256 / 4, 2)), dim3(256), 0, s, A1); break;
+    case 8: hipLaunchKernelGGL((k_clik_lanes<8, 2, false>), dim3(grid(256 / 8, 2)), dim3(256), 0, s, A1); break;
+    default: hipLaunchKernelGGL((k_clik_lanes<16, 1, false>), dim3(grid(256 / 16, 2)), dim3(256), 0, s, A1); break;
+  }
+  if (sch.cap > 0) {
+    switch (sch.g2) {
+      case 4: hipLaunchKernelGGL((k_clik_lanes<4, 4, true>), dim3(grid(256 / 4, 1)), dim3(256), 0, s, A1); break;
+      case 8: hipLaunchKernelGGL((k_clik_lanes<8, 2, true>), dim3(grid(256 / 8, 1)), dim3(256), 0, s, A1); break;
+      default: hipLaunchKernelGGL((k_clik_lanes<16, 1, true>), dim3(grid(256 / 16, 1)), dim3(256), 0, s, A1); break;
+    }
+  }
   return hipGetLastError();
 }
 

@@ -472,10 +472,11 @@ __global__ __launch_bounds__(256) void k_hetero(HeteroArgs args, int flags) {
 int tree_group(int nj) { return nj < 1 ? 0 : nj <= 8 ? 8 : nj <= 16 ? 16 : nj <= 32 ? 32 : 0; }
 
 hipError_t launch_tree_fused(const ModelTable<float> *tab, int nj, int64_t B, int flags, const float *q,
-                             const float *qd, const float *qdd, const KinOut<float> &out, hipStream_t s) {
+                             const float *qd, const float *qdd, const KinOut<float> &out, hipStream_t s, int G) {
   if (B <= 0) return hipSuccess;
-  const int G = tree_group(nj);
-  if (!G) return hipErrorNotSupported;
+  if (!tree_group(nj)) return hipErrorNotSupported;
+  if (G == 0) G = tree_group(nj);
+  if ((G != 8 && G != 16 && G != 32) || G < nj) return hipErrorInvalidValue;
   const bool use_qd = (flags & KF_RNEA) && !(flags & KF_QD0) && qd, use_qdd = (flags & KF_RNEA) && !(flags & KF_QDD0) && qdd;
   TreeIO io{q, use_qd ? qd : q, use_qdd ? qdd : q, out.poses, out.jfull, out.tau, out.M};
   io.ld_q = out.ld_q;
@@ -527,8 +528,11 @@ hipError_t launch_tree_vjp(const ModelTable<float> *tab, int nj, int64_t B, cons
 
 hipError_t launch_hetero(const HeteroArgs &args, int64_t total_blocks, int flags, hipStream_t s) {
   if (total_blocks <= 0) return hipSuccess;
-  for (int i = 0; i < args.nseg; ++i)
-    if (!tree_group(args.seg[i].nj) || args.seg[i].G != tree_group(args.seg[i].nj)) return hipErrorInvalidValue;
+  for (int i = 0; i < args.nseg; ++i) {
+    const int G = args.seg[i].G;
+    if (!tree_group(args.seg[i].nj) || (G != 8 && G != 16 && G != 32) || G < args.seg[i].nj)
+      return hipErrorInvalidValue;
+  }
   unsigned lds = 0;
   if (flags & KF_CRBA)
     for (int i = 0; i < args.nseg; ++i)

@@ -42,8 +42,9 @@ struct HeteroSegment {
 // Lanes-per-joint kernels for trees of <= 32 joints (FP32): FK + Jacobian +
 // RNEA + CRBA in one pass (flags select outputs).
 int tree_group(int nj);
+// G = lanes per environment (8, 16 or 32, >= nj; 0 = tree_group(nj)).
 hipError_t launch_tree_fused(const ModelTable<float> *tab, int nj, int64_t B, int flags, const float *q,
-                             const float *qd, const float *qdd, const KinOut<float> &out, hipStream_t s);
+                             const float *qd, const float *qdd, const KinOut<float> &out, hipStream_t s, int G = 0);
 template <class T> struct VjpArgs;
 // Tree version of the fused forward + reverse pass (pose gradients need one
 // user link per joint).
@@ -154,9 +155,17 @@ hipError_t launch_clik_hetero(const HeteroClikArgs &A, int max_chain, int waves,
 // Serves the per-model batched and the mixed-tree launches;
 // clik_lanes_group returns the first phase's group size, 0 above 16 chain
 // joints.
+// The schedule: phase one runs iterations [0, cap) at g1 lanes x (16 / g1)
+// chain joints per problem, phase two the problems still running at g2 lanes
+// (g1, g2 in {4, 8, 16}); cap = 0 runs every problem to its end in phase one.
 constexpr int kClikPhase1Iters = 128;
+struct ClikSchedule {
+  int cap = kClikPhase1Iters, g1 = 4, g2 = 8;
+};
+bool clik_schedule_valid(const ClikSchedule &s);
 int clik_lanes_group(int max_chain);
-hipError_t launch_clik_lanes(const HeteroClikArgs &A, int max_chain, int cus, hipStream_t s);
+hipError_t launch_clik_lanes(const HeteroClikArgs &A, int max_chain, int cus, const ClikSchedule &sch,
+                             hipStream_t s);
 
 // Lanes-per-joint fused FK + Jacobian + RNEA for serial chains of <= 16 joints
 // (FP32), G lanes per environment x C joints per lane. Returns

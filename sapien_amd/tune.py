@@ -1,14 +1,22 @@
-"""Autotuner for the lanes-per-joint chain kernel shape.
+"""Autotuner for the lanes-per-joint kernel shapes.
 
-The fused FK + Jacobian + RNEA kernel for serial chains maps one environment to
-G lanes with C joints per lane (G C >= nj). Small G spends fewer lane-cycles
-per environment (less scan overhead); large G spreads a batch over more
-wavefronts (more SIMDs busy, more latency hiding). The best shape depends on
-the joint count, the batch size and the GPU, so it is measured: every feasible
-shape is timed with HIP events on the launch stream over back-to-back launches
-and the fastest is installed in the model through the C ABI
-(sapamd_set_chain_shape). Results are cached per (GPU, joint count, batch) in a
-JSON file so later processes skip the sweep.
+* Chains: the fused FK + Jacobian + RNEA kernel for serial chains maps one
+  environment to G lanes with C joints per lane (G C >= nj). Small G spends
+  fewer lane-cycles per environment (less scan overhead); large G spreads a
+  batch over more wavefronts (more SIMDs busy, more latency hiding).
+  Installed with sapamd_set_chain_shape.
+* Trees and the mixed-tree launch: lanes per environment (8, 16 or 32, at
+  least the joint count) of every model's segment, timed over the whole
+  mixed launch (sapamd_set_tree_group).
+* CLIK: the library-wide schedule of the lanes-per-joint CLIK launches
+  (first-phase iteration cap, lanes per problem in each phase), timed over
+  the caller's CLIK workload (sapamd_set_clik_schedule).
+
+The best shape depends on the joint counts, the batch sizes and the GPU, so it
+is measured: every candidate is timed (HIP events on the launch stream, or the
+host clock around a synchronized call) and the fastest is installed through
+the C ABI. Results are cached per GPU and workload in a JSON file so later
+processes skip the sweep.
 """
 from __future__ import annotations
 
@@ -132,3 +140,151 @@ def autotune(model, batch_sizes=(16384,), iters=100, use_cache=True, report=None
     if use_cache:
         _save_cache(cache)
     return out
+
+
+# ---------------------------------------------------------------- tree kernels and mixed-tree launches
+def tree_groups(model):
+    """Lanes-per-environment choices of the lanes-per-joint tree kernels for this model."""
+    nj = model.nv
+    return [G for G in (8, 16, 32) if G >= nj]
+
+
+def set_tree_group(model, G):
+    from .native import check, handle, lib
+
+    check(lib().sapamd_set_tree_group(handle(model), int(G)))
+
+
+def get_tree_group(model):
+    from .native import check, handle, lib
+
+    g = ctypes.c_int32()
+    check(lib().sapamd_get_tree_group(handle(model), ctypes.byref(g)))
+    return g.value
+
+
+def _gpu_name():
+    import torch
+
+    return torch.cuda.get_device_name(torch.device("cuda", torch.cuda.current_device()))
+
+
+def _time(fn, iters, rounds=3):
+    """Best of `rounds` HIP-event timings of `iters` back-to-back calls, in microseconds."""
+    import torch
+
+    stream = torch.cuda.current_stream()
+    for _ in range(3):
+        fn()
+    best = float("inf")
+    for _ in range(rounds):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(iters):
+            fn()
+        b.record(stream)
+        b.synchronize()
+        best = min(best, a.elapsed_time(b) / iters * 1e3)
+    return best
+
+
+def autotune_hetero(models, batches, iters=50, use_cache=True, report=None):
+    """Pick the lanes per environment of every model's segment of the mixed-tree
+    FK + J + RNEA + CRBA launch (sapamd_hetero_batched) by timing every
+    combination of the models' choices over the given per-model batch sizes
+    (q, qd, qdd uniform in [-1, 1]); installs the fastest through
+    sapamd_set_tree_group and returns ({model index: G}, microseconds)."""
+    import itertools
+
+    import torch
+
+    from .native import launcher_hetero
+
+    dev = torch.device("cuda", torch.cuda.current_device())
+    key = f"{_gpu_name()}|hetero|" + "|".join(f"{m.nv}x{m.nlinks}x{b}" for m, b in zip(models, batches))
+    cache = _load_cache() if use_cache else {}
+    if key in cache:
+        gs, us = cache[key]
+        for m, G in zip(models, gs):
+            set_tree_group(m, G)
+        return dict(enumerate(gs)), us
+    gen = torch.Generator(device="cpu").manual_seed(7)
+    ins = [[(torch.rand(b, m.nv, generator=gen) * 2 - 1).to(dev) for _ in range(3)] for m, b in zip(models, batches)]
+    outs = [(torch.empty(b, m.nlinks, 7, device=dev), torch.empty(b, 6, m.nv, device=dev),
+             torch.empty(b, m.nv, device=dev), torch.empty(b, m.nv, m.nv, device=dev)) for m, b in zip(models, batches)]
+    stream = torch.cuda.current_stream(dev)
+    best = None
+    for gs in itertools.product(*[tree_groups(m) for m in models]):
+        for m, G in zip(models, gs):
+            set_tree_group(m, G)
+        launch = launcher_hetero(models, batches, [x[0] for x in ins], [x[1] for x in ins], [x[2] for x in ins],
+                                 *[[o[k] for o in outs] for k in range(4)])
+        us = _time(lambda: launch(stream.cuda_stream), iters)
+        if report is not None:
+            report.append({"lanes_per_env": list(gs), "us": us})
+        if best is None or us < best[1]:
+            best = (list(gs), us)
+    for m, G in zip(models, best[0]):
+        set_tree_group(m, G)
+    cache[key] = best
+    if use_cache:
+        _save_cache(cache)
+    return dict(enumerate(best[0])), best[1]
+
+
+# ---------------------------------------------------------------- CLIK schedule
+CLIK_SCHEDULES = [(128, 4, 8), (64, 4, 8), (256, 4, 8), (128, 4, 16), (128, 8, 16), (128, 4, 4), (0, 4, 8), (0, 8, 8)]
+
+
+def set_clik_schedule(cap, g1, g2):
+    from .native import check, lib
+
+    check(lib().sapamd_set_clik_schedule(int(cap), int(g1), int(g2)))
+
+
+def get_clik_schedule():
+    from .native import check, lib
+
+    c, a, b = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    check(lib().sapamd_get_clik_schedule(ctypes.byref(c), ctypes.byref(a), ctypes.byref(b)))
+    return c.value, a.value, b.value
+
+
+def autotune_clik(run, tag, candidates=CLIK_SCHEDULES, rounds=2, use_cache=True, report=None):
+    """Pick the library-wide lanes-per-joint CLIK schedule (first-phase
+    iteration cap, lanes per problem in each phase) by timing `run()` (one
+    batched CLIK call over the caller's workload, e.g. the mixed-tree batch)
+    under every candidate; installs the fastest and returns ((cap, g1, g2),
+    milliseconds). A problem's result bits depend on the schedule (the
+    association order of its sums), not on the batch it runs in."""
+    import time
+
+    import torch
+
+    key = f"{_gpu_name()}|clik|{tag}"
+    cache = _load_cache() if use_cache else {}
+    if key in cache:
+        sch, ms = cache[key]
+        set_clik_schedule(*sch)
+        return tuple(sch), ms
+    best = None
+    for sch in candidates:
+        set_clik_schedule(*sch)
+        run()
+        torch.cuda.synchronize()
+        t = float("inf")
+        for _ in range(rounds):
+            t0 = time.perf_counter()
+            run()
+            torch.cuda.synchronize()
+            t = min(t, (time.perf_counter() - t0) * 1e3)
+        if report is not None:
+            report.append({"cap": sch[0], "g1": sch[1], "g2": sch[2], "ms": t})
+        if best is None or t < best[1]:
+            best = (sch, t)
+    set_clik_schedule(*best[0])
+    cache[key] = [list(best[0]), best[1]]
+    if use_cache:
+        _save_cache(cache)
+    return tuple(best[0]), best[1]

@@ -879,3 +879,56 @@ def test_random_qpos(tree):
     for _ in range(5):
         q = m.get_random_qpos()
         assert q.dtype == np.float64 and q.shape == (rob.nv,) and ((q >= lo) & (q <= hi)).all()
+
+
+def test_tree_groups_and_clik_schedules():
+    """Every autotuner choice computes the same thing: the mixed-tree launch
+    with every lanes-per-environment choice of each segment against the FP64
+    oracle (FP32 tolerances), and batched CLIK under every schedule: converged
+    problems reach their targets, the success rate stays, and a mixed-tree
+    launch equals the per-model launches bit for bit."""
+    import sapien_amd as sa
+    from sapien_amd import tune
+
+    trees = ["chain7", "arm9", "quad18", "torso30"]
+    pairs = [_pair(S.robot(t)) for t in trees]
+    Bs = [300, 200, 100, 50]
+    ins = [S.samples(S.robot(t), b, 2400 + i) for i, (t, b) in enumerate(zip(trees, Bs))]
+    try:
+        for G7 in (8, 16, 32):
+            for G9 in (16, 32):
+                tune.set_tree_group(pairs[0][0], G7)
+                tune.set_tree_group(pairs[1][0], G9)
+                outs = sa.hetero_batched([p[0] for p in pairs], [_t(x[0]) for x in ins], [_t(x[1]) for x in ins],
+                                         [_t(x[2]) for x in ins])
+                for (m, o), (q, qd, qdd, _), d in zip(pairs, ins, outs):
+                    pos, ang = pose_errors(d["poses"].cpu().numpy(), o.link_poses(q))
+                    assert pos <= POSE_TOL_M and ang <= POSE_TOL_RAD, (G7, G9)
+                    assert rel_err(d["J"].cpu().numpy(), o.jacobian_world(q)) <= REL_TOL
+                    assert rel_err(d["tau"].cpu().numpy(), o.rnea(q, qd, qdd)) <= REL_TOL
+                    assert rel_err(d["M"].cpu().numpy(), o.crba(q)) <= REL_TOL
+        with pytest.raises(ValueError):
+            tune.set_tree_group(pairs[1][0], 8)  # fewer lanes than joints
+    finally:
+        for m, _ in pairs:
+            tune.set_tree_group(m, 0)
+    # CLIK schedules
+    links = [S.robot(t).ee_link for t in trees]
+    tg = [_t(o.link_poses(S.samples(S.robot(t), n, 2410 + i)[0])[:, l])
+          for i, ((m, o), t, n, l) in enumerate(zip(pairs, trees, [400, 300, 200, 100], links))]
+    try:
+        for sch in tune.CLIK_SCHEDULES:
+            tune.set_clik_schedule(*sch)
+            assert tune.get_clik_schedule() == sch
+            mixed = sa.clik_hetero_batched([p[0] for p in pairs], links, tg)
+            for (m, o), l, t, (Q, ok, err, it) in zip(pairs, links, tg, mixed):
+                Qr, okr, errr, itr = m.compute_inverse_kinematics_batched(l, t)
+                assert torch.equal(Q, Qr) and torch.equal(ok, okr) and torch.equal(it, itr), sch
+                okn = ok.cpu().numpy().astype(bool)
+                assert okn.mean() > 0.5, sch
+                reached = o.link_poses(Q.cpu().numpy()[okn].astype(np.float64))[:, l, :3]
+                assert np.abs(reached - t.cpu().numpy()[okn, :3]).max() < 2e-4, sch
+        with pytest.raises(ValueError):
+            tune.set_clik_schedule(128, 5, 8)
+    finally:
+        tune.set_clik_schedule(128, 4, 8)
