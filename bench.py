@@ -300,6 +300,21 @@ def mixed_workloads(sa, S, np, torch, dev):
                 main.wait_stream(st)
         return run
 
+    # all four trees in one launch forward and one backward
+    mq0, mqd0, mtau = [p[1] for p in per], [p[2] for p in per], [p[3] for p in per]
+    mix = {}
+
+    def fwd_mixed():
+        mix["f"] = sa.rollout_hetero_batched(models, mq0, mqd0, mtau, dt, K)
+
+    fwd_mixed()
+    mones = [torch.ones_like(x) for x in mq0]
+    mck = [f[2] for f in mix["f"]]
+
+    def bwd_mixed():
+        sa.rollout_backward_hetero_batched(models, mtau, mck, dt, mones, mones, K)
+
+    mf, mb = timed(fwd_mixed, 5), timed(bwd_mixed, 5)
     sf, sb = timed(seq(fwd_one), 3), timed(seq(bwd_one), 3)
     cf, cb = timed(conc(fwd_one), 3), timed(conc(bwd_one), 3)
     trees = {}
@@ -313,11 +328,13 @@ def mixed_workloads(sa, S, np, torch, dev):
                        # step is unstable for quad18's light feet, in FP64 as well)
                        "final_state_finite": float(torch.isfinite(ck[j][0]).all(dim=1).float().mean())}
     res["rollouts_mixed"] = {"rollouts": nr, "steps": T, "dt": dt, "checkpoint_every": K,
-                             "forward_env_steps_s": 4096 * T / cf, "forward_ms": cf * 1e3,
-                             "backward_env_steps_s": 4096 * T / cb, "backward_ms": cb * 1e3,
-                             "sequential": {"forward_ms": sf * 1e3, "backward_ms": sb * 1e3},
-                             "per_tree": trees,
-                             "note": "one launch per step and tree; the four trees on four streams"}
+                             "forward_env_steps_s": 4096 * T / mf, "forward_ms": mf * 1e3,
+                             "backward_env_steps_s": 4096 * T / mb, "backward_ms": mb * 1e3,
+                             "kernel": "all four trees in one launch forward and one backward (T steps per "
+                                       "launch, state in registers, segments recomputed inside the adjoint)",
+                             "one_launch_per_tree_on_four_streams": {"forward_ms": cf * 1e3, "backward_ms": cb * 1e3},
+                             "one_launch_per_tree_sequential": {"forward_ms": sf * 1e3, "backward_ms": sb * 1e3},
+                             "per_tree": trees}
     return res
 
 

@@ -290,9 +290,12 @@ int sapamd_pose_torque_loss_batched(const sapamd_model *m, int64_t B, const floa
 /* ---------------------------------------------------------------- rollouts (FP32, batched)
  * Semi-implicit Euler with ABA forward dynamics: qdd_t = ABA(q_t, qd_t, tau_t),
  * qd_{t+1} = qd_t + dt qdd_t, q_{t+1} = q_t + dt qd_{t+1}. No reference
- * counterpart (the reference exposes single ABA calls). Every step is one
- * launch on `stream`, so a caller may capture a whole call in a HIP graph;
- * nothing is allocated. tau is (B, T, nv). */
+ * counterpart (the reference exposes single ABA calls,
+ * pinocchio_model.cpp:221-226). Trees of <= 32 joints under the lanes-per-joint
+ * kernels run a whole call in one launch (sapamd_set_rollout_launch), other
+ * models one launch per step; either way everything is on `stream`, so a
+ * caller may capture a whole call in a HIP graph, and nothing is allocated.
+ * tau is (B, T, nv). */
 /* One step in place on q, qd (B, nv) with torques tau_t rows at tau_stride. */
 int sapamd_rollout_step(const sapamd_model *m, int64_t B, float dt, float *q, float *qd, const float *tau_t,
                         int64_t tau_stride, void *stream);
@@ -307,6 +310,25 @@ int sapamd_rollout_forward(const sapamd_model *m, int64_t B, int32_t T, int32_t 
 int sapamd_rollout_backward(const sapamd_model *m, int64_t B, int32_t T, int32_t K, float dt, const float *tau,
                             const float *ckpt, float *seg, float *q_tmp, float *qd_tmp, float *aq, float *aqd,
                             float *gtau, void *stream);
+/* Rollouts of several models in ONE launch (at most 8 models, each a tree of
+ * 1 to 32 joints under the lanes-per-joint kernels, all on one GPU; same T, K
+ * and dt): model i steps B[i] environments in place on q[i], qd[i] (B[i], nv_i)
+ * under tau[i] (B[i], T, nv_i), checkpoints into ckpt[i] (may be NULL, as may
+ * the ckpt array) like sapamd_rollout_forward. Each environment gets the
+ * same bits as in its own model's sapamd_rollout_forward (FUSED), in that
+ * model's dynamics precision. No reference counterpart (the reference steps
+ * one articulation per call, pinocchio_model.cpp:221-226). */
+int sapamd_rollout_forward_hetero(const sapamd_model *const *models, int32_t nmodels, const int64_t *B, int32_t T,
+                                  int32_t K, float dt, float *const *q, float *const *qd, const float *const *tau,
+                                  float *const *ckpt, void *stream);
+/* The adjoints of those rollouts in ONE launch, per model as
+ * sapamd_rollout_backward (FUSED): aq[i], aqd[i] (B[i], nv_i) hold dL/dq_T,
+ * dL/dqd_T on entry and dL/dq_0, dL/dqd_0 on exit; gtau[i] (B[i], T, nv_i)
+ * receives dL/dtau; seg[i] (B[i], K, 2 nv_i) is scratch. */
+int sapamd_rollout_backward_hetero(const sapamd_model *const *models, int32_t nmodels, const int64_t *B, int32_t T,
+                                   int32_t K, float dt, const float *const *tau, const float *const *ckpt,
+                                   float *const *seg, float *const *aq, float *const *aqd, float *const *gtau,
+                                   void *stream);
 
 /* ---------------------------------------------------------------- forward-dynamics precision
  * Batched forward dynamics (sapamd_aba_batched, rollouts) runs FP32 ABA unless

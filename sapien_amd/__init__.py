@@ -19,7 +19,8 @@ except ImportError as e:  # pragma: no cover - exercised only on broken installs
         "sapien_amd native extension is not built or is stale; run `python sapien_amd/build.py` "
         f"(or __graft_entry__.build()) first: {e}") from e
 
-from ._pinocchio import PinocchioModel, Pose, clik_hetero_batched, hetero_batched  # noqa: E402
+from ._pinocchio import (PinocchioModel, Pose, clik_hetero_batched, hetero_batched,  # noqa: E402
+                         rollout_backward_hetero_batched, rollout_hetero_batched)
 
 LIBRARY_PATH = os.path.join(_HERE, "libsapien_amd.so")
 
@@ -104,5 +105,6 @@ def _ik_multistart(self, link_index, poses, initial_qpos=None, n_init_qpos=20, a
 
 PinocchioModel.compute_inverse_kinematics_multistart_batched = _ik_multistart
 
-__all__ = ["PinocchioModel", "Pose", "hetero_batched", "clik_hetero_batched", "LIBRARY_PATH", "native_fk_jac_rnea", "kernel_variant",
+__all__ = ["PinocchioModel", "Pose", "hetero_batched", "clik_hetero_batched", "rollout_hetero_batched",
+           "rollout_backward_hetero_batched", "LIBRARY_PATH", "native_fk_jac_rnea", "kernel_variant",
            "set_kernel_variant", "dynamics_precision", "set_dynamics_precision", "set_rollout_launch"]

@@ -1624,6 +1624,86 @@ int sapamd_rollout_backward(const sapamd_model *m, int64_t B, int32_t T, int32_t
   });
 }
 
+namespace {
+// One segment per model for a mixed-tree rollout launch; models with no
+// environments are left out.
+RolloutMixArgs rollout_mix(const sapamd_model *const *models, int32_t nmodels, const int64_t *B, int32_t T,
+                           int32_t K, float dt) {
+  if (nmodels < 1 || nmodels > kMaxSegments) throw std::invalid_argument("1 to 8 models per mixed-tree launch");
+  if (!models || !B) throw std::invalid_argument("models and B must not be null");
+  if (T < 0 || K < 1) throw std::invalid_argument("T >= 0 and K >= 1 required");
+  RolloutMixArgs A{};
+  A.T = T;
+  A.K = K;
+  A.dt = dt;
+  for (int i = 0; i < nmodels; ++i) {
+    const sapamd_model *m = models[i];
+    check_batched(m, B[i]);
+    if (!lanes_dynamics(m))
+      throw std::runtime_error("mixed-tree rollouts need trees of 1 to 32 joints under the lanes-per-joint kernels");
+    if (m->device != models[0]->device) throw std::invalid_argument("mixed-tree rollouts need models on one GPU");
+    RolloutSegment &s = A.seg[i];
+    s.fp64 = m->fd64() ? 1 : 0;
+    s.tab = s.fp64 ? static_cast<const void *>(m->t64()) : static_cast<const void *>(m->t32());
+    s.nj = m->host->njoints();
+    s.B = B[i];
+  }
+  A.nseg = nmodels;
+  return A;
+}
+}  // namespace
+
+int sapamd_rollout_forward_hetero(const sapamd_model *const *models, int32_t nmodels, const int64_t *B, int32_t T,
+                                  int32_t K, float dt, float *const *q, float *const *qd, const float *const *tau,
+                                  float *const *ckpt, void *stream) {
+  return guarded([&] {
+    RolloutMixArgs A = rollout_mix(models, nmodels, B, T, K, dt);
+    if (T == 0) return;
+    need(q, "q");
+    need(qd, "qd");
+    need(tau, "tau");
+    for (int i = 0; i < nmodels; ++i) {
+      if (B[i] == 0) continue;
+      need(q[i], "q");
+      need(qd[i], "qd");
+      need(tau[i], "tau");
+      RolloutSegment &s = A.seg[i];
+      s.q = q[i];
+      s.qd = qd[i];
+      s.tau = tau[i];
+      s.ckpt_out = ckpt ? ckpt[i] : nullptr;
+    }
+    HIP_OK(launch_rollout_mixed(A, false, S(stream)));
+  });
+}
+
+int sapamd_rollout_backward_hetero(const sapamd_model *const *models, int32_t nmodels, const int64_t *B, int32_t T,
+                                   int32_t K, float dt, const float *const *tau, const float *const *ckpt,
+                                   float *const *seg, float *const *aq, float *const *aqd, float *const *gtau,
+                                   void *stream) {
+  return guarded([&] {
+    RolloutMixArgs A = rollout_mix(models, nmodels, B, T, K, dt);
+    if (T == 0) return;
+    for (const void *p : {(const void *)tau, (const void *)ckpt, (const void *)seg, (const void *)aq,
+                          (const void *)aqd, (const void *)gtau})
+      need(p, "rollout buffer array");
+    for (int i = 0; i < nmodels; ++i) {
+      if (B[i] == 0) continue;
+      for (const void *p : {(const void *)tau[i], (const void *)ckpt[i], (const void *)seg[i], (const void *)aq[i],
+                            (const void *)aqd[i], (const void *)gtau[i]})
+        need(p, "rollout buffer");
+      RolloutSegment &s = A.seg[i];
+      s.tau = tau[i];
+      s.ckpt = ckpt[i];
+      s.seg = seg[i];
+      s.aq = aq[i];
+      s.aqd = aqd[i];
+      s.gtau = gtau[i];
+    }
+    HIP_OK(launch_rollout_mixed(A, true, S(stream)));
+  });
+}
+
 int sapamd_set_rollout_launch(sapamd_model *m, int32_t mode) {
   return guarded([&] {
     if (!m) throw std::invalid_argument("null model");

@@ -18,6 +18,8 @@
 // only wavefront-level ordering (no block barrier).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <climits>
 #include <type_traits>
 
 #include "lanes.h"
@@ -357,19 +359,20 @@ __global__ __launch_bounds__(BLOCK) void k_tree_step_vjp(const ModelTable<Real> 
   gtau[e * tau_stride + u] = (float)lam;
 }
 
-// T steps of semi-implicit Euler in one launch, the state held in registers
-// (lane j: q_j, qd_j in FP32, exactly as the one-step kernel rounds it).
+// T steps of semi-implicit Euler for environment env of a batch of B, the
+// state held in registers (lane j: q_j, qd_j in FP32, exactly as the one-step
+// kernel rounds it). M is the model table staged in LDS, L the group's tile.
 // tau (B, T, nv); ckpt (B, ceil(T / K), 2 nv) receives the pre-step state of
 // steps 0, K, 2K, ... when given; q, qd (B, nv) hold the initial state on entry
 // and the final state on exit.
-template <int G, class Real, int BLOCK, int NMAX>
-__global__ __launch_bounds__(BLOCK) void k_tree_rollout(const ModelTable<Real> *__restrict__ Mg, int64_t B, int T,
-                                                        int K, float dt, float *__restrict__ q,
-                                                        float *__restrict__ qd, const float *__restrict__ tau,
-                                                        float *__restrict__ ckpt) {
-  SAPAMD_TREE_DYN_PROLOGUE
-  const ModelTable<Real> *M = stage_table(Mg, s_tab);
-  DynLds<G, Real> &L = s_dyn[threadIdx.x / G];
+template <int G, class Real, int NMAX>
+__device__ __forceinline__ void rollout_env(const ModelTable<Real> *M, DynLds<G, Real> &L, int64_t env, int64_t B,
+                                            int T, int K, float dt, float *__restrict__ q, float *__restrict__ qd,
+                                            const float *__restrict__ tau, float *__restrict__ ckpt) {
+  const int g = threadIdx.x & (G - 1);
+  const bool live = env < B;
+  const int64_t e = live ? env : B - 1;  // tail groups recompute the last environment and store nothing
+  const int nv = M->nv;
   TreeFd<G, Real> F;
   init_fd<G, Real>(F, M, g);
   const int u = F.J->vuser;  // lanes past the last joint repeat its dof and store nothing
@@ -398,20 +401,21 @@ __global__ __launch_bounds__(BLOCK) void k_tree_rollout(const ModelTable<Real> *
   }
 }
 
-// Adjoint of k_tree_rollout in one launch: segments of K steps from the last,
-// each recomputed from its checkpoint with the pre-step states kept in seg
-// (B, K, 2 nv; each lane reads back only what it wrote), then the step
+// Adjoint of rollout_env for environment env: segments of K steps from the
+// last, each recomputed from its checkpoint with the pre-step states kept in
+// seg (B, K, 2 nv; each lane reads back only what it wrote), then the step
 // adjoints in reverse. aq, aqd: dL/d(final state) in, dL/d(initial state)
 // out; gtau (B, T, nv) = dL/dtau.
-template <int G, class Real, int BLOCK, int NMAX>
-__global__ __launch_bounds__(BLOCK) void k_tree_rollout_vjp(const ModelTable<Real> *__restrict__ Mg, int64_t B, int T,
-                                                            int K, float dt, const float *__restrict__ tau,
-                                                            const float *__restrict__ ckpt, float *__restrict__ seg,
-                                                            float *__restrict__ aq, float *__restrict__ aqd,
-                                                            float *__restrict__ gtau) {
-  SAPAMD_TREE_DYN_PROLOGUE
-  const ModelTable<Real> *M = stage_table(Mg, s_tab);
-  DynLds<G, Real> &L = s_dyn[threadIdx.x / G];
+template <int G, class Real, int NMAX>
+__device__ __forceinline__ void rollout_vjp_env(const ModelTable<Real> *M, DynLds<G, Real> &L, int64_t env,
+                                                int64_t B, int T, int K, float dt, const float *__restrict__ tau,
+                                                const float *__restrict__ ckpt, float *__restrict__ seg,
+                                                float *__restrict__ aq, float *__restrict__ aqd,
+                                                float *__restrict__ gtau) {
+  const int g = threadIdx.x & (G - 1);
+  const bool live = env < B;
+  const int64_t e = live ? env : B - 1;
+  const int nv = M->nv;
   TreeFd<G, Real> F;
   init_fd<G, Real>(F, M, g);
   const int u = F.J->vuser;
@@ -452,6 +456,91 @@ __global__ __launch_bounds__(BLOCK) void k_tree_rollout_vjp(const ModelTable<Rea
     aq[e * nv + u] = (float)aqj;
     aqd[e * nv + u] = (float)aqdj;
   }
+}
+
+// T steps in one launch, one model: blocks of BLOCK threads, G per environment.
+template <int G, class Real, int BLOCK, int NMAX>
+__global__ __launch_bounds__(BLOCK) void k_tree_rollout(const ModelTable<Real> *__restrict__ Mg, int64_t B, int T,
+                                                        int K, float dt, float *__restrict__ q,
+                                                        float *__restrict__ qd, const float *__restrict__ tau,
+                                                        float *__restrict__ ckpt) {
+  __shared__ float4 s_tab[kDynTableWords4<Real>];
+  __shared__ DynLds<G, Real> s_dyn[BLOCK / G];
+  const ModelTable<Real> *M = stage_table(Mg, s_tab);
+  rollout_env<G, Real, NMAX>(M, s_dyn[threadIdx.x / G], (blockIdx.x * (int64_t)BLOCK + threadIdx.x) / G, B, T, K,
+                             dt, q, qd, tau, ckpt);
+}
+
+// Adjoint of k_tree_rollout in one launch.
+template <int G, class Real, int BLOCK, int NMAX>
+__global__ __launch_bounds__(BLOCK) void k_tree_rollout_vjp(const ModelTable<Real> *__restrict__ Mg, int64_t B, int T,
+                                                            int K, float dt, const float *__restrict__ tau,
+                                                            const float *__restrict__ ckpt, float *__restrict__ seg,
+                                                            float *__restrict__ aq, float *__restrict__ aqd,
+                                                            float *__restrict__ gtau) {
+  __shared__ float4 s_tab[kDynTableWords4<Real>];
+  __shared__ DynLds<G, Real> s_dyn[BLOCK / G];
+  const ModelTable<Real> *M = stage_table(Mg, s_tab);
+  rollout_vjp_env<G, Real, NMAX>(M, s_dyn[threadIdx.x / G], (blockIdx.x * (int64_t)BLOCK + threadIdx.x) / G, B, T,
+                                 K, dt, tau, ckpt, seg, aq, aqd, gtau);
+}
+
+// Rollouts of several models in one launch: block ranges map to segments (one
+// model each, ordered by the host); the table and the groups' tiles are carved
+// from one dynamic LDS allocation sized for the largest segment. Every case
+// of the switch is the per-model kernels' code, so each environment gets the
+// same bits as in its own model's launch.
+constexpr int kMixBlock = 128;
+
+template <int G, class Real, int NMAX, bool VJP>
+__device__ __forceinline__ void mixed_segment(const RolloutSegment &S, const RolloutMixArgs &A, float4 *lds) {
+  const ModelTable<Real> *M = stage_table(static_cast<const ModelTable<Real> *>(S.tab), lds);
+  DynLds<G, Real> *dyn = reinterpret_cast<DynLds<G, Real> *>(lds + kDynTableWords4<Real>);
+  const int64_t env = ((int64_t)(blockIdx.x - S.block0) * kMixBlock + threadIdx.x) / G;
+  if constexpr (VJP)
+    rollout_vjp_env<G, Real, NMAX>(M, dyn[threadIdx.x / G], env, S.B, A.T, A.K, A.dt, S.tau, S.ckpt, S.seg, S.aq,
+                                   S.aqd, S.gtau);
+  else
+    rollout_env<G, Real, NMAX>(M, dyn[threadIdx.x / G], env, S.B, A.T, A.K, A.dt, S.q, S.qd, S.tau, S.ckpt_out);
+}
+
+template <bool VJP> __global__ __launch_bounds__(kMixBlock) void k_tree_rollout_mixed(RolloutMixArgs A) {
+  extern __shared__ float4 s_mix[];
+  int s = 0;
+  while (s + 1 < A.nseg && (int64_t)blockIdx.x >= A.seg[s + 1].block0) ++s;
+  const RolloutSegment &S = A.seg[s];
+  switch (S.variant) {  // uniform per block
+    case 0: mixed_segment<8, float, 8, VJP>(S, A, s_mix); break;
+    case 1: mixed_segment<16, float, 12, VJP>(S, A, s_mix); break;
+    case 2: mixed_segment<16, float, 16, VJP>(S, A, s_mix); break;
+    case 3: mixed_segment<32, float, 24, VJP>(S, A, s_mix); break;
+    case 4: mixed_segment<32, float, 32, VJP>(S, A, s_mix); break;
+    case 5: mixed_segment<8, double, 8, VJP>(S, A, s_mix); break;
+    case 6: mixed_segment<16, double, 12, VJP>(S, A, s_mix); break;
+    case 7: mixed_segment<16, double, 16, VJP>(S, A, s_mix); break;
+    case 8: mixed_segment<32, double, 24, VJP>(S, A, s_mix); break;
+    default: mixed_segment<32, double, 32, VJP>(S, A, s_mix); break;
+  }
+}
+
+// The switch case for nj joints: (G, NMAX) as SAPAMD_TREE_DYN_LAUNCH picks
+// them, FP64 cases after the FP32 ones.
+int mixed_variant(int nj, bool fp64) {
+  const int G = tree_dyn_group(nj);
+  const int v = G == 8 ? 0 : G == 16 ? (nj <= 12 ? 1 : 2) : (nj <= 24 ? 3 : 4);
+  return v + (fp64 ? 5 : 0);
+}
+
+template <class Real, int G> constexpr size_t mixed_lds() {
+  return sizeof(float4) * kDynTableWords4<Real> + sizeof(DynLds<G, Real>) * (kMixBlock / G);
+}
+static_assert(mixed_lds<double, 32>() <= 65536, "the mixed launch's LDS must fit the default dynamic LDS limit");
+size_t mixed_lds_bytes(int variant) {
+  constexpr size_t t[10] = {mixed_lds<float, 8>(),  mixed_lds<float, 16>(),  mixed_lds<float, 16>(),
+                            mixed_lds<float, 32>(), mixed_lds<float, 32>(),  mixed_lds<double, 8>(),
+                            mixed_lds<double, 16>(), mixed_lds<double, 16>(), mixed_lds<double, 32>(),
+                            mixed_lds<double, 32>()};
+  return t[variant];
 }
 
 // FP32: 256-thread blocks; FP64: 128-thread blocks (half the LDS per block).
@@ -545,6 +634,39 @@ hipError_t launch_tree_rollout_vjp(const ModelTable<float> *tab, const ModelTabl
   if (!tree_dyn_group(nj) || K < 1) return hipErrorNotSupported;
   return tab64 ? rollout_vjp_impl<double>(tab64, nj, B, T, K, dt, tau, ckpt, seg, aq, aqd, gtau, s)
                : rollout_vjp_impl<float>(tab, nj, B, T, K, dt, tau, ckpt, seg, aq, aqd, gtau, s);
+}
+
+hipError_t launch_rollout_mixed(RolloutMixArgs A, bool vjp, hipStream_t s) {
+  if (A.nseg < 1 || A.nseg > kMaxSegments || A.K < 1 || A.T < 0) return hipErrorInvalidValue;
+  if (A.T == 0) return hipSuccess;
+  // heaviest first: FP64 before FP32, then by lanes per environment and joints
+  RolloutSegment *b = A.seg, *e = A.seg + A.nseg;
+  for (RolloutSegment *x = b; x != e; ++x) {
+    if (!tree_dyn_group(x->nj) || x->B < 0) return hipErrorInvalidValue;
+    x->variant = mixed_variant(x->nj, x->fp64 != 0);
+  }
+  std::stable_sort(b, e, [](const RolloutSegment &x, const RolloutSegment &y) {
+    return x.variant != y.variant ? x.variant > y.variant : x.nj > y.nj;
+  });
+  int64_t blocks = 0;
+  size_t lds = 0;
+  int n = 0;
+  for (int i = 0; i < A.nseg; ++i) {
+    if (A.seg[i].B == 0) continue;
+    RolloutSegment &x = A.seg[n++] = A.seg[i];
+    const int G = tree_dyn_group(x.nj);
+    x.block0 = blocks;
+    blocks += (x.B * G + kMixBlock - 1) / kMixBlock;
+    lds = std::max(lds, mixed_lds_bytes(x.variant));
+  }
+  A.nseg = n;
+  if (blocks == 0) return hipSuccess;
+  if (blocks > INT32_MAX) return hipErrorInvalidValue;
+  if (vjp)
+    hipLaunchKernelGGL(k_tree_rollout_mixed<true>, dim3((unsigned)blocks), dim3(kMixBlock), (unsigned)lds, s, A);
+  else
+    hipLaunchKernelGGL(k_tree_rollout_mixed<false>, dim3((unsigned)blocks), dim3(kMixBlock), (unsigned)lds, s, A);
+  return hipGetLastError();
 }
 
 }  // namespace sapamd

@@ -238,6 +238,28 @@ hipError_t launch_tree_rollout(const ModelTable<float> *tab, const ModelTable<do
 hipError_t launch_tree_rollout_vjp(const ModelTable<float> *tab, const ModelTable<double> *tab64, int nj, int64_t B,
                                    int T, int K, float dt, const float *tau, const float *ckpt, float *seg, float *aq,
                                    float *aqd, float *gtau, hipStream_t s);
+// One model of a mixed-tree rollout launch: its table (ModelTable<float>, or
+// ModelTable<double> when fp64), batch, first block and buffers as in
+// launch_tree_rollout (q, qd, tau, ckpt) / launch_tree_rollout_vjp (tau, ckpt,
+// seg, aq, aqd, gtau). variant and block0 are filled by launch_rollout_mixed.
+struct RolloutSegment {
+  const void *tab;
+  int32_t nj, fp64, variant;
+  int64_t B, block0;
+  float *q, *qd;
+  const float *tau, *ckpt;
+  float *ckpt_out, *seg, *aq, *aqd, *gtau;
+};
+struct RolloutMixArgs {
+  RolloutSegment seg[kMaxSegments];
+  int32_t nseg, T, K;
+  float dt;
+};
+// Rollouts (vjp = false: forward with checkpoints into ckpt_out) or their
+// adjoints (vjp = true) of up to kMaxSegments models in ONE launch; the
+// segments are ordered heaviest first so the longest-running blocks start
+// first. Same bits per environment as the per-model launches.
+hipError_t launch_rollout_mixed(RolloutMixArgs A, bool vjp, hipStream_t s);
 // rows of (q | qd) at stride -> q, qd (B, nv)
 hipError_t launch_restore_state(int64_t B, int nv, const float *src, int64_t stride, float *q, float *qd,
                                 hipStream_t s);

@@ -769,6 +769,107 @@ class PyModel {
     return out;
   }
 
+  // Mixed-tree rollouts: several models' batches in one launch (forward, then
+  // its adjoint), per model as rollout_batched / rollout_backward_batched.
+  template <class BatchT> static int64_t rollout_steps(BatchT &b, py::object tau) {
+    auto shp = b.torch.attr("as_tensor")(tau).attr("shape").template cast<std::vector<int64_t>>();
+    if (shp.size() != 3 || shp[0] != b.B || shp[2] != b.n)
+      throw std::runtime_error("tau must have shape (B, T, " + std::to_string(b.n) + ")");
+    return shp[1];
+  }
+  static py::list rollout_hetero_batched(py::list models, py::list q0s, py::list qd0s, py::list taus, double dt,
+                                         int64_t K, py::object stream) {
+    StreamScope scope_(stream, models.size() ? models[0].cast<PyModel &>().info().device : 0);
+    const size_t n = models.size();
+    if (q0s.size() != n || qd0s.size() != n || taus.size() != n)
+      throw py::value_error("one qpos, qvel and qf batch per model");
+    if (K < 1) throw py::value_error("checkpoint_every must be >= 1");
+    std::vector<std::unique_ptr<Batch>> batches;
+    std::vector<std::unique_ptr<DLIn>> views;
+    std::vector<py::object> keep;
+    std::vector<const sapamd_model *> ms;
+    std::vector<int64_t> Bs;
+    std::vector<float *> Q, QD, C;
+    std::vector<const float *> tau;
+    py::list out;
+    int64_t T = -1;
+    void *s = nullptr;
+    for (size_t i = 0; i < n; ++i) {
+      PyModel &pm = models[i].cast<PyModel &>();
+      batches.push_back(std::make_unique<Batch>(pm, q0s[i]));
+      Batch &b = *batches.back();
+      if (i == 0) s = b.stream(stream);
+      const int64_t Ti = rollout_steps(b, taus[i]);
+      if (T >= 0 && Ti != T) throw std::runtime_error("every model's tau needs the same number of steps");
+      T = Ti;
+      py::object tauT = b.as_shape(taus[i], {b.B, T, b.n});
+      py::object Qt = b.q.attr("clone")(), QDt = b.input(qd0s[i]).attr("clone")();
+      py::object Ct = b.empty({b.B, T ? ckpt_count(T, K) : 0, 2 * b.n});
+      keep.push_back(tauT);
+      for (py::object *t : {&Qt, &QDt, &tauT, &Ct}) views.push_back(b.view(*t));
+      Q.push_back(f(views[views.size() - 4]));
+      QD.push_back(f(views[views.size() - 3]));
+      tau.push_back(f(views[views.size() - 2]));
+      C.push_back(f(views[views.size() - 1]));
+      ms.push_back(pm.raw());
+      Bs.push_back(b.B);
+      out.append(py::make_tuple(Qt, QDt, Ct));
+    }
+    if (n)
+      check(sapamd_rollout_forward_hetero(ms.data(), (int32_t)n, Bs.data(), (int32_t)T, (int32_t)K, (float)dt,
+                                          Q.data(), QD.data(), tau.data(), C.data(), s));
+    return out;
+  }
+  static py::list rollout_backward_hetero_batched(py::list models, py::list taus, py::list ckpts, double dt,
+                                                  py::list gqs, py::list gqds, int64_t K, py::object stream) {
+    StreamScope scope_(stream, models.size() ? models[0].cast<PyModel &>().info().device : 0);
+    const size_t n = models.size();
+    if (taus.size() != n || ckpts.size() != n || gqs.size() != n || gqds.size() != n)
+      throw py::value_error("one qf, checkpoint and gradient batch per model");
+    if (K < 1) throw py::value_error("checkpoint_every must be >= 1");
+    std::vector<std::unique_ptr<Batch>> batches;
+    std::vector<std::unique_ptr<DLIn>> views;
+    std::vector<py::object> keep;
+    std::vector<const sapamd_model *> ms;
+    std::vector<int64_t> Bs;
+    std::vector<const float *> tau, C;
+    std::vector<float *> SG, AQ, AQD, GT;
+    py::list out;
+    int64_t T = -1;
+    void *s = nullptr;
+    for (size_t i = 0; i < n; ++i) {
+      PyModel &pm = models[i].cast<PyModel &>();
+      batches.push_back(std::make_unique<Batch>(pm, gqs[i]));
+      Batch &b = *batches.back();
+      if (i == 0) s = b.stream(stream);
+      const int64_t Ti = rollout_steps(b, taus[i]);
+      if (T >= 0 && Ti != T) throw std::runtime_error("every model's tau needs the same number of steps");
+      T = Ti;
+      py::object tauT = b.as_shape(taus[i], {b.B, T, b.n});
+      py::object Ct = b.as_shape(ckpts[i], {b.B, T ? ckpt_count(T, K) : 0, 2 * b.n});
+      py::object aq = b.q.attr("clone")(), aqd = b.input(gqds[i]).attr("clone")();
+      py::object Gt = b.torch.attr("zeros")(py::make_tuple(b.B, T, b.n), py::arg("device") = b.dev,
+                                            py::arg("dtype") = b.torch.attr("float32"));
+      py::object seg = b.empty({b.B, K, 2 * b.n});
+      keep.insert(keep.end(), {tauT, Ct, seg});
+      for (py::object *t : {&tauT, &Ct, &seg, &aq, &aqd, &Gt}) views.push_back(b.view(*t));
+      const size_t v = views.size() - 6;
+      tau.push_back(f(views[v]));
+      C.push_back(f(views[v + 1]));
+      SG.push_back(f(views[v + 2]));
+      AQ.push_back(f(views[v + 3]));
+      AQD.push_back(f(views[v + 4]));
+      GT.push_back(f(views[v + 5]));
+      ms.push_back(pm.raw());
+      Bs.push_back(b.B);
+      out.append(py::make_tuple(aq, aqd, Gt));
+    }
+    if (n)
+      check(sapamd_rollout_backward_hetero(ms.data(), (int32_t)n, Bs.data(), (int32_t)T, (int32_t)K, (float)dt,
+                                           tau.data(), C.data(), SG.data(), AQ.data(), AQD.data(), GT.data(), s));
+    return out;
+  }
+
  private:
   sapamd_model *m_ = nullptr;
   std::mt19937_64 rng_{std::random_device{}()};
@@ -875,6 +976,15 @@ PYBIND11_MODULE(_pinocchio, m) {
         py::arg("qacc") = py::none(), py::arg("stream") = py::none(),
         "Heterogeneous batch of several PinocchioModels (trees of <= 32 joints) in ONE kernel launch: per model "
         "a dict of link poses (B, L, 7), WORLD Jacobian (B, 6, nv), RNEA torques (B, nv) and mass matrix (B, nv, nv).");
+
+  m.def("rollout_hetero_batched", &PyModel::rollout_hetero_batched, py::arg("models"), py::arg("qpos"),
+        py::arg("qvel"), py::arg("qf"), py::arg("dt"), py::arg("checkpoint_every") = 16, py::arg("stream") = py::none(),
+        "Rollouts of several PinocchioModels (trees of <= 32 joints) in ONE launch: per model (q_T, qd_T, "
+        "checkpoints), as rollout_batched.");
+  m.def("rollout_backward_hetero_batched", &PyModel::rollout_backward_hetero_batched, py::arg("models"),
+        py::arg("qf"), py::arg("checkpoints"), py::arg("dt"), py::arg("grad_qpos"), py::arg("grad_qvel"),
+        py::arg("checkpoint_every") = 16, py::arg("stream") = py::none(),
+        "Adjoints of rollout_hetero_batched in ONE launch: per model (dL/dq0, dL/dqd0, dL/dqf).");
 
   m.def("clik_hetero_batched", &PyModel::clik_hetero_batched, py::arg("models"), py::arg("link_indices"),
         py::arg("poses"), py::arg("initial_qpos") = py::none(), py::arg("active_qmask") = py::none(),

@@ -317,3 +317,56 @@ def test_rollout_workload_size_fused_matches_per_step(tree):
         if tree != "quad18":
             assert np.isfinite(b).all(), name
         assert np.array_equal(a, b, equal_nan=True), (tree, name)
+
+
+def _mixed_case(trees, Bs, T, K, seed):
+    import sapien_amd as sa
+
+    models, per, grads = [], [], []
+    for i, (tree, B) in enumerate(zip(trees, Bs)):
+        rob = S.robot(tree)
+        m, _ = _pair(rob)
+        sa.set_rollout_launch(m, "fused")
+        q0, qd0, tau = _problem(rob, B, T, seed + i)
+        models.append(m)
+        per.append((_t(q0), _t(qd0), _t(tau)))
+        grads.append(_t(np.random.default_rng(seed + 50 + i).normal(size=(B, rob.nv))))
+    fwd = sa.rollout_hetero_batched(models, [p[0] for p in per], [p[1] for p in per], [p[2] for p in per], DT, K)
+    bwd = sa.rollout_backward_hetero_batched(models, [p[2] for p in per], [f[2] for f in fwd], DT, grads, grads, K)
+    for i, m in enumerate(models):
+        ref_f = m.rollout_batched(*per[i], DT, K)
+        ref_b = m.rollout_backward_batched(per[i][2], ref_f[2], DT, grads[i], grads[i], K)
+        for name, a, b in zip(("qT", "qdT", "ckpt", "dq0", "dqd0", "dtau"), (*ref_f, *ref_b), (*fwd[i], *bwd[i])):
+            assert a.shape == b.shape, (trees[i], name)
+            assert np.array_equal(a.cpu().numpy(), b.cpu().numpy(), equal_nan=True), (trees[i], name)
+
+
+def test_mixed_tree_rollouts_match_per_model_at_workload_size():
+    """The benchmark's rollout workload (4096 rollouts of chain7 / arm9 /
+    quad18 / torso30 split 4:3:2:1, 128 steps, checkpoint every 16) in one
+    launch forward and one backward: every model's final state, checkpoints
+    and gradients equal its own one-launch rollout bit for bit (quad18 in
+    FP64, the others in FP32)."""
+    _mixed_case(["chain7", "arm9", "quad18", "torso30"], [1638, 1229, 819, 410], 128, 16, 1530)
+
+
+def test_mixed_tree_rollouts_ragged():
+    """Models in another order (the launch reorders its segments), one with no
+    environments, partly filled blocks, T = 29 with K = 8."""
+    _mixed_case(["torso30", "chain7", "arm9", "quad18", "chain7"], [5, 37, 0, 21, 64], 29, 8, 1540)
+
+
+def test_mixed_tree_rollout_errors():
+    import sapien_amd as sa
+
+    rob = S.chain7()
+    m, _ = _pair(rob)
+    z = torch.zeros(4, 7, device=DEV)
+    with pytest.raises(Exception, match="same number of steps"):
+        sa.rollout_hetero_batched([m, m], [z, z], [z, z], [torch.zeros(4, 3, 7, device=DEV),
+                                                           torch.zeros(4, 2, 7, device=DEV)], DT)
+    with pytest.raises(Exception, match="1 to 8 models"):
+        sa.rollout_hetero_batched([m] * 9, [z] * 9, [z] * 9, [torch.zeros(4, 3, 7, device=DEV)] * 9, DT)
+    sa.set_kernel_variant(m, "thread_per_env")
+    with pytest.raises(Exception, match="lanes-per-joint"):
+        sa.rollout_hetero_batched([m], [z], [z], [torch.zeros(4, 3, 7, device=DEV)], DT)
