@@ -279,6 +279,8 @@ struct sapamd_model {
     }
     for (int j = nj - 1; j >= 0; --j)
       if (J[j].parent >= 0) jr[J[j].parent].subtree += jr[j].subtree;
+    for (int j = 0; j < nj && nj <= 32; ++j)
+      for (int a = j; a >= 0; a = J[a].parent) jr[j].ancmask |= 1u << a;
     hdr->links_le1 = on_root <= 1;
     for (int j = 0; j < nj; ++j)
       if (per_joint[j] > 1) hdr->links_le1 = 0;

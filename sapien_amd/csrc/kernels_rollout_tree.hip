@@ -50,6 +50,20 @@ __device__ __forceinline__ int opaque_lane(int g) {
   return g;
 }
 
+// 1 / x by the hardware reciprocal refined with Newton-Raphson steps
+// r' = r + r (1 - x r) (one for FP32, two for FP64): within an ulp or two of
+// the quotient, without the full IEEE division sequence on the critical path of
+// every elimination step.
+__device__ __forceinline__ float recip(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  return __builtin_fmaf(r, __builtin_fmaf(-x, r, 1.0f), r);
+}
+__device__ __forceinline__ double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+  return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+}
+
 template <int B, int E, class F> __device__ __forceinline__ void static_for(F &&f) {
   if constexpr (B < E) {
     f(std::integral_constant<int, B>{});
@@ -76,7 +90,7 @@ __device__ __forceinline__ Real ldlt_factor(Real (&row)[G], Real *buf, int i, in
     static_for<k, NMAX>([&](auto jc) { c[decltype(jc)::value] = buf[decltype(jc)::value]; });
     wave_sync();
     const bool act = i > k && i < n;
-    const Real l = act ? row[k] / c[k] : Real(0);
+    const Real l = act ? row[k] * recip(c[k]) : Real(0);
     static_for<k + 1, NMAX>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
       row[j] -= l * c[j];
@@ -106,8 +120,9 @@ __device__ __forceinline__ void ldlt_solve(const Real (&row)[G], Real d, Real (*
 #pragma unroll
     for (int m = 0; m < R; ++m) r[m] -= lo[k] * bperm(base4 + 4 * k, r[m]);
   });
+  const Real dinv = recip(d);
 #pragma unroll
-  for (int m = 0; m < R; ++m) r[m] = i < n ? r[m] / d : Real(0);
+  for (int m = 0; m < R; ++m) r[m] = i < n ? r[m] * dinv : Real(0);
   static_for<0, NMAX>([&](auto jc) { A[i][decltype(jc)::value] = row[decltype(jc)::value]; });
   wave_sync();
   Real col[G];
@@ -134,7 +149,7 @@ template <int G, class Real> struct DynLds {
 // One environment's kinematics and M / bias; the factorization is left in L.
 template <int G, class Real> struct TreeFd {
   Tree<G, Real> T;
-  int g, nj;
+  int g, nj, depth;
   bool valid;
   const JointRec<Real> *J;
   XfT<Real> X;
@@ -182,16 +197,19 @@ template <int G, class Real> struct TreeFd {
     L.S[g][3] = col.a.x;
     L.S[g][4] = col.a.y;
     L.S[g][5] = col.a.z;
+#pragma unroll
+    for (int i = 0; i < G; ++i) L.A[g][i] = Real(0);
     wave_sync();
-#pragma unroll 4
-    for (int i = 0; i < nj; ++i) {  // branch-free: loads issued up front, product selected
+    // M_gi is nonzero only for i an ancestor of g (or g): walk the set bits of
+    // the ancestor mask, at most the tree depth of them, instead of every joint
+    uint32_t am = valid ? J->ancmask : 0u;
+    for (int k = 0; k < depth; ++k) {
+      const int i = am ? __builtin_ctz(am) : 0;
       const S6<Real> ci = {{L.S[i][0], L.S[i][1], L.S[i][2]}, {L.S[i][3], L.S[i][4], L.S[i][5]}};
-      const int sub_i = joints_of(M)[i].subtree;
       const Real dij = dot6(ci, Fg);
-      const bool anc = valid && i <= g && g < i + sub_i;  // i ancestor-or-self of g
-      L.A[g][i] = anc ? dij : Real(0);
+      if (am) L.A[g][i] = dij;
+      am &= am - 1u;
     }
-    for (int i = nj; i < G; ++i) L.A[g][i] = Real(0);
     wave_sync();
     static_for<0, G>([&](auto ic) { row[decltype(ic)::value] = L.A[g][decltype(ic)::value]; });
     wave_sync();
@@ -203,6 +221,7 @@ template <int G, class Real>
 __device__ __forceinline__ void init_fd(TreeFd<G, Real> &F, const ModelTable<Real> *M, int g) {
   F.g = g;
   F.nj = __builtin_amdgcn_readfirstlane(M->nj);  // the same in every lane: a scalar
+  F.depth = __builtin_amdgcn_readfirstlane(M->max_depth);
   F.valid = g < F.nj;
   F.J = joints_of(M) + (F.valid ? g : F.nj - 1);
   F.T.g = g;
@@ -513,22 +532,25 @@ template <bool VJP> __global__ __launch_bounds__(kMixBlock) void k_tree_rollout_
     case 0: mixed_segment<8, float, 8, VJP>(S, A, s_mix); break;
     case 1: mixed_segment<16, float, 12, VJP>(S, A, s_mix); break;
     case 2: mixed_segment<16, float, 16, VJP>(S, A, s_mix); break;
-    case 3: mixed_segment<32, float, 24, VJP>(S, A, s_mix); break;
-    case 4: mixed_segment<32, float, 32, VJP>(S, A, s_mix); break;
-    case 5: mixed_segment<8, double, 8, VJP>(S, A, s_mix); break;
-    case 6: mixed_segment<16, double, 12, VJP>(S, A, s_mix); break;
-    case 7: mixed_segment<16, double, 16, VJP>(S, A, s_mix); break;
-    case 8: mixed_segment<32, double, 24, VJP>(S, A, s_mix); break;
+    case 3: mixed_segment<32, float, 20, VJP>(S, A, s_mix); break;
+    case 4: mixed_segment<32, float, 24, VJP>(S, A, s_mix); break;
+    case 5: mixed_segment<32, float, 32, VJP>(S, A, s_mix); break;
+    case 6: mixed_segment<8, double, 8, VJP>(S, A, s_mix); break;
+    case 7: mixed_segment<16, double, 12, VJP>(S, A, s_mix); break;
+    case 8: mixed_segment<16, double, 16, VJP>(S, A, s_mix); break;
+    case 9: mixed_segment<32, double, 20, VJP>(S, A, s_mix); break;
+    case 10: mixed_segment<32, double, 24, VJP>(S, A, s_mix); break;
     default: mixed_segment<32, double, 32, VJP>(S, A, s_mix); break;
   }
 }
 
 // The switch case for nj joints: (G, NMAX) as SAPAMD_TREE_DYN_LAUNCH picks
 // them, FP64 cases after the FP32 ones.
+constexpr int kMixVariants = 12;
 int mixed_variant(int nj, bool fp64) {
   const int G = tree_dyn_group(nj);
-  const int v = G == 8 ? 0 : G == 16 ? (nj <= 12 ? 1 : 2) : (nj <= 24 ? 3 : 4);
-  return v + (fp64 ? 5 : 0);
+  const int v = G == 8 ? 0 : G == 16 ? (nj <= 12 ? 1 : 2) : (nj <= 20 ? 3 : nj <= 24 ? 4 : 5);
+  return v + (fp64 ? kMixVariants / 2 : 0);
 }
 
 template <class Real, int G> constexpr size_t mixed_lds() {
@@ -536,10 +558,10 @@ template <class Real, int G> constexpr size_t mixed_lds() {
 }
 static_assert(mixed_lds<double, 32>() <= 65536, "the mixed launch's LDS must fit the default dynamic LDS limit");
 size_t mixed_lds_bytes(int variant) {
-  constexpr size_t t[10] = {mixed_lds<float, 8>(),  mixed_lds<float, 16>(),  mixed_lds<float, 16>(),
-                            mixed_lds<float, 32>(), mixed_lds<float, 32>(),  mixed_lds<double, 8>(),
-                            mixed_lds<double, 16>(), mixed_lds<double, 16>(), mixed_lds<double, 32>(),
-                            mixed_lds<double, 32>()};
+  constexpr size_t t[kMixVariants] = {mixed_lds<float, 8>(),  mixed_lds<float, 16>(),  mixed_lds<float, 16>(),
+                                      mixed_lds<float, 32>(), mixed_lds<float, 32>(),  mixed_lds<float, 32>(),
+                                      mixed_lds<double, 8>(), mixed_lds<double, 16>(), mixed_lds<double, 16>(),
+                                      mixed_lds<double, 32>(), mixed_lds<double, 32>(), mixed_lds<double, 32>()};
   return t[variant];
 }
 
@@ -547,7 +569,7 @@ size_t mixed_lds_bytes(int variant) {
 template <class Real> constexpr int kStepBlock = sizeof(Real) == 4 ? 256 : 128;
 
 // Launches KERNEL<G, Real, BL, NMAX> for nj joints: G lanes per environment,
-// the factorization bounded by NMAX, the smallest of {8, 12, 16, 24, 32} >= nj
+// the factorization bounded by NMAX, the smallest of {8, 12, 16, 20, 24, 32} >= nj
 // within the group size.
 #define SAPAMD_TREE_DYN_LAUNCH(KERNEL, ...)                                                          \
   do {                                                                                               \
@@ -560,6 +582,8 @@ template <class Real> constexpr int kStepBlock = sizeof(Real) == 4 ? 256 : 128;
       hipLaunchKernelGGL((KERNEL<16, Real, BL, 12>), grid, dim3(BL), 0, s, __VA_ARGS__);             \
     else if (G == 16)                                                                                \
       hipLaunchKernelGGL((KERNEL<16, Real, BL, 16>), grid, dim3(BL), 0, s, __VA_ARGS__);             \
+    else if (nj <= 20)                                                                               \
+      hipLaunchKernelGGL((KERNEL<32, Real, BL, 20>), grid, dim3(BL), 0, s, __VA_ARGS__);             \
     else if (nj <= 24)                                                                               \
       hipLaunchKernelGGL((KERNEL<32, Real, BL, 24>), grid, dim3(BL), 0, s, __VA_ARGS__);             \
     else                                                                                             \

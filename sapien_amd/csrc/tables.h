@@ -39,7 +39,8 @@ template <class T> struct alignas(16) JointRec {
   int32_t lident; // 1 when the attached link's frame is the joint frame (lR = I, lp = 0)
   int32_t anc[5]; // 2^k-th ancestor (k = 0..4), -1 past the root
   int32_t subtree;  // joints in the subtree (the subtree is [j, j + subtree) in depth-first order)
-  int32_t pad2[2];
+  uint32_t ancmask; // bit i set when joint i is this joint or one of its ancestors (joints 0..31; 0 past 31)
+  int32_t pad2;
 };
 
 template <class T> struct alignas(16) LinkRec {
