@@ -149,6 +149,14 @@ struct sapamd_model {
   // smallest of 8, 16, 32 that holds the joints; the autotuner may pick more).
   int tree_G = 0;
   int tree_lanes() const { return tree_G ? tree_G : tree_group(host->njoints()); }
+  // Lanes-per-joint forward dynamics: the factorization of M per root subtree
+  // (M is block diagonal over them) when that shortens it; rows = its size.
+  DynFactorPlan factor_plan() const {
+    std::vector<int> parent;
+    for (const auto &j : host->joints()) parent.push_back(j.parent);
+    return dyn_factor_plan(parent.data(), (int)parent.size());
+  }
+  int dyn_rows = 0;  // factor_plan().rows of the current tables
 
   ~sapamd_model() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -315,6 +323,7 @@ struct sapamd_model {
     HIP_OK(hipStreamSynchronize(stream));
     auto t32 = build_table<float>();
     auto t64 = build_table<double>();
+    dyn_rows = factor_plan().rows;
     if (tab32) HIP_OK(hipFree(tab32));
     if (tab64) HIP_OK(hipFree(tab64));
     tab32 = tab64 = nullptr;
@@ -1319,7 +1328,7 @@ int sapamd_aba_batched(const sapamd_model *m, int64_t B, const float *q, const f
     if (lanes_dynamics(m))
       HIP_OK(launch_tree_step(m->t32(), m->fd64() ? m->t64() : nullptr, m->host->njoints(), B, 0, 0.f,
                               const_cast<float *>(q), const_cast<float *>(qd), tau, m->nv(), qdd, nullptr, 0,
-                              S(stream)));
+                              S(stream), m->dyn_rows));
     else if (m->fd64())
       HIP_OK(launch_fd_ldlt(m->t64(), m->host->njoints(), B, q, qd, tau, qdd, S(stream)));
     else
@@ -1544,7 +1553,7 @@ void step_launch(const sapamd_model *m, int64_t B, float dt, float *q, float *qd
   if (lanes_dynamics(m))
     HIP_OK(launch_tree_step(m->t32(), m->fd64() ? m->t64() : nullptr, m->host->njoints(), B, 1, dt, q, qd, tau,
                             tau_stride, nullptr, save,
-                            save_stride, S(stream)));
+                            save_stride, S(stream), m->dyn_rows));
   else
     HIP_OK(launch_rollout_step<float>(m->t32(), m->fd64() ? m->t64() : nullptr, m->host->njoints(), B, dt, q, qd,
                                       tau, tau_stride, save, save_stride, S(stream)));
@@ -1554,7 +1563,7 @@ void step_vjp_launch(const sapamd_model *m, int64_t B, float dt, const float *st
   if (lanes_dynamics(m))
     HIP_OK(launch_tree_step_vjp(m->t32(), m->fd64() ? m->t64() : nullptr, m->host->njoints(), B, dt, state,
                                 state_stride, tau, tau_stride, aq, aqd,
-                                gtau, S(stream)));
+                                gtau, S(stream), m->dyn_rows));
   else
     HIP_OK(launch_rollout_step_vjp<float>(m->t32(), m->fd64() ? m->t64() : nullptr, m->host->njoints(), B, dt, state,
                                           state_stride, tau, tau_stride, aq, aqd, gtau, S(stream)));
@@ -1584,7 +1593,7 @@ int sapamd_rollout_forward(const sapamd_model *m, int64_t B, int32_t T, int32_t 
     need(tau, "tau");
     if (lanes_dynamics(m) && !m->per_step_rollouts) {
       HIP_OK(launch_tree_rollout(m->t32(), m->fd64() ? m->t64() : nullptr, m->host->njoints(), B, T, K, dt, q, qd,
-                                 tau, ckpt, S(stream)));
+                                 tau, ckpt, S(stream), m->dyn_rows));
       return;
     }
     const int nv = m->nv();
@@ -1608,7 +1617,7 @@ int sapamd_rollout_backward(const sapamd_model *m, int64_t B, int32_t T, int32_t
       need(p, "rollout buffer");
     if (lanes_dynamics(m) && !m->per_step_rollouts) {
       HIP_OK(launch_tree_rollout_vjp(m->t32(), m->fd64() ? m->t64() : nullptr, m->host->njoints(), B, T, K, dt, tau,
-                                     ckpt, seg, aq, aqd, gtau, S(stream)));
+                                     ckpt, seg, aq, aqd, gtau, S(stream), m->dyn_rows));
       return;
     }
     const int nv = m->nv();
@@ -1648,6 +1657,7 @@ RolloutMixArgs rollout_mix(const sapamd_model *const *models, int32_t nmodels, c
     s.fp64 = m->fd64() ? 1 : 0;
     s.tab = s.fp64 ? static_cast<const void *>(m->t64()) : static_cast<const void *>(m->t32());
     s.nj = m->host->njoints();
+    s.rows = m->dyn_rows;
     s.B = B[i];
   }
   A.nseg = nmodels;

@@ -71,23 +71,28 @@ template <int B, int E, class F> __device__ __forceinline__ void static_for(F &&
   }
 }
 
-// LDL^T of the symmetric positive definite n x n matrix whose row i (lower
-// triangle, zeros above the diagonal) lane i holds in row[]: right-looking
-// elimination with the rows in registers, step k broadcasting column k through
-// the group's G-entry LDS buffer. After the call row[j < i] = L_ij (entries
-// above the diagonal hold junk: the update runs over the whole row to need no
-// per-entry select, and nothing reads them); returns D_i. Lanes i >= n hold
-// zero rows. The steps run over the first NMAX >= n columns, a compile-time
-// bound (steps past n change nothing), so no step depends on a runtime value.
-template <int G, int NMAX, class Real>
-__device__ __forceinline__ Real ldlt_factor(Real (&row)[G], Real *buf, int i, int n) {
+// LDL^T of one diagonal block of the symmetric positive definite M (M is block
+// diagonal over the root subtrees of the tree; with one block, M itself). The
+// block's n rows sit on lanes o .. o + n - 1 of the group; lane o + i holds
+// block row i (lower triangle, zeros above the diagonal, block column j at
+// row[j]). Right-looking elimination with the rows in registers, step k
+// broadcasting block column k through the group's G-entry LDS buffer (indexed
+// by lane; lane is this lane's index in the group). After the call
+// row[j < i] = L_ij (entries above the diagonal hold junk: the update runs
+// over the whole row to need no per-entry select, and nothing reads them);
+// returns D_i. Lanes with i >= n hold zero rows. The steps run over the first
+// NMAX >= n columns, a compile-time bound (steps past n change nothing), so no
+// step depends on a runtime value; o + NMAX <= G keeps every read in the group.
+template <int G, int NMAX, class Real, bool BLOCKS>
+__device__ __forceinline__ Real ldlt_factor(Real (&row)[G], Real *buf, int lane, int o, int i, int n) {
   Real d = Real(1);
+  const Real *blk = BLOCKS ? buf + o : buf;
   static_for<0, NMAX>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
-    buf[i] = row[k];
+    buf[lane] = row[k];
     wave_sync();
     Real c[G];
-    static_for<k, NMAX>([&](auto jc) { c[decltype(jc)::value] = buf[decltype(jc)::value]; });
+    static_for<k, NMAX>([&](auto jc) { c[decltype(jc)::value] = blk[decltype(jc)::value]; });
     wave_sync();
     const bool act = i > k && i < n;
     const Real l = act ? row[k] * recip(c[k]) : Real(0);
@@ -101,15 +106,17 @@ __device__ __forceinline__ Real ldlt_factor(Real (&row)[G], Real *buf, int i, in
   return d;
 }
 
-// R right-hand sides r[m] (lane i holding entry i, zero for lanes i >= n) ->
-// (L D L^T)^-1 r. Forward substitution reads L from the lane's row; the rows
-// are then transposed through the group's LDS tile so backward substitution
-// reads lane i's column. Each step broadcasts the finished entry with
-// ds_bpermute; the factor entries are masked once (strictly lower row,
-// strictly below-diagonal column), so the steps need no selects.
-template <int G, int NMAX, class Real, int R>
-__device__ __forceinline__ void ldlt_solve(const Real (&row)[G], Real d, Real (*A)[G + 1], int base4, int i, int n,
-                                           Real (&r)[R]) {
+// R right-hand sides r[m] (block row i's entry on lane o + i, zero for lanes
+// with i >= n) -> (L D L^T)^-1 r over the block. Forward substitution reads L
+// from the lane's row; the rows are then transposed through the group's LDS
+// tile so backward substitution reads lane i's block column. Each step
+// broadcasts the finished entry of block row k (lane o + k) with ds_bpermute;
+// the factor entries are masked once (strictly lower row, strictly
+// below-diagonal column within the block), so the steps need no selects.
+template <int G, int NMAX, class Real, int R, bool BLOCKS>
+__device__ __forceinline__ void ldlt_solve(const Real (&row)[G], Real d, Real (*A)[G + 1], int base4, int lane, int o,
+                                           int i, int n, Real (&r)[R]) {
+  const int ob4 = BLOCKS ? base4 + 4 * o : base4;
   Real lo[G];
   static_for<0, NMAX>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
@@ -118,23 +125,25 @@ __device__ __forceinline__ void ldlt_solve(const Real (&row)[G], Real d, Real (*
   static_for<0, NMAX>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
 #pragma unroll
-    for (int m = 0; m < R; ++m) r[m] -= lo[k] * bperm(base4 + 4 * k, r[m]);
+    for (int m = 0; m < R; ++m) r[m] -= lo[k] * bperm(ob4 + 4 * k, r[m]);
   });
   const Real dinv = recip(d);
 #pragma unroll
   for (int m = 0; m < R; ++m) r[m] = i < n ? r[m] * dinv : Real(0);
-  static_for<0, NMAX>([&](auto jc) { A[i][decltype(jc)::value] = row[decltype(jc)::value]; });
+  static_for<0, NMAX>([&](auto jc) { A[lane][decltype(jc)::value] = row[decltype(jc)::value]; });
   wave_sync();
   Real col[G];
+  const Real(*Ab)[G + 1] = BLOCKS ? A + o : A;
   static_for<0, NMAX>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
-    col[k] = k > i ? A[k][i] : Real(0);
+    // rows past the block belong to the next blocks; a single block is followed by zero rows
+    col[k] = (BLOCKS ? k > i && k < n : k > i) ? Ab[k][i] : Real(0);
   });
   wave_sync();
   static_for<0, NMAX>([&](auto kc) {
     constexpr int k = NMAX - 1 - decltype(kc)::value;
 #pragma unroll
-    for (int m = 0; m < R; ++m) r[m] -= col[k] * bperm(base4 + 4 * k, r[m]);
+    for (int m = 0; m < R; ++m) r[m] -= col[k] * bperm(ob4 + 4 * k, r[m]);
   });
 }
 
@@ -150,6 +159,7 @@ template <int G, class Real> struct DynLds {
 template <int G, class Real> struct TreeFd {
   Tree<G, Real> T;
   int g, nj, depth;
+  int o, nb;  // first lane and size of this joint's block of M (its root subtree, or the whole tree)
   bool valid;
   const JointRec<Real> *J;
   XfT<Real> X;
@@ -157,7 +167,8 @@ template <int G, class Real> struct TreeFd {
 
   // q, qd in joint order (this lane's joint); loads row with the lower
   // triangle of row g of M (zeros above the diagonal), assembled in the
-  // group's LDS tile, and returns the bias b_g = RNEA(q, qd, 0)_g.
+  // group's LDS tile (block columns: column i at i - o), and returns the bias
+  // b_g = RNEA(q, qd, 0)_g.
   __device__ __forceinline__ Real setup(const ModelTable<Real> *M, Real qj, Real qdj, DynLds<G, Real> &L,
                                         Real (&row)[G]) {
     Real s, c;
@@ -207,7 +218,7 @@ template <int G, class Real> struct TreeFd {
       const int i = am ? __builtin_ctz(am) : 0;
       const S6<Real> ci = {{L.S[i][0], L.S[i][1], L.S[i][2]}, {L.S[i][3], L.S[i][4], L.S[i][5]}};
       const Real dij = dot6(ci, Fg);
-      if (am) L.A[g][i] = dij;
+      if (am) L.A[g][i - o] = dij;
       am &= am - 1u;
     }
     wave_sync();
@@ -217,7 +228,7 @@ template <int G, class Real> struct TreeFd {
   }
 };
 
-template <int G, class Real>
+template <int G, class Real, bool BLOCKS>
 __device__ __forceinline__ void init_fd(TreeFd<G, Real> &F, const ModelTable<Real> *M, int g) {
   F.g = g;
   F.nj = __builtin_amdgcn_readfirstlane(M->nj);  // the same in every lane: a scalar
@@ -229,6 +240,13 @@ __device__ __forceinline__ void init_fd(TreeFd<G, Real> &F, const ModelTable<Rea
 #pragma unroll
   for (int k = 0; k < Tree<G, Real>::K; ++k) F.T.anc[k] = F.valid ? F.J->anc[k] : -1;
   F.T.size = F.valid ? F.J->subtree : 0;
+  if constexpr (BLOCKS) {  // per root subtree: the lowest ancestor bit is its root
+    F.o = F.valid ? (int)__builtin_ctz(F.J->ancmask) : 0;
+    F.nb = F.valid ? joints_of(M)[F.o].subtree : 0;
+  } else {
+    F.o = 0;
+    F.nb = F.nj;
+  }
 }
 
 // Stage the model table in LDS (block-wide) and return the LDS copy.
@@ -243,32 +261,32 @@ __device__ __forceinline__ const ModelTable<Real> *stage_table(const ModelTable<
 }
 
 // Forward dynamics of this lane's joint at (q, qd, tau) (joint order).
-template <int G, int NMAX, class Real>
+template <int G, int NMAX, class Real, bool BLOCKS>
 __device__ __forceinline__ Real lane_fd(TreeFd<G, Real> &F, const ModelTable<Real> *M, DynLds<G, Real> &L, Real qj,
                                         Real qdj, Real tj) {
   Real row[G];
   const Real bias = F.setup(M, qj, qdj, L, row);
-  const int i = opaque_lane(F.g);
-  const Real d = ldlt_factor<G, NMAX, Real>(row, L.y, i, F.nj);
+  const int i = opaque_lane(F.g - F.o);
+  const Real d = ldlt_factor<G, NMAX, Real, BLOCKS>(row, L.y, F.g, F.o, i, F.nb);
   Real r[1] = {tj - bias};
-  ldlt_solve<G, NMAX, Real, 1>(row, d, L.A, F.T.base4, i, F.nj, r);
+  ldlt_solve<G, NMAX, Real, 1, BLOCKS>(row, d, L.A, F.T.base4, F.g, F.o, i, F.nb, r);
   return r[0];
 }
 
 // Adjoint of one semi-implicit Euler step at the pre-step state (q, qd) for
 // this lane's joint: (aq, aqd) post-step adjoint in, pre-step adjoint out;
 // returns dL/dtau_j = lambda_j.
-template <int G, int NMAX, class Real>
+template <int G, int NMAX, class Real, bool BLOCKS>
 __device__ __forceinline__ Real lane_step_vjp(TreeFd<G, Real> &F, const ModelTable<Real> *M, DynLds<G, Real> &L,
                                               float dt, Real qj, Real qdj, Real tj, Real &aqj, Real &aqdj) {
   Real row[G];
   const Real bias = F.setup(M, qj, qdj, L, row);
-  const int i = opaque_lane(F.g);
-  const Real d = ldlt_factor<G, NMAX, Real>(row, L.y, i, F.nj);
+  const int i = opaque_lane(F.g - F.o);
+  const Real d = ldlt_factor<G, NMAX, Real, BLOCKS>(row, L.y, F.g, F.o, i, F.nb);
   const Real dtr = dt;
   const Real cj = aqdj + dtr * aqj;
   Real r[2] = {tj - bias, dtr * cj};  // qdd and lambda = M^-1 (dt c) in one pass
-  ldlt_solve<G, NMAX, Real, 2>(row, d, L.A, F.T.base4, i, F.nj, r);
+  ldlt_solve<G, NMAX, Real, 2, BLOCKS>(row, d, L.A, F.T.base4, F.g, F.o, i, F.nb, r);
   const Real qddj = r[0], lam = r[1];
 
   // RNEA vector-Jacobian product at (q, qd, qdd) for the upstream lambda
@@ -315,7 +333,7 @@ __device__ __forceinline__ Real lane_step_vjp(TreeFd<G, Real> &F, const ModelTab
 
 // mode 0: forward dynamics qdd = FD(q, qd, tau) into out (B, nv)
 // mode 1: rollout step in place on q, qd; save (pre-step q | qd) when given
-template <int G, class Real, int BLOCK, int NMAX>
+template <int G, class Real, int BLOCK, int NMAX, bool BLOCKS>
 __global__ __launch_bounds__(BLOCK) void k_tree_step(const ModelTable<Real> *__restrict__ Mg, int64_t B, int mode,
                                                      float dt, float *__restrict__ q, float *__restrict__ qd,
                                                      const float *__restrict__ tau, int64_t tau_stride,
@@ -328,11 +346,11 @@ __global__ __launch_bounds__(BLOCK) void k_tree_step(const ModelTable<Real> *__r
   const ModelTable<Real> *M = stage_table(Mg, s_tab);
   DynLds<G, Real> &L = s_dyn[threadIdx.x / G];
   TreeFd<G, Real> F;
-  init_fd<G, Real>(F, M, g);
+  init_fd<G, Real, BLOCKS>(F, M, g);
   const int src = F.T.base4 + 4 * F.J->vuser;
   Real qj = bperm(src, qu), qdj = bperm(src, qdu), tj = bperm(src, tu);
   if (!F.valid) qj = qdj = tj = Real(0);
-  const Real qddj = lane_fd<G, NMAX, Real>(F, M, L, qj, qdj, tj);
+  const Real qddj = lane_fd<G, NMAX, Real, BLOCKS>(F, M, L, qj, qdj, tj);
   if (!live) return;
   if (mode == 0) {
     if (F.valid) out[e * nv + F.J->vuser] = (float)qddj;
@@ -352,7 +370,7 @@ __global__ __launch_bounds__(BLOCK) void k_tree_step(const ModelTable<Real> *__r
 
 // Adjoint of one step; state = pre-step (q | qd) rows, aq / aqd hold the
 // post-step adjoint on entry and the pre-step adjoint on exit.
-template <int G, class Real, int BLOCK, int NMAX>
+template <int G, class Real, int BLOCK, int NMAX, bool BLOCKS>
 __global__ __launch_bounds__(BLOCK) void k_tree_step_vjp(const ModelTable<Real> *__restrict__ Mg, int64_t B, float dt,
                                                          const float *__restrict__ state, int64_t state_stride,
                                                          const float *__restrict__ tau, int64_t tau_stride,
@@ -365,12 +383,12 @@ __global__ __launch_bounds__(BLOCK) void k_tree_step_vjp(const ModelTable<Real> 
   const ModelTable<Real> *M = stage_table(Mg, s_tab);
   DynLds<G, Real> &L = s_dyn[threadIdx.x / G];
   TreeFd<G, Real> F;
-  init_fd<G, Real>(F, M, g);
+  init_fd<G, Real, BLOCKS>(F, M, g);
   const int src = F.T.base4 + 4 * F.J->vuser;
   Real qj = bperm(src, qu), qdj = bperm(src, qdu), tj = bperm(src, tu);
   Real aqj = bperm(src, aqu), aqdj = bperm(src, aqdu);
   if (!F.valid) qj = qdj = tj = aqj = aqdj = Real(0);
-  const Real lam = lane_step_vjp<G, NMAX, Real>(F, M, L, dt, qj, qdj, tj, aqj, aqdj);
+  const Real lam = lane_step_vjp<G, NMAX, Real, BLOCKS>(F, M, L, dt, qj, qdj, tj, aqj, aqdj);
   if (!live || !F.valid) return;
   const int u = F.J->vuser;
   aq[e * nv + u] = (float)aqj;
@@ -384,7 +402,7 @@ __global__ __launch_bounds__(BLOCK) void k_tree_step_vjp(const ModelTable<Real> 
 // tau (B, T, nv); ckpt (B, ceil(T / K), 2 nv) receives the pre-step state of
 // steps 0, K, 2K, ... when given; q, qd (B, nv) hold the initial state on entry
 // and the final state on exit.
-template <int G, class Real, int NMAX>
+template <int G, class Real, int NMAX, bool BLOCKS>
 __device__ __forceinline__ void rollout_env(const ModelTable<Real> *M, DynLds<G, Real> &L, int64_t env, int64_t B,
                                             int T, int K, float dt, float *__restrict__ q, float *__restrict__ qd,
                                             const float *__restrict__ tau, float *__restrict__ ckpt) {
@@ -393,7 +411,7 @@ __device__ __forceinline__ void rollout_env(const ModelTable<Real> *M, DynLds<G,
   const int64_t e = live ? env : B - 1;  // tail groups recompute the last environment and store nothing
   const int nv = M->nv;
   TreeFd<G, Real> F;
-  init_fd<G, Real>(F, M, g);
+  init_fd<G, Real, BLOCKS>(F, M, g);
   const int u = F.J->vuser;  // lanes past the last joint repeat its dof and store nothing
   const bool owner = live && F.valid;
   const int64_t C = (T + K - 1) / K;
@@ -408,7 +426,7 @@ __device__ __forceinline__ void rollout_env(const ModelTable<Real> *M, DynLds<G,
       c[u] = qs;
       c[nv + u] = qds;
     }
-    const Real qddj = lane_fd<G, NMAX, Real>(F, M, L, F.valid ? Real(qs) : Real(0), F.valid ? Real(qds) : Real(0),
+    const Real qddj = lane_fd<G, NMAX, Real, BLOCKS>(F, M, L, F.valid ? Real(qs) : Real(0), F.valid ? Real(qds) : Real(0),
                                        F.valid ? Real(tcur) : Real(0));
     const float v1 = qds + dt * (float)qddj;
     qs = qs + dt * v1;
@@ -425,7 +443,7 @@ __device__ __forceinline__ void rollout_env(const ModelTable<Real> *M, DynLds<G,
 // seg (B, K, 2 nv; each lane reads back only what it wrote), then the step
 // adjoints in reverse. aq, aqd: dL/d(final state) in, dL/d(initial state)
 // out; gtau (B, T, nv) = dL/dtau.
-template <int G, class Real, int NMAX>
+template <int G, class Real, int NMAX, bool BLOCKS>
 __device__ __forceinline__ void rollout_vjp_env(const ModelTable<Real> *M, DynLds<G, Real> &L, int64_t env,
                                                 int64_t B, int T, int K, float dt, const float *__restrict__ tau,
                                                 const float *__restrict__ ckpt, float *__restrict__ seg,
@@ -436,7 +454,7 @@ __device__ __forceinline__ void rollout_vjp_env(const ModelTable<Real> *M, DynLd
   const int64_t e = live ? env : B - 1;
   const int nv = M->nv;
   TreeFd<G, Real> F;
-  init_fd<G, Real>(F, M, g);
+  init_fd<G, Real, BLOCKS>(F, M, g);
   const int u = F.J->vuser;
   const bool owner = live && F.valid;
   const int C = (T + K - 1) / K;
@@ -454,7 +472,7 @@ __device__ __forceinline__ void rollout_vjp_env(const ModelTable<Real> *M, DynLd
         sg[(int64_t)(t - t0) * 2 * nv + nv] = qds;
       }
       if (t + 1 == t1) break;  // the last pre-step state needs no step after it
-      const Real qddj = lane_fd<G, NMAX, Real>(F, M, L, F.valid ? Real(qs) : Real(0), F.valid ? Real(qds) : Real(0),
+      const Real qddj = lane_fd<G, NMAX, Real, BLOCKS>(F, M, L, F.valid ? Real(qs) : Real(0), F.valid ? Real(qds) : Real(0),
                                          F.valid ? Real(tr[(int64_t)t * nv]) : Real(0));
       const float v1 = qds + dt * (float)qddj;
       qs = qs + dt * v1;
@@ -464,7 +482,7 @@ __device__ __forceinline__ void rollout_vjp_env(const ModelTable<Real> *M, DynLd
       const Real qj = F.valid ? Real(sg[(int64_t)(t - t0) * 2 * nv]) : Real(0);
       const Real qdj = F.valid ? Real(sg[(int64_t)(t - t0) * 2 * nv + nv]) : Real(0);
       const Real tj = F.valid ? Real(tr[(int64_t)t * nv]) : Real(0);
-      const Real lam = lane_step_vjp<G, NMAX, Real>(F, M, L, dt, qj, qdj, tj, aqj, aqdj);
+      const Real lam = lane_step_vjp<G, NMAX, Real, BLOCKS>(F, M, L, dt, qj, qdj, tj, aqj, aqdj);
       // the one-step kernel hands the adjoint on in FP32: round the same way
       aqj = F.valid ? Real((float)aqj) : Real(0);
       aqdj = F.valid ? Real((float)aqdj) : Real(0);
@@ -478,7 +496,7 @@ __device__ __forceinline__ void rollout_vjp_env(const ModelTable<Real> *M, DynLd
 }
 
 // T steps in one launch, one model: blocks of BLOCK threads, G per environment.
-template <int G, class Real, int BLOCK, int NMAX>
+template <int G, class Real, int BLOCK, int NMAX, bool BLOCKS>
 __global__ __launch_bounds__(BLOCK) void k_tree_rollout(const ModelTable<Real> *__restrict__ Mg, int64_t B, int T,
                                                         int K, float dt, float *__restrict__ q,
                                                         float *__restrict__ qd, const float *__restrict__ tau,
@@ -486,12 +504,12 @@ __global__ __launch_bounds__(BLOCK) void k_tree_rollout(const ModelTable<Real> *
   __shared__ float4 s_tab[kDynTableWords4<Real>];
   __shared__ DynLds<G, Real> s_dyn[BLOCK / G];
   const ModelTable<Real> *M = stage_table(Mg, s_tab);
-  rollout_env<G, Real, NMAX>(M, s_dyn[threadIdx.x / G], (blockIdx.x * (int64_t)BLOCK + threadIdx.x) / G, B, T, K,
+  rollout_env<G, Real, NMAX, BLOCKS>(M, s_dyn[threadIdx.x / G], (blockIdx.x * (int64_t)BLOCK + threadIdx.x) / G, B, T, K,
                              dt, q, qd, tau, ckpt);
 }
 
 // Adjoint of k_tree_rollout in one launch.
-template <int G, class Real, int BLOCK, int NMAX>
+template <int G, class Real, int BLOCK, int NMAX, bool BLOCKS>
 __global__ __launch_bounds__(BLOCK) void k_tree_rollout_vjp(const ModelTable<Real> *__restrict__ Mg, int64_t B, int T,
                                                             int K, float dt, const float *__restrict__ tau,
                                                             const float *__restrict__ ckpt, float *__restrict__ seg,
@@ -500,7 +518,7 @@ __global__ __launch_bounds__(BLOCK) void k_tree_rollout_vjp(const ModelTable<Rea
   __shared__ float4 s_tab[kDynTableWords4<Real>];
   __shared__ DynLds<G, Real> s_dyn[BLOCK / G];
   const ModelTable<Real> *M = stage_table(Mg, s_tab);
-  rollout_vjp_env<G, Real, NMAX>(M, s_dyn[threadIdx.x / G], (blockIdx.x * (int64_t)BLOCK + threadIdx.x) / G, B, T,
+  rollout_vjp_env<G, Real, NMAX, BLOCKS>(M, s_dyn[threadIdx.x / G], (blockIdx.x * (int64_t)BLOCK + threadIdx.x) / G, B, T,
                                  K, dt, tau, ckpt, seg, aq, aqd, gtau);
 }
 
@@ -511,16 +529,16 @@ __global__ __launch_bounds__(BLOCK) void k_tree_rollout_vjp(const ModelTable<Rea
 // same bits as in its own model's launch.
 constexpr int kMixBlock = 128;
 
-template <int G, class Real, int NMAX, bool VJP>
+template <int G, class Real, int NMAX, bool BLOCKS, bool VJP>
 __device__ __forceinline__ void mixed_segment(const RolloutSegment &S, const RolloutMixArgs &A, float4 *lds) {
   const ModelTable<Real> *M = stage_table(static_cast<const ModelTable<Real> *>(S.tab), lds);
   DynLds<G, Real> *dyn = reinterpret_cast<DynLds<G, Real> *>(lds + kDynTableWords4<Real>);
   const int64_t env = ((int64_t)(blockIdx.x - S.block0) * kMixBlock + threadIdx.x) / G;
   if constexpr (VJP)
-    rollout_vjp_env<G, Real, NMAX>(M, dyn[threadIdx.x / G], env, S.B, A.T, A.K, A.dt, S.tau, S.ckpt, S.seg, S.aq,
+    rollout_vjp_env<G, Real, NMAX, BLOCKS>(M, dyn[threadIdx.x / G], env, S.B, A.T, A.K, A.dt, S.tau, S.ckpt, S.seg, S.aq,
                                    S.aqd, S.gtau);
   else
-    rollout_env<G, Real, NMAX>(M, dyn[threadIdx.x / G], env, S.B, A.T, A.K, A.dt, S.q, S.qd, S.tau, S.ckpt_out);
+    rollout_env<G, Real, NMAX, BLOCKS>(M, dyn[threadIdx.x / G], env, S.B, A.T, A.K, A.dt, S.q, S.qd, S.tau, S.ckpt_out);
 }
 
 template <bool VJP> __global__ __launch_bounds__(kMixBlock) void k_tree_rollout_mixed(RolloutMixArgs A) {
@@ -529,27 +547,35 @@ template <bool VJP> __global__ __launch_bounds__(kMixBlock) void k_tree_rollout_
   while (s + 1 < A.nseg && (int64_t)blockIdx.x >= A.seg[s + 1].block0) ++s;
   const RolloutSegment &S = A.seg[s];
   switch (S.variant) {  // uniform per block
-    case 0: mixed_segment<8, float, 8, VJP>(S, A, s_mix); break;
-    case 1: mixed_segment<16, float, 12, VJP>(S, A, s_mix); break;
-    case 2: mixed_segment<16, float, 16, VJP>(S, A, s_mix); break;
-    case 3: mixed_segment<32, float, 20, VJP>(S, A, s_mix); break;
-    case 4: mixed_segment<32, float, 24, VJP>(S, A, s_mix); break;
-    case 5: mixed_segment<32, float, 32, VJP>(S, A, s_mix); break;
-    case 6: mixed_segment<8, double, 8, VJP>(S, A, s_mix); break;
-    case 7: mixed_segment<16, double, 12, VJP>(S, A, s_mix); break;
-    case 8: mixed_segment<16, double, 16, VJP>(S, A, s_mix); break;
-    case 9: mixed_segment<32, double, 20, VJP>(S, A, s_mix); break;
-    case 10: mixed_segment<32, double, 24, VJP>(S, A, s_mix); break;
-    default: mixed_segment<32, double, 32, VJP>(S, A, s_mix); break;
+    case 0: mixed_segment<8, float, 8, false, VJP>(S, A, s_mix); break;
+    case 1: mixed_segment<16, float, 12, false, VJP>(S, A, s_mix); break;
+    case 2: mixed_segment<16, float, 16, false, VJP>(S, A, s_mix); break;
+    case 3: mixed_segment<32, float, 20, false, VJP>(S, A, s_mix); break;
+    case 4: mixed_segment<32, float, 24, false, VJP>(S, A, s_mix); break;
+    case 5: mixed_segment<32, float, 32, false, VJP>(S, A, s_mix); break;
+    case 6: mixed_segment<32, float, 20, true, VJP>(S, A, s_mix); break;
+    case 7: mixed_segment<32, float, 24, true, VJP>(S, A, s_mix); break;
+    case 8: mixed_segment<8, double, 8, false, VJP>(S, A, s_mix); break;
+    case 9: mixed_segment<16, double, 12, false, VJP>(S, A, s_mix); break;
+    case 10: mixed_segment<16, double, 16, false, VJP>(S, A, s_mix); break;
+    case 11: mixed_segment<32, double, 20, false, VJP>(S, A, s_mix); break;
+    case 12: mixed_segment<32, double, 24, false, VJP>(S, A, s_mix); break;
+    case 13: mixed_segment<32, double, 32, false, VJP>(S, A, s_mix); break;
+    case 14: mixed_segment<32, double, 20, true, VJP>(S, A, s_mix); break;
+    default: mixed_segment<32, double, 24, true, VJP>(S, A, s_mix); break;
   }
 }
 
-// The switch case for nj joints: (G, NMAX) as SAPAMD_TREE_DYN_LAUNCH picks
-// them, FP64 cases after the FP32 ones.
-constexpr int kMixVariants = 12;
-int mixed_variant(int nj, bool fp64) {
-  const int G = tree_dyn_group(nj);
-  const int v = G == 8 ? 0 : G == 16 ? (nj <= 12 ? 1 : 2) : (nj <= 20 ? 3 : nj <= 24 ? 4 : 5);
+// The switch case for nj joints and a factorization of `rows` rows: (G, NMAX)
+// as SAPAMD_TREE_DYN_LAUNCH picks them, per-block factorizations (rows < nj)
+// after the single-block ones, FP64 cases after the FP32 ones.
+constexpr int kMixVariants = 16;
+int mixed_variant(int nj, int rows, bool fp64) {
+  const int G = tree_dyn_group(nj), ns = tree_dyn_steps(rows, G);
+  const int v = G == 8    ? 0
+                : G == 16 ? (ns == 12 ? 1 : 2)
+                : rows < nj ? (ns == 20 ? 6 : 7)
+                            : (ns == 20 ? 3 : ns == 24 ? 4 : 5);
   return v + (fp64 ? kMixVariants / 2 : 0);
 }
 
@@ -558,40 +584,49 @@ template <class Real, int G> constexpr size_t mixed_lds() {
 }
 static_assert(mixed_lds<double, 32>() <= 65536, "the mixed launch's LDS must fit the default dynamic LDS limit");
 size_t mixed_lds_bytes(int variant) {
-  constexpr size_t t[kMixVariants] = {mixed_lds<float, 8>(),  mixed_lds<float, 16>(),  mixed_lds<float, 16>(),
-                                      mixed_lds<float, 32>(), mixed_lds<float, 32>(),  mixed_lds<float, 32>(),
-                                      mixed_lds<double, 8>(), mixed_lds<double, 16>(), mixed_lds<double, 16>(),
-                                      mixed_lds<double, 32>(), mixed_lds<double, 32>(), mixed_lds<double, 32>()};
+  constexpr size_t t[kMixVariants] = {
+      mixed_lds<float, 8>(),   mixed_lds<float, 16>(),  mixed_lds<float, 16>(),  mixed_lds<float, 32>(),
+      mixed_lds<float, 32>(),  mixed_lds<float, 32>(),  mixed_lds<float, 32>(),  mixed_lds<float, 32>(),
+      mixed_lds<double, 8>(),  mixed_lds<double, 16>(), mixed_lds<double, 16>(), mixed_lds<double, 32>(),
+      mixed_lds<double, 32>(), mixed_lds<double, 32>(), mixed_lds<double, 32>(), mixed_lds<double, 32>()};
   return t[variant];
 }
 
 // FP32: 256-thread blocks; FP64: 128-thread blocks (half the LDS per block).
 template <class Real> constexpr int kStepBlock = sizeof(Real) == 4 ? 256 : 128;
 
-// Launches KERNEL<G, Real, BL, NMAX> for nj joints: G lanes per environment,
-// the factorization bounded by NMAX, the smallest of {8, 12, 16, 20, 24, 32} >= nj
-// within the group size.
+// Launches KERNEL<G, Real, BL, NMAX, BLOCKS> for nj joints and a
+// factorization of nr rows: G lanes per environment, the elimination bounded
+// by NMAX = tree_dyn_steps(nr, G), per root subtree when nr < nj (planned by
+// dyn_factor_plan only for 32-lane groups and 20 or 24 steps).
 #define SAPAMD_TREE_DYN_LAUNCH(KERNEL, ...)                                                          \
   do {                                                                                               \
     constexpr int BL = kStepBlock<Real>;                                                             \
     const int G = tree_dyn_group(nj);                                                                \
+    const int NS = tree_dyn_steps(nr, G);                                                            \
+    const bool blocks = nr < nj;                                                                     \
     const dim3 grid((unsigned)((B * G + BL - 1) / BL));                                              \
+    if (blocks && (G != 32 || (NS != 20 && NS != 24))) return hipErrorNotSupported;                  \
     if (G == 8)                                                                                      \
-      hipLaunchKernelGGL((KERNEL<8, Real, BL, 8>), grid, dim3(BL), 0, s, __VA_ARGS__);               \
-    else if (G == 16 && nj <= 12)                                                                    \
-      hipLaunchKernelGGL((KERNEL<16, Real, BL, 12>), grid, dim3(BL), 0, s, __VA_ARGS__);             \
+      hipLaunchKernelGGL((KERNEL<8, Real, BL, 8, false>), grid, dim3(BL), 0, s, __VA_ARGS__);        \
+    else if (G == 16 && NS == 12)                                                                    \
+      hipLaunchKernelGGL((KERNEL<16, Real, BL, 12, false>), grid, dim3(BL), 0, s, __VA_ARGS__);      \
     else if (G == 16)                                                                                \
-      hipLaunchKernelGGL((KERNEL<16, Real, BL, 16>), grid, dim3(BL), 0, s, __VA_ARGS__);             \
-    else if (nj <= 20)                                                                               \
-      hipLaunchKernelGGL((KERNEL<32, Real, BL, 20>), grid, dim3(BL), 0, s, __VA_ARGS__);             \
-    else if (nj <= 24)                                                                               \
-      hipLaunchKernelGGL((KERNEL<32, Real, BL, 24>), grid, dim3(BL), 0, s, __VA_ARGS__);             \
+      hipLaunchKernelGGL((KERNEL<16, Real, BL, 16, false>), grid, dim3(BL), 0, s, __VA_ARGS__);      \
+    else if (NS == 20 && blocks)                                                                     \
+      hipLaunchKernelGGL((KERNEL<32, Real, BL, 20, true>), grid, dim3(BL), 0, s, __VA_ARGS__);       \
+    else if (NS == 24 && blocks)                                                                     \
+      hipLaunchKernelGGL((KERNEL<32, Real, BL, 24, true>), grid, dim3(BL), 0, s, __VA_ARGS__);       \
+    else if (NS == 20)                                                                               \
+      hipLaunchKernelGGL((KERNEL<32, Real, BL, 20, false>), grid, dim3(BL), 0, s, __VA_ARGS__);      \
+    else if (NS == 24)                                                                               \
+      hipLaunchKernelGGL((KERNEL<32, Real, BL, 24, false>), grid, dim3(BL), 0, s, __VA_ARGS__);      \
     else                                                                                             \
-      hipLaunchKernelGGL((KERNEL<32, Real, BL, 32>), grid, dim3(BL), 0, s, __VA_ARGS__);             \
+      hipLaunchKernelGGL((KERNEL<32, Real, BL, 32, false>), grid, dim3(BL), 0, s, __VA_ARGS__);      \
   } while (0)
 
 template <class Real>
-hipError_t step_impl(const ModelTable<Real> *tab, int nj, int64_t B, int mode, float dt, float *q, float *qd,
+hipError_t step_impl(const ModelTable<Real> *tab, int nj, int nr, int64_t B, int mode, float dt, float *q, float *qd,
                      const float *tau, int64_t tau_stride, float *out, float *save, int64_t save_stride,
                      hipStream_t s) {
   SAPAMD_TREE_DYN_LAUNCH(k_tree_step, tab, B, mode, dt, q, qd, tau, tau_stride, out, save, save_stride);
@@ -599,7 +634,7 @@ hipError_t step_impl(const ModelTable<Real> *tab, int nj, int64_t B, int mode, f
 }
 
 template <class Real>
-hipError_t step_vjp_impl(const ModelTable<Real> *tab, int nj, int64_t B, float dt, const float *state,
+hipError_t step_vjp_impl(const ModelTable<Real> *tab, int nj, int nr, int64_t B, float dt, const float *state,
                          int64_t state_stride, const float *tau, int64_t tau_stride, float *aq, float *aqd,
                          float *gtau, hipStream_t s) {
   SAPAMD_TREE_DYN_LAUNCH(k_tree_step_vjp, tab, B, dt, state, state_stride, tau, tau_stride, aq, aqd, gtau);
@@ -607,14 +642,14 @@ hipError_t step_vjp_impl(const ModelTable<Real> *tab, int nj, int64_t B, float d
 }
 
 template <class Real>
-hipError_t rollout_impl(const ModelTable<Real> *tab, int nj, int64_t B, int T, int K, float dt, float *q, float *qd,
+hipError_t rollout_impl(const ModelTable<Real> *tab, int nj, int nr, int64_t B, int T, int K, float dt, float *q, float *qd,
                         const float *tau, float *ckpt, hipStream_t s) {
   SAPAMD_TREE_DYN_LAUNCH(k_tree_rollout, tab, B, T, K, dt, q, qd, tau, ckpt);
   return hipGetLastError();
 }
 
 template <class Real>
-hipError_t rollout_vjp_impl(const ModelTable<Real> *tab, int nj, int64_t B, int T, int K, float dt, const float *tau,
+hipError_t rollout_vjp_impl(const ModelTable<Real> *tab, int nj, int nr, int64_t B, int T, int K, float dt, const float *tau,
                             const float *ckpt, float *seg, float *aq, float *aqd, float *gtau, hipStream_t s) {
   SAPAMD_TREE_DYN_LAUNCH(k_tree_rollout_vjp, tab, B, T, K, dt, tau, ckpt, seg, aq, aqd, gtau);
   return hipGetLastError();
@@ -624,40 +659,69 @@ hipError_t rollout_vjp_impl(const ModelTable<Real> *tab, int nj, int64_t B, int 
 
 int tree_dyn_group(int nj) { return nj >= 1 && nj <= 8 ? 8 : nj <= 16 ? 16 : nj <= 32 ? 32 : 0; }
 
+int tree_dyn_steps(int rows, int G) {
+  if (G == 8) return 8;
+  if (G == 16) return rows <= 12 ? 12 : 16;
+  return rows <= 20 ? 20 : rows <= 24 ? 24 : 32;
+}
+
+DynFactorPlan dyn_factor_plan(const int *parent, int nj) {
+  const int G = tree_dyn_group(nj);
+  DynFactorPlan p{false, nj};
+  if (!G) return p;
+  // root subtrees: [j, j + size) in depth-first order for every joint j hanging from the universe
+  int roots = 0, maxo = 0, maxsize = 0;
+  for (int j = 0; j < nj; ++j) {
+    if (parent[j] >= 0) continue;
+    int e = j + 1;
+    while (e < nj && parent[e] >= j) ++e;  // descendants follow their root contiguously
+    ++roots;
+    maxo = j;
+    maxsize = e - j > maxsize ? e - j : maxsize;
+  }
+  const int steps = tree_dyn_steps(maxsize, G);
+  if (G == 32 && roots > 1 && steps < tree_dyn_steps(nj, G) && maxo + steps <= G) p = {true, maxsize};
+  return p;
+}
+
 hipError_t launch_tree_step(const ModelTable<float> *tab, const ModelTable<double> *tab64, int nj, int64_t B,
                             int mode, float dt, float *q, float *qd, const float *tau, int64_t tau_stride, float *out,
-                            float *save, int64_t save_stride, hipStream_t s) {
+                            float *save, int64_t save_stride, hipStream_t s, int rows) {
   if (B <= 0) return hipSuccess;
-  if (!tree_dyn_group(nj)) return hipErrorNotSupported;
-  return tab64 ? step_impl<double>(tab64, nj, B, mode, dt, q, qd, tau, tau_stride, out, save, save_stride, s)
-               : step_impl<float>(tab, nj, B, mode, dt, q, qd, tau, tau_stride, out, save, save_stride, s);
+  if (!tree_dyn_group(nj) || rows < 0 || rows > nj) return hipErrorNotSupported;
+  const int nr = rows ? rows : nj;
+  return tab64 ? step_impl<double>(tab64, nj, nr, B, mode, dt, q, qd, tau, tau_stride, out, save, save_stride, s)
+               : step_impl<float>(tab, nj, nr, B, mode, dt, q, qd, tau, tau_stride, out, save, save_stride, s);
 }
 
 hipError_t launch_tree_step_vjp(const ModelTable<float> *tab, const ModelTable<double> *tab64, int nj, int64_t B,
                                 float dt, const float *state, int64_t state_stride, const float *tau,
-                                int64_t tau_stride, float *aq, float *aqd, float *gtau, hipStream_t s) {
+                                int64_t tau_stride, float *aq, float *aqd, float *gtau, hipStream_t s, int rows) {
   if (B <= 0) return hipSuccess;
-  if (!tree_dyn_group(nj)) return hipErrorNotSupported;
-  return tab64 ? step_vjp_impl<double>(tab64, nj, B, dt, state, state_stride, tau, tau_stride, aq, aqd, gtau, s)
-               : step_vjp_impl<float>(tab, nj, B, dt, state, state_stride, tau, tau_stride, aq, aqd, gtau, s);
+  if (!tree_dyn_group(nj) || rows < 0 || rows > nj) return hipErrorNotSupported;
+  const int nr = rows ? rows : nj;
+  return tab64 ? step_vjp_impl<double>(tab64, nj, nr, B, dt, state, state_stride, tau, tau_stride, aq, aqd, gtau, s)
+               : step_vjp_impl<float>(tab, nj, nr, B, dt, state, state_stride, tau, tau_stride, aq, aqd, gtau, s);
 }
 
 hipError_t launch_tree_rollout(const ModelTable<float> *tab, const ModelTable<double> *tab64, int nj, int64_t B,
                                int T, int K, float dt, float *q, float *qd, const float *tau, float *ckpt,
-                               hipStream_t s) {
+                               hipStream_t s, int rows) {
   if (B <= 0 || T <= 0) return hipSuccess;
-  if (!tree_dyn_group(nj) || K < 1) return hipErrorNotSupported;
-  return tab64 ? rollout_impl<double>(tab64, nj, B, T, K, dt, q, qd, tau, ckpt, s)
-               : rollout_impl<float>(tab, nj, B, T, K, dt, q, qd, tau, ckpt, s);
+  if (!tree_dyn_group(nj) || K < 1 || rows < 0 || rows > nj) return hipErrorNotSupported;
+  const int nr = rows ? rows : nj;
+  return tab64 ? rollout_impl<double>(tab64, nj, nr, B, T, K, dt, q, qd, tau, ckpt, s)
+               : rollout_impl<float>(tab, nj, nr, B, T, K, dt, q, qd, tau, ckpt, s);
 }
 
 hipError_t launch_tree_rollout_vjp(const ModelTable<float> *tab, const ModelTable<double> *tab64, int nj, int64_t B,
                                    int T, int K, float dt, const float *tau, const float *ckpt, float *seg, float *aq,
-                                   float *aqd, float *gtau, hipStream_t s) {
+                                   float *aqd, float *gtau, hipStream_t s, int rows) {
   if (B <= 0 || T <= 0) return hipSuccess;
-  if (!tree_dyn_group(nj) || K < 1) return hipErrorNotSupported;
-  return tab64 ? rollout_vjp_impl<double>(tab64, nj, B, T, K, dt, tau, ckpt, seg, aq, aqd, gtau, s)
-               : rollout_vjp_impl<float>(tab, nj, B, T, K, dt, tau, ckpt, seg, aq, aqd, gtau, s);
+  if (!tree_dyn_group(nj) || K < 1 || rows < 0 || rows > nj) return hipErrorNotSupported;
+  const int nr = rows ? rows : nj;
+  return tab64 ? rollout_vjp_impl<double>(tab64, nj, nr, B, T, K, dt, tau, ckpt, seg, aq, aqd, gtau, s)
+               : rollout_vjp_impl<float>(tab, nj, nr, B, T, K, dt, tau, ckpt, seg, aq, aqd, gtau, s);
 }
 
 hipError_t launch_rollout_mixed(RolloutMixArgs A, bool vjp, hipStream_t s) {
@@ -666,8 +730,11 @@ hipError_t launch_rollout_mixed(RolloutMixArgs A, bool vjp, hipStream_t s) {
   // heaviest first: FP64 before FP32, then by lanes per environment and joints
   RolloutSegment *b = A.seg, *e = A.seg + A.nseg;
   for (RolloutSegment *x = b; x != e; ++x) {
-    if (!tree_dyn_group(x->nj) || x->B < 0) return hipErrorInvalidValue;
-    x->variant = mixed_variant(x->nj, x->fp64 != 0);
+    if (!tree_dyn_group(x->nj) || x->B < 0 || x->rows < 0 || x->rows > x->nj) return hipErrorInvalidValue;
+    if (x->rows && x->rows < x->nj &&
+        (tree_dyn_group(x->nj) != 32 || tree_dyn_steps(x->rows, 32) > 24))
+      return hipErrorInvalidValue;
+    x->variant = mixed_variant(x->nj, x->rows ? x->rows : x->nj, x->fp64 != 0);
   }
   std::stable_sort(b, e, [](const RolloutSegment &x, const RolloutSegment &y) {
     return x.variant != y.variant ? x.variant > y.variant : x.nj > y.nj;

@@ -223,28 +223,42 @@ hipError_t launch_fd_ldlt(const ModelTable<double> *tab64, int nj, int64_t B, co
 // outputs and state either way). mode 0: out = qdd (B, nv); mode 1: one rollout
 // step in place on q, qd (save as launch_rollout_step). hipErrorNotSupported
 // above 32 joints.
+// rows: the factorization size, nj, or the largest root subtree when the
+// factorization runs per root subtree (dyn_factor_plan); 0 = nj.
 int tree_dyn_group(int nj);
+// The compile-time elimination bound for a factorization of `rows` rows in a
+// group of G lanes: the smallest of 8, 12, 16, 20, 24, 32 >= rows (<= G).
+int tree_dyn_steps(int rows, int G);
+// Whether the factorization runs per root subtree (M is block diagonal over
+// them) and its size: per subtree when there are several, it shortens the
+// elimination, and every block's rows stay in the group (first lane + steps <= G).
+struct DynFactorPlan {
+  bool blocks;
+  int rows;
+};
+DynFactorPlan dyn_factor_plan(const int *parent, int nj);
 hipError_t launch_tree_step(const ModelTable<float> *tab, const ModelTable<double> *tab64, int nj, int64_t B,
                             int mode, float dt, float *q, float *qd, const float *tau, int64_t tau_stride, float *out,
-                            float *save, int64_t save_stride, hipStream_t s);
+                            float *save, int64_t save_stride, hipStream_t s, int rows = 0);
 hipError_t launch_tree_step_vjp(const ModelTable<float> *tab, const ModelTable<double> *tab64, int nj, int64_t B,
                                 float dt, const float *state, int64_t state_stride, const float *tau,
-                                int64_t tau_stride, float *aq, float *aqd, float *gtau, hipStream_t s);
+                                int64_t tau_stride, float *aq, float *aqd, float *gtau, hipStream_t s, int rows = 0);
 // T rollout steps in one launch (state in registers; ckpt (B, ceil(T/K), 2 nv)
 // optional), and its checkpointed adjoint in one launch (seg (B, K, 2 nv)).
 hipError_t launch_tree_rollout(const ModelTable<float> *tab, const ModelTable<double> *tab64, int nj, int64_t B,
                                int T, int K, float dt, float *q, float *qd, const float *tau, float *ckpt,
-                               hipStream_t s);
+                               hipStream_t s, int rows = 0);
 hipError_t launch_tree_rollout_vjp(const ModelTable<float> *tab, const ModelTable<double> *tab64, int nj, int64_t B,
                                    int T, int K, float dt, const float *tau, const float *ckpt, float *seg, float *aq,
-                                   float *aqd, float *gtau, hipStream_t s);
+                                   float *aqd, float *gtau, hipStream_t s, int rows = 0);
 // One model of a mixed-tree rollout launch: its table (ModelTable<float>, or
-// ModelTable<double> when fp64), batch, first block and buffers as in
+// ModelTable<double> when fp64), joints and factorization rows (as for
+// launch_tree_rollout), batch, first block and buffers as in
 // launch_tree_rollout (q, qd, tau, ckpt) / launch_tree_rollout_vjp (tau, ckpt,
 // seg, aq, aqd, gtau). variant and block0 are filled by launch_rollout_mixed.
 struct RolloutSegment {
   const void *tab;
-  int32_t nj, fp64, variant;
+  int32_t nj, rows, fp64, variant;
   int64_t B, block0;
   float *q, *qd;
   const float *tau, *ckpt;
