@@ -11,8 +11,10 @@ graph, bracketed by HIP events on the launch stream and by the host clock.
 
 Beside the headline number it reports:
   roofline      algorithmic bytes per launch / mean launch duration (HIP events
-                on the launch stream over the timed graph replays), against
-                8 TB/s HBM3E; traffic from rocprofv3 FETCH_SIZE + WRITE_SIZE;
+                on the launch stream around the timed graph replay, per launch;
+                they include the graph's launch latency, so they read high at
+                few steps), against 8 TB/s HBM3E; traffic from rocprofv3
+                FETCH_SIZE + WRITE_SIZE;
                 the FP32 VALU work (rocprofv3 SQ_INSTS_VALU_* counters) against
                 the 157.3 TFLOP/s vector peak; the bandwidth an elementwise
                 copy reaches on this GPU;
@@ -427,13 +429,60 @@ def main():
     if not graph_ok:
         raise RuntimeError("the graph replay differs from the eager launch on buffer set 0")
     value = B * args.steps / elapsed
-    # average launch duration on the launch stream over the timed region
+
+    # Not the roofline's figure: the per-launch time of the timed region without
+    # the graph's launch latency:
+    # the events around one replay also count the ~30 us between the replay
+    # call and its first kernel, which 20 launches do not amortize. Replays of
+    # the timed graph and of a graph of one launch (set 0), each after the
+    # same cache flush: kernel_ms = (t_steps - t_1) / (steps - 1), the time each
+    # launch adds when they run back to back (what a kernel trace's span per
+    # launch shows).
+    graph1 = torch.cuda.CUDAGraph()
+    cap.wait_stream(stream)
+    with torch.cuda.graph(graph1, stream=cap):
+        step(0, torch.cuda.current_stream(dev).cuda_stream)
+    stream.wait_stream(cap)
+    torch.cuda.synchronize()
+
+    def replay_ms(g):
+        flush.sum()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        g.replay()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    t_n = sorted([graph_ms * args.steps] + [replay_ms(graph) for _ in range(4)])
+    t_1 = sorted(replay_ms(graph1) for _ in range(5))
+    t_n, t_1 = t_n[len(t_n) // 2], t_1[len(t_1) // 2]
+    per_launch_ms = (t_n - t_1) / (args.steps - 1) if args.steps > 1 else t_1
+    # the roofline's launch duration: HIP events over the timed region, per launch
     kern_ms = graph_ms
+
+    # each kernel's own start and stop (hipExtLaunchKernel events) over the
+    # timed region's launches queued back to back behind a sleep kernel; these
+    # end when the kernel's writes are issued, before the end-of-kernel cache
+    # release a trace counts, so they read lower (reported, not the roofline's)
+    timed_launch = sa.native_fk_jac_rnea_timed(m)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    for a, b in evs:  # torch creates an event at its first record
+        a.record(stream)
+        b.record(stream)
+    flush.sum()
+    tune.hold_gpu(2.0 + 0.05 * args.steps)
+    for i, (a, b) in enumerate(evs):
+        s = sets[i % NSETS]
+        timed_launch(B, s["q"], s["qd"], s["qdd"], s["P"], s["J"], s["T"], a.cuda_event, b.cuda_event,
+                     stream.cuda_stream)
+    torch.cuda.synchronize()
+    kernel_durations = [a.elapsed_time(b) for a, b in evs]
+    ext_ms = sum(kernel_durations) / len(kernel_durations)
     achieved = BYTES_PER_EVAL * B / (kern_ms * 1e-3) / 1e9
 
-    # not the metric: the same graph replayed a second time, and `steps` eager
-    # launches (one C-ABI call per step from Python), each after the same
-    # cache flush, HIP events around each sequence
+    # not the metric: `steps` eager launches (one C-ABI call per step from
+    # Python) after the same cache flush, HIP events around the sequence
     def timed_seq(fn):
         flush.sum()
         torch.cuda.synchronize()
@@ -444,7 +493,6 @@ def main():
         torch.cuda.synchronize()
         return (time.perf_counter() - s0) / args.steps, e0.elapsed_time(e1) / args.steps
 
-    second_wall, second_ms = timed_seq(graph.replay)
     eager_wall, eager_ms = timed_seq(lambda: [step(i) for i in range(args.steps)])
     del flush
     traffic = None if args.no_traffic else measure_traffic(tuned[B][:2] if tuned else None)
@@ -467,8 +515,16 @@ def main():
     launch_modes = {
         "graph": {"ms_per_step": elapsed / args.steps * 1e3, "events_ms_per_launch": graph_ms,
                   "replay_matches_eager": graph_ok,
-                  "what": f"the timed region: one replay of a HIP graph of {args.steps} launches"},
-        "graph_second_replay": {"ms_per_step": second_wall * 1e3, "events_ms_per_launch": second_ms},
+                  "what": f"the timed region: one replay of a HIP graph of {args.steps} launches; its events "
+                          f"(roofline.kernel_ms) also span the graph's launch latency"},
+        "per_launch_without_graph_latency": {
+            "ms": per_launch_ms, "graph_ms_steps": t_n, "graph_ms_one_launch": t_1,
+            "what": f"(median replay of the {args.steps}-launch graph - median replay of a one-launch graph) / "
+                    f"{args.steps - 1}, each replay after the cache flush"},
+        "kernel_events": {"ms_per_kernel": ext_ms, "min_ms": min(kernel_durations), "max_ms": max(kernel_durations),
+                          "what": f"the timed region's {args.steps} launches queued back to back, each kernel's "
+                                  f"own start and stop (hipExtLaunchKernel events; they end before the "
+                                  f"end-of-kernel cache release)"},
         "eager": {"ms_per_step": eager_wall * 1e3, "events_ms_per_launch": eager_ms,
                   "what": "one C-ABI call per step from Python"},
     }
@@ -540,15 +596,9 @@ def main():
                           torch.empty(Bs, n, device=dev)])
         for i in range(2 * nset):
             launch(Bs, *bsets[i % nset], stream.cuda_stream)
-        s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         reps_t = max(50, nset)
-        torch.cuda.synchronize()
-        s0.record(stream)
-        for i in range(reps_t):
-            launch(Bs, *bsets[i % nset], stream.cuda_stream)
-        s1.record(stream)
-        torch.cuda.synchronize()
-        kus = s0.elapsed_time(s1) / reps_t * 1e3
+        # each kernel's own duration, the launches queued back to back
+        kus = tune.kernel_us(lambda i, e0, e1: timed_launch(Bs, *bsets[i % nset], e0, e1, stream.cuda_stream), reps_t)
         gbs = BYTES_PER_EVAL * Bs / (kus * 1e-6) / 1e9
         sweep.append({"envs": Bs, "kernel_us": kus, "evals_s": Bs / (kus * 1e-6), "achieved_gbs": gbs,
                       "frac": gbs / HBM_PEAK_GBS, "shape": list(shp) if shp else None, "buffer_sets": nset})

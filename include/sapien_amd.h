@@ -151,6 +151,14 @@ int sapamd_get_clik_schedule(int32_t *cap, int32_t *g1, int32_t *g2);
  * the throughput-critical entry point. Any of poses/J/tau may be NULL to skip it. */
 int sapamd_fk_jac_rnea_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd,
                                const float *qdd, float *poses, float *J, float *tau, void *stream);
+/* The same launch with the kernel's own start and stop recorded into two
+ * caller-created hipEvent_t (hipExtLaunchKernel), so hipEventElapsedTime gives
+ * the kernel's execution time without the dispatch gaps around it: kernel
+ * timing without a profiler (lanes-per-joint kernels only; not for graph
+ * capture). No reference counterpart. */
+int sapamd_fk_jac_rnea_batched_timed(const sapamd_model *m, int64_t B, const float *q, const float *qd,
+                                     const float *qdd, float *poses, float *J, float *tau, void *start_event,
+                                     void *stop_event, void *stream);
 /* The same over padded buffers, without copies: q/qd/qdd/tau rows every *_ld
  * floats (0 = nv), link poses every pose_link_stride floats (0 = 7) within
  * rows of pose_env_stride floats (0 = L * 7); J stays (B, 6, nv). This reads a

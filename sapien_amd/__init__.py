@@ -32,6 +32,14 @@ def native_fk_jac_rnea(model):
     return launcher_fk_jac_rnea(model)
 
 
+def native_fk_jac_rnea_timed(model):
+    """The same launcher recording the kernel's own start and stop into two HIP
+    events (kernel timing without a profiler)."""
+    from .native import launcher_fk_jac_rnea_timed
+
+    return launcher_fk_jac_rnea_timed(model)
+
+
 _VARIANTS = {"auto": 0, "generic": 1, "thread_per_env": 1, "fast": 2, "lanes_per_joint": 2}
 _VARIANT_NAMES = {1: "thread_per_env", 2: "lanes_per_joint"}
 

@@ -925,16 +925,18 @@ namespace {
 // Launch of the fused FK / Jacobian / RNEA kernel family for one model; flags
 // carry the outputs and RNEA variants, o the pointers and strides.
 void fused_launch(const sapamd_model *m, int64_t B, int flags, const float *q, const float *qd, const float *qdd,
-                  const KinOut<float> &o, void *stream) {
+                  const KinOut<float> &o, void *stream, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
   if (m->effective_variant() == SAPAMD_KERNEL_LANES_PER_JOINT) {
     if (m->chain_kernel()) {
       int G, C;
       m->shape_for(B, &G, &C);
-      HIP_OK(launch_chain_fused(m->t32(), m->host->njoints(), B, flags, q, qd, qdd, o, S(stream), G, C));
+      HIP_OK(launch_chain_fused(m->t32(), m->host->njoints(), B, flags, q, qd, qdd, o, S(stream), G, C, ev0, ev1));
     } else {
-      HIP_OK(launch_tree_fused(m->t32(), m->host->njoints(), B, flags, q, qd, qdd, o, S(stream), m->tree_lanes()));
+      HIP_OK(launch_tree_fused(m->t32(), m->host->njoints(), B, flags, q, qd, qdd, o, S(stream), m->tree_lanes(), ev0,
+                               ev1));
     }
   } else {
+    if (ev0 || ev1) throw std::runtime_error("kernel timing events need the lanes-per-joint kernels");
     // the generic kernel reads qd / qdd whenever RNEA runs: alias the unused ones to q
     HIP_OK(launch_kinematics<float>(m->t32(), m->host->njoints(), B, flags, q,
                                     (flags & KF_QD0) || !qd ? q : qd, (flags & KF_QDD0) || !qdd ? q : qdd, o,
@@ -946,9 +948,26 @@ void check_ld(int64_t ld, int64_t min, const char *what) {
 }
 }  // namespace
 
+namespace {
+void fk_jac_rnea(const sapamd_model *m, int64_t B, const float *q, int64_t q_ld, const float *qd, int64_t qd_ld,
+                 const float *qdd, int64_t qdd_ld, float *poses, int64_t pose_env_stride, int64_t pose_link_stride,
+                 float *J, float *tau, int64_t tau_ld, void *stream, hipEvent_t ev0 = nullptr,
+                 hipEvent_t ev1 = nullptr);
+}  // namespace
+
 int sapamd_fk_jac_rnea_batched(const sapamd_model *m, int64_t B, const float *q, const float *qd,
                                const float *qdd, float *poses, float *J, float *tau, void *stream) {
   return sapamd_fk_jac_rnea_strided(m, B, q, 0, qd, 0, qdd, 0, poses, 0, 0, J, tau, 0, stream);
+}
+
+int sapamd_fk_jac_rnea_batched_timed(const sapamd_model *m, int64_t B, const float *q, const float *qd,
+                                     const float *qdd, float *poses, float *J, float *tau, void *start_event,
+                                     void *stop_event, void *stream) {
+  return guarded([&] {
+    if (!start_event || !stop_event) throw std::invalid_argument("start and stop events must not be null");
+    fk_jac_rnea(m, B, q, 0, qd, 0, qdd, 0, poses, 0, 0, J, tau, 0, stream, static_cast<hipEvent_t>(start_event),
+                static_cast<hipEvent_t>(stop_event));
+  });
 }
 
 int sapamd_fk_jac_rnea_strided(const sapamd_model *m, int64_t B, const float *q, int64_t q_ld, const float *qd,
@@ -956,6 +975,16 @@ int sapamd_fk_jac_rnea_strided(const sapamd_model *m, int64_t B, const float *q,
                                int64_t pose_env_stride, int64_t pose_link_stride, float *J, float *tau,
                                int64_t tau_ld, void *stream) {
   return guarded([&] {
+    fk_jac_rnea(m, B, q, q_ld, qd, qd_ld, qdd, qdd_ld, poses, pose_env_stride, pose_link_stride, J, tau, tau_ld,
+                stream);
+  });
+}
+
+namespace {
+void fk_jac_rnea(const sapamd_model *m, int64_t B, const float *q, int64_t q_ld, const float *qd, int64_t qd_ld,
+                 const float *qdd, int64_t qdd_ld, float *poses, int64_t pose_env_stride, int64_t pose_link_stride,
+                 float *J, float *tau, int64_t tau_ld, void *stream, hipEvent_t ev0, hipEvent_t ev1) {
+  {
     check_batched(m, B);
     if (B == 0) return;
     need(q, "q");
@@ -977,9 +1006,10 @@ int sapamd_fk_jac_rnea_strided(const sapamd_model *m, int64_t B, const float *q,
     if (poses) { flags |= KF_POSES; o.poses = poses; if (L == 0) throw std::runtime_error("link index out of bound"); }
     if (J) { flags |= KF_JFULL; o.jfull = J; }
     if (tau) { need(qd, "qd"); need(qdd, "qdd"); flags |= KF_RNEA; o.tau = tau; }
-    fused_launch(m, B, flags, q, qd, qdd, o, stream);
-  });
+    fused_launch(m, B, flags, q, qd, qdd, o, stream, ev0, ev1);
+  }
 }
+}  // namespace
 
 int sapamd_link_state_strided(const sapamd_model *m, int64_t B, const float *q, int64_t q_ld, const float *qd,
                               int64_t qd_ld, float *body, int64_t body_env_stride, int64_t body_link_stride,

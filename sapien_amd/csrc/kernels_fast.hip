@@ -21,6 +21,7 @@
 // ds_bpermute. Each lane then writes the poses of the links attached to its own
 // joints (one per joint when the model allows it), its Jacobian columns and
 // torques.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "lanes.h"
@@ -265,11 +266,15 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
 
 template <int G, int C>
 void launch_gc(const ModelTable<float> *tab, int64_t B, int flags, const float *q, const float *qd, const float *qdd,
-               KinOut<float> out, hipStream_t s, const Layout &lay) {
+               KinOut<float> out, hipStream_t s, const Layout &lay, hipEvent_t ev0, hipEvent_t ev1) {
   const int64_t envs_per_block = 256 / G;
   dim3 grid((unsigned)((B + envs_per_block - 1) / envs_per_block));
-  hipLaunchKernelGGL((k_chain_fused<G, C>), grid, dim3(256), 0, s, tab, B, flags, q, qd, qdd, out.poses, out.jfull,
-                     out.tau, lay);
+  if (ev0 || ev1)  // the kernel's own start and stop recorded into the caller's events
+    hipExtLaunchKernelGGL((k_chain_fused<G, C>), grid, dim3(256), 0, s, ev0, ev1, 0, tab, B, flags, q, qd, qdd,
+                          out.poses, out.jfull, out.tau, lay);
+  else
+    hipLaunchKernelGGL((k_chain_fused<G, C>), grid, dim3(256), 0, s, tab, B, flags, q, qd, qdd, out.poses, out.jfull,
+                       out.tau, lay);
 }
 
 }  // namespace
@@ -294,7 +299,8 @@ void default_chain_shape(int nj, int *G, int *C) {
 int fast_chain_group(int nj) { return nj >= 1 && nj <= 16 ? 1 : 0; }
 
 hipError_t launch_chain_fused(const ModelTable<float> *tab, int nj, int64_t B, int flags, const float *q,
-                              const float *qd, const float *qdd, KinOut<float> out, hipStream_t s, int G, int C) {
+                              const float *qd, const float *qdd, KinOut<float> out, hipStream_t s, int G, int C,
+                              hipEvent_t ev0, hipEvent_t ev1) {
   if (B <= 0) return hipSuccess;
   if (nj < 1 || G * C < nj) return hipErrorNotSupported;
   if (!(flags & KF_RNEA) || (flags & KF_QD0) || !qd) qd = q;  // read but unused
@@ -308,7 +314,7 @@ hipError_t launch_chain_fused(const ModelTable<float> *tab, int nj, int64_t B, i
   lay.pose_link = out.pose_link ? out.pose_link : 7;
   lay.pose_env = out.pose_env;  // 0: L * 7, resolved in the kernel from the table
 #define SAPAMD_SHAPE(g_, c_) \
-  if (G == g_ && C == c_) { launch_gc<g_, c_>(tab, B, flags, q, qd, qdd, out, s, lay); return hipGetLastError(); }
+  if (G == g_ && C == c_) { launch_gc<g_, c_>(tab, B, flags, q, qd, qdd, out, s, lay, ev0, ev1); return hipGetLastError(); }
   SAPAMD_SHAPE(16, 1) SAPAMD_SHAPE(8, 1) SAPAMD_SHAPE(8, 2) SAPAMD_SHAPE(4, 2) SAPAMD_SHAPE(4, 4)
   SAPAMD_SHAPE(2, 4) SAPAMD_SHAPE(2, 8) SAPAMD_SHAPE(1, 8) SAPAMD_SHAPE(1, 16)
 #undef SAPAMD_SHAPE

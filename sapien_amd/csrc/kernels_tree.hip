@@ -13,6 +13,7 @@
 //    decomposition of size_j. Exact, without prefix differences (no cancellation).
 // The CRBA lane j writes row j of M: for i an ancestor-or-self of j the entry is
 // col_i . (Yc_j col_j), for i a descendant it is col_j . (Yc_i col_i), else 0.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -472,7 +473,8 @@ __global__ __launch_bounds__(256) void k_hetero(HeteroArgs args, int flags) {
 int tree_group(int nj) { return nj < 1 ? 0 : nj <= 8 ? 8 : nj <= 16 ? 16 : nj <= 32 ? 32 : 0; }
 
 hipError_t launch_tree_fused(const ModelTable<float> *tab, int nj, int64_t B, int flags, const float *q,
-                             const float *qd, const float *qdd, const KinOut<float> &out, hipStream_t s, int G) {
+                             const float *qd, const float *qdd, const KinOut<float> &out, hipStream_t s, int G,
+                             hipEvent_t ev0, hipEvent_t ev1) {
   if (B <= 0) return hipSuccess;
   if (!tree_group(nj)) return hipErrorNotSupported;
   if (G == 0) G = tree_group(nj);
@@ -489,9 +491,17 @@ hipError_t launch_tree_fused(const ModelTable<float> *tab, int nj, int64_t B, in
   dim3 grid((unsigned)((B + per_block - 1) / per_block));
   if (!out.M) flags &= ~KF_CRBA;  // the CRBA tile is allocated only with an M output
   const unsigned lds = (flags & KF_CRBA) ? (unsigned)crba_lds_bytes(G) : 0u;
-  if (G == 8) hipLaunchKernelGGL(k_tree_fused<8>, grid, dim3(256), lds, s, tab, B, flags, io);
-  else if (G == 16) hipLaunchKernelGGL(k_tree_fused<16>, grid, dim3(256), lds, s, tab, B, flags, io);
-  else hipLaunchKernelGGL(k_tree_fused<32>, grid, dim3(256), lds, s, tab, B, flags, io);
+  if (ev0 || ev1) {  // the kernel's own start and stop recorded into the caller's events
+    if (G == 8) hipExtLaunchKernelGGL(k_tree_fused<8>, grid, dim3(256), lds, s, ev0, ev1, 0, tab, B, flags, io);
+    else if (G == 16) hipExtLaunchKernelGGL(k_tree_fused<16>, grid, dim3(256), lds, s, ev0, ev1, 0, tab, B, flags, io);
+    else hipExtLaunchKernelGGL(k_tree_fused<32>, grid, dim3(256), lds, s, ev0, ev1, 0, tab, B, flags, io);
+  } else if (G == 8) {
+    hipLaunchKernelGGL(k_tree_fused<8>, grid, dim3(256), lds, s, tab, B, flags, io);
+  } else if (G == 16) {
+    hipLaunchKernelGGL(k_tree_fused<16>, grid, dim3(256), lds, s, tab, B, flags, io);
+  } else {
+    hipLaunchKernelGGL(k_tree_fused<32>, grid, dim3(256), lds, s, tab, B, flags, io);
+  }
   return hipGetLastError();
 }
 

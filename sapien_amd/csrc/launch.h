@@ -43,8 +43,11 @@ struct HeteroSegment {
 // RNEA + CRBA in one pass (flags select outputs).
 int tree_group(int nj);
 // G = lanes per environment (8, 16 or 32, >= nj; 0 = tree_group(nj)).
+// ev0 / ev1 (optional): events that receive the kernel's own start and stop
+// (hipExtLaunchKernel), for kernel timing without a profiler.
 hipError_t launch_tree_fused(const ModelTable<float> *tab, int nj, int64_t B, int flags, const float *q,
-                             const float *qd, const float *qdd, const KinOut<float> &out, hipStream_t s, int G = 0);
+                             const float *qd, const float *qdd, const KinOut<float> &out, hipStream_t s, int G = 0,
+                             hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 template <class T> struct VjpArgs;
 // Tree version of the fused forward + reverse pass (pose gradients need one
 // user link per joint).
@@ -174,8 +177,10 @@ int fast_chain_group(int nj);
 int chain_shape_count();
 void chain_shape(int i, int *G, int *C);
 void default_chain_shape(int nj, int *G, int *C);
+// ev0 / ev1 as for launch_tree_fused.
 hipError_t launch_chain_fused(const ModelTable<float> *tab, int nj, int64_t B, int flags, const float *q,
-                              const float *qd, const float *qdd, KinOut<float> out, hipStream_t s, int G, int C);
+                              const float *qd, const float *qdd, KinOut<float> out, hipStream_t s, int G, int C,
+                              hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
 // Reverse mode (vector-Jacobian products) of FK link poses and RNEA.
 // VJP mode: upstream gradients gtau (B, nv) and/or gposes (B, L, 7).

@@ -92,6 +92,31 @@ def launcher_fk_jac_rnea(model):
     return launch
 
 
+def launcher_fk_jac_rnea_timed(model):
+    """f(B, q, qd, qdd, P, J, T, start_event, stop_event, stream): the fused
+    launch with the kernel's own start and stop recorded into two HIP events
+    (raw hipEvent_t handles, e.g. torch.cuda.Event(enable_timing=True).cuda_event
+    after one record)."""
+    L = lib()
+    h = handle(model)
+    fn = L.sapamd_fk_jac_rnea_batched_timed
+    P, I64 = ctypes.c_void_p, ctypes.c_int64
+    fn.argtypes = [P, I64, P, P, P, P, P, P, P, P, P]
+    fn.restype = ctypes.c_int
+
+    def ptr(x):
+        if x is None:
+            return None
+        return x if isinstance(x, int) else x.data_ptr()
+
+    def launch(B, q, qd, qdd, Pp, J, T, ev0, ev1, stream):
+        rc = fn(h, B, ptr(q), ptr(qd), ptr(qdd), ptr(Pp), ptr(J), ptr(T), ev0, ev1, stream)
+        if rc:
+            check(rc)
+
+    return launch
+
+
 def launcher_hetero(models, B, q, qd, qdd, poses, J, tau, M):
     """f(stream) launching sapamd_hetero_batched (FK + J + RNEA + CRBA of
     several models in one launch) over fixed per-model device tensors (lists,

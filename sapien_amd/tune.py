@@ -72,16 +72,18 @@ def _save_cache(d):
 def autotune(model, batch_sizes=(16384,), iters=100, use_cache=True, report=None):
     """Pick and install the fastest chain-kernel shape for each batch size.
 
-    Each candidate is timed with HIP events over launches that rotate through
-    enough input/output sets to miss the Infinity Cache, three rounds over all
-    candidates, best round each. Returns {B: (G, C, microseconds)}. Models that
+    Each candidate is timed by its kernels' own durations (kernel_us: launches
+    queued back to back, each recording its kernel's start and stop into HIP
+    events; host and dispatch overheads, a profiler's included, do not count)
+    over launches that rotate through enough input/output sets to miss the
+    Infinity Cache, three rounds over all candidates, best round each. Returns {B: (G, C, microseconds)}. Models that
     are not serial chains of <= 16 joints (where the thread-per-environment
     kernels run) return {}.
     """
     import torch
 
     from . import kernel_variant
-    from .native import launcher_fk_jac_rnea
+    from .native import launcher_fk_jac_rnea, launcher_fk_jac_rnea_timed
 
     if kernel_variant(model) != "lanes_per_joint" or not model.is_chain or model.nv > 16:
         return {}
@@ -90,6 +92,7 @@ def autotune(model, batch_sizes=(16384,), iters=100, use_cache=True, report=None
     gpu = torch.cuda.get_device_name(dev)
     cache = _load_cache() if use_cache else {}
     launch = launcher_fk_jac_rnea(model)
+    timed = launcher_fk_jac_rnea_timed(model)
     stream = torch.cuda.current_stream(dev)
     L = model.nlinks
     lo, hi = model.joint_limits()
@@ -120,14 +123,7 @@ def autotune(model, batch_sizes=(16384,), iters=100, use_cache=True, report=None
                 set_shape(model, B, G, C)
                 for i in range(10):
                     launch(B, *sets[i % nsets], stream.cuda_stream)
-                a = torch.cuda.Event(enable_timing=True)
-                b = torch.cuda.Event(enable_timing=True)
-                a.record(stream)
-                for i in range(iters):
-                    launch(B, *sets[i % nsets], stream.cuda_stream)
-                b.record(stream)
-                b.synchronize()
-                t = a.elapsed_time(b) / iters * 1e3
+                t = kernel_us(lambda i, e0, e1: timed(B, *sets[i % nsets], e0, e1, stream.cuda_stream), iters)
                 best[(G, C)] = min(best.get((G, C), t), t)
         del sets
         timings = [(t, G, C) for (G, C), t in best.items()]
@@ -167,6 +163,49 @@ def _gpu_name():
     import torch
 
     return torch.cuda.get_device_name(torch.device("cuda", torch.cuda.current_device()))
+
+
+_SLEEP_MS_PER_MCYCLE = None
+
+
+def hold_gpu(ms):
+    """Keep the current stream's GPU busy for about `ms` milliseconds (a sleep
+    kernel), so that the launches the host queues meanwhile run back to back."""
+    import torch
+
+    global _SLEEP_MS_PER_MCYCLE
+    if _SLEEP_MS_PER_MCYCLE is None:
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        torch.cuda._sleep(1 << 20)
+        b.record()
+        b.synchronize()
+        _SLEEP_MS_PER_MCYCLE = max(a.elapsed_time(b), 1e-3) / ((1 << 20) / 1e6)
+    torch.cuda._sleep(int(min(ms / _SLEEP_MS_PER_MCYCLE, 4e3) * 1e6))
+
+
+_EVENTS = []
+
+
+def kernel_us(run_timed, iters):
+    """Mean kernel duration in microseconds of `iters` launches queued behind a
+    sleep kernel (so they run back to back, as in a captured graph);
+    run_timed(i, start, stop) makes launch i recording its kernel's own start
+    and stop into the two raw event handles. Host launch overhead and the
+    dispatch gaps between launches are not counted (nor a profiler's)."""
+    import torch
+
+    while len(_EVENTS) < iters:
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()  # torch creates an event at its first record
+        b.record()
+        _EVENTS.append((a, b))
+    torch.cuda.synchronize()
+    hold_gpu(2.0 + 0.05 * iters)
+    for i in range(iters):
+        run_timed(i, _EVENTS[i][0].cuda_event, _EVENTS[i][1].cuda_event)
+    torch.cuda.synchronize()
+    return 1e3 * sum(_EVENTS[i][0].elapsed_time(_EVENTS[i][1]) for i in range(iters)) / iters
 
 
 def _time(fn, iters, rounds=3):

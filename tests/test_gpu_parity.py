@@ -537,6 +537,40 @@ def test_fused_launch_graph_replay_matches_eager():
         torch.cuda.synchronize()
 
 
+@pytest.mark.parametrize("tree", ["chain7", "arm9"])
+def test_timed_launch_records_the_kernel(tree):
+    """sapamd_fk_jac_rnea_batched_timed: the same outputs as the plain launch,
+    and the two events bracket the kernel (positive, shorter than the host
+    time around launch + synchronize); the thread-per-env variant refuses."""
+    import time
+
+    import sapien_amd as sa
+
+    rob = S.robot(tree)
+    m, _ = _pair(rob)
+    B, L, n = 4096, rob.nlinks, rob.nv
+    q, qd, qdd, _ = (_t(x) for x in S.samples(rob, B, 1810))
+    out = [torch.full((B, L, 7), float("nan"), device=DEV), torch.full((B, 6, n), float("nan"), device=DEV),
+           torch.full((B, n), float("nan"), device=DEV)]
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    e1.record()
+    torch.cuda.synchronize()
+    timed = sa.native_fk_jac_rnea_timed(m)
+    t0 = time.perf_counter()
+    timed(B, q, qd, qdd, *out, e0.cuda_event, e1.cuda_event, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    host_ms = (time.perf_counter() - t0) * 1e3
+    ref = m.compute_fk_jacobian_rnea_batched(q, qd, qdd)
+    assert all(torch.equal(a, b) for a, b in zip(out, ref))
+    ms = e0.elapsed_time(e1)
+    print(f"{tree}: kernel {ms * 1e3:.1f} us, host {host_ms * 1e3:.1f} us")
+    assert 0 < ms < host_ms
+    sa.set_kernel_variant(m, "thread_per_env")
+    with pytest.raises(Exception, match="lanes-per-joint"):
+        timed(B, q, qd, qdd, *out, e0.cuda_event, e1.cuda_event, torch.cuda.current_stream().cuda_stream)
+
+
 @pytest.mark.parametrize("scale", [16.0, 200.0])
 def test_large_joint_angles(scale):
     """Joint angles far outside the sampled range (both sides of the 16 rad
