@@ -55,7 +55,6 @@ def parse():
     ap.add_argument("--shape", type=int, nargs=2, default=None, metavar=("G", "C"),
                     help="lanes-per-joint shape instead of autotuning (profiling runs, where tracing skews the tuner)")
     ap.add_argument("--probe-traffic", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--probe-shape", type=int, nargs=2, default=None, help=argparse.SUPPRESS)
     return ap.parse_args()
 
 
@@ -67,11 +66,11 @@ VALU_COUNTERS = ("SQ_INSTS_VALU_FLOPS_FP32", "SQ_INSTS_VALU_FLOPS_FP32_TRANS", "
 FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32, vendor figure
 
 
-def probe_traffic(shape=None):
-    """Child of a rocprofv3 --pmc pass: 16 launches of the fused kernel (the
-    lanes-per-joint shape the timed run used) over 16 buffer sets, then 5
-    elementwise passes over 256 MiB (read 256 MiB, write 256 MiB) for the
-    counter calibration."""
+def probe_traffic(kernel="auto"):
+    """Child of a rocprofv3 --pmc pass: 16 launches of the fused kernel over 16
+    buffer sets for every lanes-per-joint shape the autotuner may pick (the
+    parent keeps the figures of the one it picks), then 5 elementwise passes
+    over 256 MiB (read 256 MiB, write 256 MiB) for the counter calibration."""
     import numpy as np
     import torch
 
@@ -84,8 +83,9 @@ def probe_traffic(shape=None):
     m = sa.PinocchioModel(rob.urdf, [0.0, 0.0, -9.81])
     m.set_joint_order(rob.joint_names)
     m.set_link_order(rob.link_names)
-    if shape is not None and sa.kernel_variant(m) == "lanes_per_joint":
-        tune.set_shape(m, B, shape[0], shape[1])
+    if kernel != "auto":
+        sa.set_kernel_variant(m, kernel)
+    shapes = tune.chain_candidates(rob.nv) if sa.kernel_variant(m) == "lanes_per_joint" else [None]
     # 16 rotating buffer sets (132 MB, beyond the 32 MB of L2) so that each
     # launch's inputs come from outside L2, as in the timed run
     sets = []
@@ -95,8 +95,11 @@ def probe_traffic(shape=None):
                      torch.empty(B, rob.nv, device=dev)))
     launch = sa.native_fk_jac_rnea(m)
     stream = torch.cuda.current_stream(dev)
-    for k in range(16):
-        launch(B, *sets[k], stream.cuda_stream)
+    for shape in shapes:
+        if shape is not None:
+            tune.set_shape(m, B, shape[0], shape[1])
+        for k in range(16):
+            launch(B, *sets[k], stream.cuda_stream)
     src = torch.ones(COPY_BYTES // 4, device=dev)
     dst = torch.empty_like(src)
     for _ in range(5):
@@ -104,13 +107,15 @@ def probe_traffic(shape=None):
     torch.cuda.synchronize()
 
 
-def measure_traffic(shape=None, timeout=120):
-    """HBM bytes per launch of the fused kernel from two rocprofv3 --pmc passes
+def measure_traffic(kernel="auto", timeout=180):
+    """HBM bytes per launch of the fused kernel, per kernel instantiation (every
+    shape the autotuner may pick), from two rocprofv3 --pmc passes
     (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by the factor
     that makes the same counter read 256 MiB for an elementwise kernel that
-    reads and writes 256 MiB.
-    Each pass is a child process (rocprofv3 -> python3 bench.py
-    --probe-traffic); returns None if rocprofv3 is missing or fails."""
+    reads and writes 256 MiB, and a pass of SQ VALU counters. Each pass is a
+    child process (rocprofv3 -> python3 bench.py --probe-traffic), started
+    before this process touches the GPU. Returns {kernel name: figures}, or
+    None if rocprofv3 is missing or fails."""
     import csv
     import glob
     import shutil
@@ -125,9 +130,7 @@ def measure_traffic(shape=None, timeout=120):
         counter = counters[0]
         d = tempfile.mkdtemp(prefix="sapamd_pmc_")
         cmd = ["rocprofv3", "--pmc", *counters, "-d", d, "-o", "pmc", "--output-format", "csv", "--",
-               sys.executable, os.path.abspath(__file__), "--probe-traffic"]
-        if shape is not None:
-            cmd += ["--probe-shape", str(shape[0]), str(shape[1])]
+               sys.executable, os.path.abspath(__file__), "--probe-traffic", "--kernel", kernel]
         p = subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, start_new_session=True)
         try:
             p.wait(timeout=timeout)
@@ -135,7 +138,7 @@ def measure_traffic(shape=None, timeout=120):
             os.killpg(p.pid, signal.SIGKILL)
             p.wait()
             return None
-        kern, copy, valu = [], [], {c: [] for c in VALU_COUNTERS}
+        kern, copy = {}, []
         for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
             with open(f) as fh:
                 for row in csv.DictReader(fh):
@@ -143,26 +146,37 @@ def measure_traffic(shape=None, timeout=120):
                     if cname not in counters:
                         continue
                     name, v = row.get("Kernel_Name", ""), float(row.get("Counter_Value", "nan"))
-                    fused = "k_chain_fused" in name or "k_kinematics" in name or "k_tree_fused" in name
-                    if counters is VALU_COUNTERS:
-                        if fused:
-                            valu[cname].append(v)
-                    elif fused:
-                        kern.append(v)
+                    if "k_chain_fused" in name or "k_kinematics" in name or "k_tree_fused" in name:
+                        kern.setdefault(name, {}).setdefault(cname, []).append(v)
                     elif "neg" in name:
                         copy.append(v)
         shutil.rmtree(d, ignore_errors=True)
+        med = lambda v: sorted(v)[len(v) // 2]
         if counters is VALU_COUNTERS:
-            if all(valu.values()):
-                res["VALU"] = {c: sorted(v)[len(v) // 2] for c, v in valu.items()}
+            for name, c in kern.items():
+                if all(c.get(x) for x in VALU_COUNTERS):
+                    res.setdefault(name, {})["VALU"] = {x: med(c[x]) for x in VALU_COUNTERS}
             continue
         if not kern or not copy:
             return None
-        kern = sorted(kern)[len(kern) // 2]
-        copy = sorted(copy)[len(copy) // 2]
-        scale = (COPY_BYTES / 1024.0) / copy
-        res[counter] = {"kib_raw": kern, "copy_kib_raw": copy, "scale": scale, "bytes": kern * 1024.0 * scale}
+        scale = (COPY_BYTES / 1024.0) / med(copy)
+        for name, c in kern.items():
+            kib = med(c[counter])
+            res.setdefault(name, {})[counter] = {"kib_raw": kib, "copy_kib_raw": med(copy), "scale": scale,
+                                                 "bytes": kib * 1024.0 * scale}
     return res
+
+
+def traffic_of(traffic, shape, variant):
+    """The figures of the kernel the timed run used, from measure_traffic."""
+    if not traffic:
+        return None
+    key = f"k_chain_fused<{shape[0]}, {shape[1]}>" if shape else ("k_kinematics" if variant != "lanes_per_joint"
+                                                                  else "k_tree_fused")
+    for name, v in traffic.items():
+        if key in name and "FETCH_SIZE" in v and "WRITE_SIZE" in v:
+            return v
+    return None
 
 
 def mixed_workloads(sa, S, np, torch, dev):
@@ -343,12 +357,14 @@ def mixed_workloads(sa, S, np, torch, dev):
 def main():
     args = parse()
     if args.probe_traffic:
-        probe_traffic(args.probe_shape)
+        probe_traffic(args.kernel)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X "
               "(the environments shard trivially across GPUs; no multi-GPU run is made).", file=sys.stderr)
         sys.exit(2)
+    # the counter passes (child processes) before this process touches the GPU
+    all_traffic = None if (args.no_traffic or args.workloads_only) else measure_traffic(args.kernel)
     import numpy as np
     import torch
 
@@ -495,7 +511,7 @@ def main():
 
     eager_wall, eager_ms = timed_seq(lambda: [step(i) for i in range(args.steps)])
     del flush
-    traffic = None if args.no_traffic else measure_traffic(tuned[B][:2] if tuned else None)
+    traffic = traffic_of(all_traffic, tuned[B][:2] if tuned else None, sa.kernel_variant(m))
 
     # achievable HBM bandwidth on this GPU: an elementwise pass over 256 MiB
     # (read + write), the calibration kernel of the traffic counters
