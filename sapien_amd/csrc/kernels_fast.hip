@@ -38,9 +38,35 @@ constexpr int kMaxChainJoints = 16;
 constexpr int kTableWords4 = (int)((kHeaderBytes + sizeof(JointRec<float>) * kMaxChainJoints) / 16);
 
 // Row strides of the batched arrays, resolved on the host (elements).
+// staged: the outputs are contiguous (poses (B, L, 7), J (B, 6, nv), tau
+// (B, nv)) and the shape streams them (G >= 8): each wavefront's outputs,
+// whose environments are consecutive, go through per-wavefront LDS of
+// per_wave floats and leave as contiguous runs, every store instruction
+// writing 256 consecutive bytes (whole 64-byte sectors), instead of 28-byte
+// pieces of many sectors.
 struct Layout {
   int64_t q, qd, qdd, tau, pose_env, pose_link;
+  int32_t staged, per_wave;
 };
+
+// The wavefront's lanes copy n floats from its LDS run to global memory: with
+// a 16-byte aligned destination (vec4; the LDS runs always are) 64 float4 per
+// store instruction, else 64 consecutive floats per instruction.
+template <bool NT> __device__ __forceinline__ void copy_run(const float *src, float *dst, int n, int lane, bool vec4) {
+  int i0 = 0;
+  if (vec4) {
+    const int n4 = n >> 2;
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const v4f *s4 = reinterpret_cast<const v4f *>(src);
+    v4f *d4 = reinterpret_cast<v4f *>(dst);
+    for (int i = lane; i < n4; i += 64) {
+      if constexpr (NT) __builtin_nontemporal_store(s4[i], d4 + i);
+      else d4[i] = s4[i];
+    }
+    i0 = n4 << 2;
+  }
+  for (int i = i0 + lane; i < n; i += 64) st<NT>(src[i], dst + i);
+}
 
 template <int G, int C>
 __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__restrict__ Mg, int64_t B, int flags,
@@ -55,6 +81,7 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
   constexpr int lstep = IL ? 2 : 1;  // lane distance between consecutive joint slots
   // model header + joint records staged in LDS once per block
   __shared__ float4 s_tab[kTableWords4];
+  extern __shared__ float s_out[];  // lay.per_wave floats per wavefront when lay.staged
   const int lane = threadIdx.x & 63;
   const int g = IL ? (lane & 15) >> 1 : (int)(threadIdx.x & (G - 1));
   const int lane0 = IL ? ((lane & ~15) | (lane & 1)) : (lane & ~(G - 1));  // the group's joint slot 0
@@ -148,10 +175,29 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
   float *const jrow = Jout + e * (int64_t)(6 * nv);
   float *const trow = tau + e * lay.tau;
   const int pl = (int)lay.pose_link;
+  // staged outputs: this wavefront's environments e0 .. e0 + nlive - 1, this
+  // lane's environment at slot le of the wavefront's LDS runs
+  constexpr int EPW = 64 / G;
+  const bool staged = NT && lay.staged;
+  const int le = IL ? (((lane >> 4) << 1) | (lane & 1)) : lane / G;
+  const int64_t e0 = (blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63)) / G;
+  const int nlive = (int)(B - e0 < EPW ? (B - e0 > 0 ? B - e0 : 0) : EPW);
+  float *const wJ = s_out + (threadIdx.x >> 6) * lay.per_wave;  // [EPW][6][nv]
+  float *const wP = wJ + EPW * 6 * nv;                           // [EPW][nj + 1][7]
+  float *const wT = wP + EPW * (nj + 1) * 7;                     // [EPW][nv]
+  const bool stage_poses = staged && M->links_le1 && L <= nj + 1;
 
   // ---- Jacobian columns and link poses, emitted before the dynamics so that
   // their stores drain while RNEA computes
-  if (flags & KF_JFULL) {
+  if ((flags & KF_JFULL) && staged) {
+#pragma unroll
+    for (int t = 0; t < C; ++t) {
+      if (!valid[t]) continue;
+      float *Jb = wJ + le * 6 * nv + J[t]->vuser;
+      Jb[0 * nv] = col[t].l.x; Jb[1 * nv] = col[t].l.y; Jb[2 * nv] = col[t].l.z;
+      Jb[3 * nv] = col[t].a.x; Jb[4 * nv] = col[t].a.y; Jb[5 * nv] = col[t].a.z;
+    }
+  } else if (flags & KF_JFULL) {
 #pragma unroll
     for (int t = 0; t < C; ++t) {
       if (!valid[t]) continue;
@@ -175,10 +221,13 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
         pw = mv(X[t].R, load3(J[t]->lp)) + X[t].p;
         R = Rw;
       }
-      if (live) store_pose<NT>(prow + J[t]->link * pl, R, pw);
+      if (stage_poses) store_pose<false>(wP + (le * L + J[t]->link) * 7, R, pw);
+      else if (live) store_pose<NT>(prow + J[t]->link * pl, R, pw);
     }
-    if (g == 0 && M->root_link >= 0 && live)
-      store_pose_q<NT>(prow + M->root_link * pl, M->root_p, M->root_q);
+    if (g == 0 && M->root_link >= 0) {
+      if (stage_poses) store_pose_q<false>(wP + (le * L + M->root_link) * 7, M->root_p, M->root_q);
+      else if (live) store_pose_q<NT>(prow + M->root_link * pl, M->root_p, M->root_q);
+    }
   } else if (flags & KF_POSES) {
     // general link lists: link slot l = l0 + g fetches its joint's pose from the owner lane
     for (int l0 = 0; l0 < L; l0 += G) {
@@ -205,6 +254,13 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
       }
       if (live && l < L) store_pose<NT>(prow + l * pl, Rw, pw);
     }
+  }
+
+  if (staged) {  // the wavefront's Jacobian and pose runs, whole sectors per store
+    wave_sync();
+    if (flags & KF_JFULL) copy_run<NT>(wJ, Jout + e0 * 6 * nv, nlive * 6 * nv, lane, lay.staged & 2);
+    if ((flags & KF_POSES) && stage_poses)
+      copy_run<NT>(wP, poses + e0 * L * 7, nlive * L * 7, lane, lay.staged & 4);
   }
 
   // ---- RNEA
@@ -257,6 +313,14 @@ __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__
     }
     f[0] = incl;
   }
+  if (staged) {
+#pragma unroll
+    for (int t = 0; t < C; ++t)
+      if (valid[t]) wT[le * nv + J[t]->vuser] = dot6(col[t], f[t]);
+    wave_sync();
+    copy_run<NT>(wT, tau + e0 * nv, nlive * nv, lane, lay.staged & 8);
+    return;
+  }
 #pragma unroll
   for (int t = 0; t < C; ++t) {
     if (!valid[t]) continue;
@@ -269,12 +333,13 @@ void launch_gc(const ModelTable<float> *tab, int64_t B, int flags, const float *
                KinOut<float> out, hipStream_t s, const Layout &lay, hipEvent_t ev0, hipEvent_t ev1) {
   const int64_t envs_per_block = 256 / G;
   dim3 grid((unsigned)((B + envs_per_block - 1) / envs_per_block));
+  const unsigned lds = lay.staged ? 4u * (unsigned)lay.per_wave * (unsigned)sizeof(float) : 0u;  // 4 wavefronts
   if (ev0 || ev1)  // the kernel's own start and stop recorded into the caller's events
-    hipExtLaunchKernelGGL((k_chain_fused<G, C>), grid, dim3(256), 0, s, ev0, ev1, 0, tab, B, flags, q, qd, qdd,
+    hipExtLaunchKernelGGL((k_chain_fused<G, C>), grid, dim3(256), lds, s, ev0, ev1, 0, tab, B, flags, q, qd, qdd,
                           out.poses, out.jfull, out.tau, lay);
   else
-    hipLaunchKernelGGL((k_chain_fused<G, C>), grid, dim3(256), 0, s, tab, B, flags, q, qd, qdd, out.poses, out.jfull,
-                       out.tau, lay);
+    hipLaunchKernelGGL((k_chain_fused<G, C>), grid, dim3(256), lds, s, tab, B, flags, q, qd, qdd, out.poses,
+                       out.jfull, out.tau, lay);
 }
 
 }  // namespace
@@ -313,6 +378,15 @@ hipError_t launch_chain_fused(const ModelTable<float> *tab, int nj, int64_t B, i
   lay.tau = out.ld_tau ? out.ld_tau : nv;
   lay.pose_link = out.pose_link ? out.pose_link : 7;
   lay.pose_env = out.pose_env;  // 0: L * 7, resolved in the kernel from the table
+  // staged outputs for the streaming shapes over contiguous outputs (the pose
+  // run assumes at most one link per joint plus the root, checked in the kernel)
+  lay.staged = G >= 8 && out.pose_env == 0 && lay.pose_link == 7 && lay.tau == nv ? 1 : 0;
+  lay.per_wave = (64 / G) * (6 * nv + 7 * (nj + 1) + nv);  // a multiple of 4: the runs stay 16-byte aligned
+  // bits 2, 4, 8: the J, pose and tau runs start 16-byte aligned (each
+  // wavefront's run starts a multiple of 16 bytes after the array's start)
+  auto al16 = [](const void *p) { return ((uintptr_t)p & 15) == 0; };
+  if (lay.staged)
+    lay.staged |= (al16(out.jfull) ? 2 : 0) | (al16(out.poses) ? 4 : 0) | (al16(out.tau) ? 8 : 0);
 #define SAPAMD_SHAPE(g_, c_) \
   if (G == g_ && C == c_) { launch_gc<g_, c_>(tab, B, flags, q, qd, qdd, out, s, lay, ev0, ev1); return hipGetLastError(); }
   SAPAMD_SHAPE(16, 1) SAPAMD_SHAPE(8, 1) SAPAMD_SHAPE(8, 2) SAPAMD_SHAPE(4, 2) SAPAMD_SHAPE(4, 4)

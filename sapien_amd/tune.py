@@ -69,6 +69,13 @@ def _save_cache(d):
         pass
 
 
+def chain_candidates(nj):
+    """The (lanes per environment, joints per lane) shapes the autotuner tries
+    for a chain of nj joints: they cover the chain and waste less than half
+    the lanes."""
+    return [(G, C) for G, C in shapes() if not (G * C < nj or (G * C) // 2 >= nj and G * C > 8)]
+
+
 def autotune(model, batch_sizes=(16384,), iters=100, use_cache=True, report=None):
     """Pick and install the fastest chain-kernel shape for each batch size.
 
@@ -115,8 +122,7 @@ def autotune(model, batch_sizes=(16384,), iters=100, use_cache=True, report=None
             sets.append((q.float().to(dev), torch.randn(B, nj, generator=gen).to(dev),
                          torch.randn(B, nj, generator=gen).to(dev), torch.empty(B, L, 7, device=dev),
                          torch.empty(B, 6, nj, device=dev), torch.empty(B, nj, device=dev)))
-        cands = [(G, C) for G, C in shapes()
-                 if not (G * C < nj or (G * C) // 2 >= nj and G * C > 8)]  # covers the chain, wastes < half the lanes
+        cands = chain_candidates(nj)
         best = {}
         for _ in range(3):  # rounds over all candidates; the best round of each counts
             for G, C in cands:
