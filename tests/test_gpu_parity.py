@@ -654,6 +654,49 @@ def test_padded_simulator_buffers(tree):
         assert pos <= POSE_TOL_M and ang <= POSE_TOL_RAD
 
 
+@pytest.mark.parametrize("B", [16389, 1, 9])
+def test_chain_outputs_staged_through_lds(B):
+    """The streaming chain shapes write each wavefront's outputs as contiguous
+    runs through LDS: a partial last wavefront (B not a multiple of 8 or 4),
+    and output arrays 16-byte aligned (float4 runs) or 4 bytes off (float
+    runs), give the same bits as the direct row stores of the strided entry
+    point, and nothing outside the outputs is written."""
+    import sapien_amd as sa
+    from sapien_amd import tune
+
+    rob = S.chain7()
+    m, o = _pair(rob)
+    n, L = rob.nv, rob.nlinks
+    q, qd, qdd, _ = S.samples(rob, B, 1960)
+    qt, qdt, qddt = _t(q), _t(qd), _t(qdd)
+    launch = sa.native_fk_jac_rnea(m)
+    cs = torch.cuda.current_stream().cuda_stream
+    for G, C in ((8, 1), (8, 2), (16, 1)):
+        tune.set_shape(m, 1 << 40, G, C)
+        # reference: the strided entry point with padded rows takes the direct stores
+        state = torch.zeros(B, n + 1, device=DEV)
+        body = torch.full((B, L + 1, 13), -5.0, device=DEV)
+        Jr = torch.empty(B, 6, n, device=DEV)
+        state[:, :n] = qt
+        m.compute_fk_jacobian_rnea_into(state[:, :n], qdt, qddt, poses=body, jacobian=Jr, qf=None)
+        for off in (0, 1):  # element offset of every output array
+            Pbuf = torch.full((B * L * 7 + 8,), -7.0, device=DEV)
+            Jbuf = torch.full((B * 6 * n + 8,), -7.0, device=DEV)
+            Tbuf = torch.full((B * n + 8,), -7.0, device=DEV)
+            P = Pbuf[off:off + B * L * 7].view(B, L, 7)
+            J = Jbuf[off:off + B * 6 * n].view(B, 6, n)
+            T = Tbuf[off:off + B * n].view(B, n)
+            launch(B, qt, qdt, qddt, P, J, T, cs)
+            torch.cuda.synchronize()
+            assert torch.equal(P, body[:, :L, :7]) and torch.equal(J, Jr), (G, C, off)
+            for buf, used in ((Pbuf, B * L * 7), (Jbuf, B * 6 * n), (Tbuf, B * n)):
+                assert bool((buf[:off] == -7.0).all()) and bool((buf[off + used:] == -7.0).all()), (G, C, off)
+            _, _, Tp = m.compute_fk_jacobian_rnea_batched(qt, qdt, qddt)
+            assert torch.equal(T, Tp), (G, C, off)
+            assert rel_err(T.cpu().numpy(), o.rnea(q, qd, qdd)) <= REL_TOL
+    tune.set_shape(m, 0, 0, 0)
+
+
 @pytest.mark.parametrize("tree", TREES)
 def test_simulator_link_state(tree):
     """sapamd_link_state_strided on a padded [n_art, max_links, 13] body-state
