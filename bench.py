@@ -629,6 +629,34 @@ def main():
     # = the fused loss-and-gradient kernel (dq, dqd, dqdd).
     from sapien_amd.native import launcher_pose_torque_loss
 
+    def graph_launch_s(fns, per_graph=50, replays=10):
+        """Seconds per step of `fns` (each fn(stream) launches once) from HIP
+        events around replays of a graph of per_graph steps: GPU time only,
+        not the host's launch rate."""
+        gcap = torch.cuda.Stream(dev)
+        gcap.wait_stream(stream)
+        with torch.cuda.stream(gcap):
+            for fn in fns:  # warm-up outside capture
+                fn(gcap.cuda_stream)
+        stream.wait_stream(gcap)
+        torch.cuda.synchronize()
+        gg = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(gg, stream=gcap):
+            cst = torch.cuda.current_stream(dev).cuda_stream
+            for _ in range(per_graph):
+                for fn in fns:
+                    fn(cst)
+        stream.wait_stream(gcap)
+        gg.replay()
+        torch.cuda.synchronize()
+        g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        g0.record(stream)
+        for _ in range(replays):
+            gg.replay()
+        g1.record(stream)
+        torch.cuda.synchronize()
+        return g0.elapsed_time(g1) * 1e-3 / (per_graph * replays)
+
     grads = {}
     mixed = {}  # per tree: (forward, forward + backward) over the first 4096 environments
     for gname in ("chain7", "arm9"):
@@ -648,25 +676,16 @@ def main():
         if sa.kernel_variant(gm) == "lanes_per_joint":
             tune.autotune(gm, batch_sizes=(GB // 2, GB), use_cache=False)
 
-        def run_f(nb=GB, fwd=fwd, gq=gq, gqd=gqd, gqdd=gqdd, bufs=bufs):
-            fwd(nb, gq, gqd, gqdd, bufs["P"], None, bufs["T"], stream.cuda_stream)
+        def run_f(st, nb=GB, fwd=fwd, gq=gq, gqd=gqd, gqdd=gqdd, bufs=bufs):
+            fwd(nb, gq, gqd, gqdd, bufs["P"], None, bufs["T"], st)
 
-        def run_fb(nb=GB, fb=fb, gq=gq, gqd=gqd, gqdd=gqdd, gts=gts, ee=grob.ee_link, gt=gt, bufs=bufs):
-            fb(nb, gq, gqd, gqdd, gts, ee, gt, bufs["loss"], None, bufs["dq"], bufs["dqd"], bufs["dqdd"],
-               stream.cuda_stream)
+        def run_fb(st, nb=GB, fb=fb, gq=gq, gqd=gqd, gqdd=gqdd, gts=gts, ee=grob.ee_link, gt=gt, bufs=bufs):
+            fb(nb, gq, gqd, gqdd, gts, ee, gt, bufs["loss"], None, bufs["dq"], bufs["dqd"], bufs["dqdd"], st)
 
-        mixed[gname] = (lambda f=run_f: f(GB // 2), lambda f=run_fb: f(GB // 2))
-
+        mixed[gname] = (lambda st, f=run_f: f(st, GB // 2), lambda st, f=run_fb: f(st, GB // 2))
         res = {}
         for key, fn in (("forward_evals_s", run_f), ("fwd_bwd_evals_s", run_fb)):
-            for _ in range(20):
-                fn()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(500):
-                fn()
-            torch.cuda.synchronize()
-            res[key] = GB * 500 / (time.perf_counter() - t0)
+            res[key] = GB / graph_launch_s([fn])
         res["ratio"] = res["fwd_bwd_evals_s"] / res["forward_evals_s"]
         res["kernel"] = sa.kernel_variant(gm)
         grads[gname] = res
@@ -674,18 +693,10 @@ def main():
     # loss + gradient) step launches both trees' kernels back to back
     res = {"envs": {"chain7": 4096, "arm9": 4096}}
     for key, idx in (("forward_evals_s", 0), ("fwd_bwd_evals_s", 1)):
-        fns = [mixed[k][idx] for k in ("chain7", "arm9")]
-        for _ in range(20):
-            for fn in fns:
-                fn()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(500):
-            for fn in fns:
-                fn()
-        torch.cuda.synchronize()
-        res[key] = 8192 * 500 / (time.perf_counter() - t0)
+        res[key] = 8192 / graph_launch_s([mixed[k][idx] for k in ("chain7", "arm9")])
     res["ratio"] = res["fwd_bwd_evals_s"] / res["forward_evals_s"]
+    res["timing"] = ("HIP events around replays of a graph of 50 steps (each step the listed launches), 10 "
+                     "replays after a warm-up one; per step = elapsed / 500")
     grads["mixed_chain7_arm9"] = res
     out["gradients"] = grads
 
