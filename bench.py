@@ -724,9 +724,9 @@ def main():
         o = OracleModel(rob.urdf)
         o.set_joint_order(rob.joint_names)
         o.set_link_order(rob.link_names)
-        # host cores: the GPU box allots 16 host cores to each GPU, so the
-        # multi-core baseline uses at most 16 threads (and no more than this
-        # process may run on)
+        # host cores: the MI355X hosts this runs on allot 16 host cores to each
+        # GPU, so the multi-core baseline uses at most 16 threads (and no more
+        # than this process may run on)
         try:
             allowed = len(os.sched_getaffinity(0))
         except (AttributeError, OSError):
@@ -755,7 +755,7 @@ def main():
                                "repeats": reps, "min_max_evals_s": [chunk / max(times), chunk / min(times)],
                                "host": {"cpu_model": cpu_model, "os_cpu_count": os.cpu_count(),
                                         "affinity_cores": allowed,
-                                        "threads_used": threads, "cap": "16 host cores per GPU on the GPU box"},
+                                        "threads_used": threads, "cap": "16 host cores per GPU, the host's allotment"},
                                "sample": f"{reps} x {chunk} seeded chain7 samples after a warm-up pass, FP64 FK + "
                                          f"8 link poses + WORLD Jacobian + RNEA per sample, C++ oracle statically "
                                          f"split over {threads} threads ({secs:.1f} s); value = the median repeat"}
