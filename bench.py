@@ -478,7 +478,7 @@ def main():
     kern_ms = graph_ms
 
     # each kernel's own start and stop (hipExtLaunchKernel events) over the
-    # timed region's launches queued back to back behind a sleep kernel; these
+    # timed region's launches queued behind a sleep kernel; these
     # end when the kernel's writes are issued, before the end-of-kernel cache
     # release a trace counts, so they read lower (reported, not the roofline's)
     timed_launch = sa.native_fk_jac_rnea_timed(m)
@@ -538,7 +538,7 @@ def main():
             "what": f"(median replay of the {args.steps}-launch graph - median replay of a one-launch graph) / "
                     f"{args.steps - 1}, each replay after the cache flush"},
         "kernel_events": {"ms_per_kernel": ext_ms, "min_ms": min(kernel_durations), "max_ms": max(kernel_durations),
-                          "what": f"the timed region's {args.steps} launches queued back to back, each kernel's "
+                          "what": f"the timed region's {args.steps} launches queued behind a sleep kernel, each kernel's "
                                   f"own start and stop (hipExtLaunchKernel events; they end before the "
                                   f"end-of-kernel cache release)"},
         "eager": {"ms_per_step": eager_wall * 1e3, "events_ms_per_launch": eager_ms,

@@ -80,12 +80,12 @@ def autotune(model, batch_sizes=(16384,), iters=100, use_cache=True, report=None
     """Pick and install the fastest chain-kernel shape for each batch size.
 
     Each candidate is timed by its kernels' own durations (kernel_us: launches
-    queued back to back, each recording its kernel's start and stop into HIP
-    events; host and dispatch overheads, a profiler's included, do not count)
-    over launches that rotate through enough input/output sets to miss the
-    Infinity Cache, three rounds over all candidates, best round each. Returns {B: (G, C, microseconds)}. Models that
-    are not serial chains of <= 16 joints (where the thread-per-environment
-    kernels run) return {}.
+    queued without host gaps, each recording its kernel's start and stop into
+    HIP events; host and dispatch overheads, a profiler's included, do not
+    count) over launches that rotate through enough input/output sets to miss
+    the Infinity Cache, three rounds over all candidates, best round each.
+    Returns {B: (G, C, microseconds)}. Models that are not serial chains of
+    <= 16 joints (where the thread-per-environment kernels run) return {}.
     """
     import torch
 
@@ -176,7 +176,8 @@ _SLEEP_MS_PER_MCYCLE = None
 
 def hold_gpu(ms):
     """Keep the current stream's GPU busy for about `ms` milliseconds (a sleep
-    kernel), so that the launches the host queues meanwhile run back to back."""
+    kernel), so that the launches the host queues meanwhile wait on the GPU,
+    not on the host."""
     import torch
 
     global _SLEEP_MS_PER_MCYCLE
@@ -195,7 +196,7 @@ _EVENTS = []
 
 def kernel_us(run_timed, iters):
     """Mean kernel duration in microseconds of `iters` launches queued behind a
-    sleep kernel (so they run back to back, as in a captured graph);
+    sleep kernel (so the host's launch rate does not space them out);
     run_timed(i, start, stop) makes launch i recording its kernel's own start
     and stop into the two raw event handles. Host launch overhead and the
     dispatch gaps between launches are not counted (nor a profiler's)."""
