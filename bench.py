@@ -3,17 +3,17 @@ B = 16384 environments of the seeded 7-dof chain, FP32, one MI355X.
 
 One step = one launch of the fused kernel over one batch of 16384 environments,
 inputs already resident in HBM. Step i uses input/output buffer set i % 64
-(64 sets, ~530 MB, more than the 256 MB Infinity Cache). The `steps` launches
-are captured once in a HIP graph; after `warmup` eager launches and a 512 MiB
-read that evicts the Infinity Cache, the timed region is one replay of that
-graph, bracketed by HIP events on the launch stream and by the host clock.
-`value` = 16384 x steps / host time. Prints ONE JSON line.
+(64 sets, ~530 MB, more than the 256 MB Infinity Cache). After `warmup`
+launches and a 512 MiB read that evicts the Infinity Cache, the timed region
+is `steps` calls of the C-ABI entry point sapamd_fk_jac_rnea_batched (ctypes,
+device pointers resolved beforehand), bracketed by HIP events on the launch
+stream and by the host clock. `value` = 16384 x steps / host time. Prints ONE
+JSON line.
 
 Beside the headline number it reports:
   roofline      algorithmic bytes per launch / mean launch duration (HIP events
-                on the launch stream around the timed graph replay, per launch;
-                they include the graph's launch latency, so they read high at
-                few steps), against 8 TB/s HBM3E; traffic from rocprofv3
+                on the launch stream around the timed region, per launch),
+                against 8 TB/s HBM3E; traffic from rocprofv3
                 FETCH_SIZE + WRITE_SIZE;
                 the FP32 VALU work (rocprofv3 SQ_INSTS_VALU_* counters) against
                 the 157.3 TFLOP/s vector peak; the bandwidth an elementwise
@@ -21,6 +21,8 @@ Beside the headline number it reports:
   cpu_baseline  the FP64 C++ oracle (same algorithms) timed on this host's cores
                 on a bounded sample, on one core, and through its per-call
                 Python API (one configuration per call);
+  launch_modes  the same launches captured in a HIP graph, each kernel's own
+                start and stop, and the tensor-level Python launcher;
   plain_gpu     the drop-in per-call path (one configuration per call, FP64 on
                 the GPU, host copies) on a bounded sample.
 """
@@ -399,23 +401,60 @@ def main():
                          P=torch.zeros(B, L, 7, device=dev), J=torch.zeros(B, 6, n, device=dev),
                          T=torch.zeros(B, n, device=dev)))  # outputs written once at set-up
     stream = torch.cuda.current_stream(dev)
-    launch = sa.native_fk_jac_rnea(m)  # C-ABI launcher over raw device pointers
+    launch = sa.native_fk_jac_rnea(m)  # C-ABI launcher over tensors (set-up and diagnostics)
 
     def step(i, st=None):
         s = sets[i % NSETS]
         launch(B, s["q"], s["qd"], s["qdd"], s["P"], s["J"], s["T"], st or stream.cuda_stream)
+
+    # The timed region's launches: sapamd_fk_jac_rnea_batched called through
+    # ctypes with the device pointers resolved beforehand, as a C or cgo caller
+    # holds them; one call = one launch over one buffer set.
+    from sapien_amd import native
+
+    abi = native.lib().sapamd_fk_jac_rnea_batched
+    handle = native.handle(m)
+    calls = [(handle, B) + tuple(sets[i % NSETS][k].data_ptr() for k in ("q", "qd", "qdd", "P", "J", "T")) +
+             (stream.cuda_stream,) for i in range(args.steps)]
+
+    def run_calls(cs):
+        rcs = [abi(*c) for c in cs]
+        for rc in rcs:
+            native.check(rc)
 
     # 512 MiB read between the warm-up and the timed region: the 256 MB
     # Infinity Cache then holds none of the timed launches' buffers (and no
     # dirty lines of its own to write back)
     flush = torch.ones(128 << 20, dtype=torch.float32, device=dev)
 
-    # The timed region: exactly `steps` launches, step i over buffer set
-    # i % 64, captured once in a HIP graph and replayed once. Set 0's outputs
-    # from an eager launch are the check that the replay computes the same bits.
-    step(0)
+    # Set-up pass: one launch over every buffer set. Its outputs are the
+    # reference that the timed launches are checked against bit for bit (the
+    # timed sets' outputs are NaN-filled before the timed region).
+    for i in range(NSETS):
+        step(i)
     torch.cuda.synchronize()
-    ref0 = [t.clone() for t in (sets[0]["P"], sets[0]["J"], sets[0]["T"])]
+    outs = [(sets[i]["P"], sets[i]["J"], sets[i]["T"]) for i in range(min(args.steps, NSETS))]
+    refs = [[t.clone() for t in o] for o in outs]
+
+    def nan_fill():
+        for o in outs:
+            for t in o:
+                t.fill_(float("nan"))
+
+    def same_as_setup():
+        return all(torch.equal(a, b) for o, r in zip(outs, refs) for a, b in zip(r, o))
+
+    # the timing events exist before the timed region (torch creates an
+    # event at its first record)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    e1.record(stream)
+
+    # Not the metric: the same `steps` launches captured once in a HIP graph
+    # and replayed after the same cache flush. Its first replay carries the
+    # graph's one-time launch cost, and kernels inside a replay run slower
+    # than the same launches on the stream; reported, first replay and the
+    # median of three later ones.
     graph = torch.cuda.CUDAGraph()
     cap = torch.cuda.Stream(dev)
     cap.wait_stream(stream)
@@ -425,57 +464,24 @@ def main():
             step(i, cs)
     stream.wait_stream(cap)
     torch.cuda.synchronize()
-    # warm-up: `warmup` eager launches over the sets counted down from the
-    # last one (with few steps, not the timed sets)
-    for i in range(args.warmup):
-        step(NSETS - 1 - (i % NSETS))
-    for t in (sets[0]["P"], sets[0]["J"], sets[0]["T"]):
-        t.fill_(float("nan"))
-    flush.sum()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record(stream)
-    graph.replay()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    graph_ms = e0.elapsed_time(e1) / args.steps
-    graph_ok = all(torch.equal(a, b) for a, b in zip(ref0, (sets[0]["P"], sets[0]["J"], sets[0]["T"])))
-    if not graph_ok:
-        raise RuntimeError("the graph replay differs from the eager launch on buffer set 0")
-    value = B * args.steps / elapsed
 
-    # Not the roofline's figure: the per-launch time of the timed region without
-    # the graph's launch latency:
-    # the events around one replay also count the ~30 us between the replay
-    # call and its first kernel, which 20 launches do not amortize. Replays of
-    # the timed graph and of a graph of one launch (set 0), each after the
-    # same cache flush: kernel_ms = (t_steps - t_1) / (steps - 1), the time each
-    # launch adds when they run back to back (what a kernel trace's span per
-    # launch shows).
-    graph1 = torch.cuda.CUDAGraph()
-    cap.wait_stream(stream)
-    with torch.cuda.graph(graph1, stream=cap):
-        step(0, torch.cuda.current_stream(dev).cuda_stream)
-    stream.wait_stream(cap)
-    torch.cuda.synchronize()
-
-    def replay_ms(g):
+    def timed_seq(fn):
         flush.sum()
         torch.cuda.synchronize()
+        s0 = time.perf_counter()
         e0.record(stream)
-        g.replay()
+        fn()
         e1.record(stream)
         torch.cuda.synchronize()
-        return e0.elapsed_time(e1)
+        return (time.perf_counter() - s0) / args.steps, e0.elapsed_time(e1) / args.steps
 
-    t_n = sorted([graph_ms * args.steps] + [replay_ms(graph) for _ in range(4)])
-    t_1 = sorted(replay_ms(graph1) for _ in range(5))
-    t_n, t_1 = t_n[len(t_n) // 2], t_1[len(t_1) // 2]
-    per_launch_ms = (t_n - t_1) / (args.steps - 1) if args.steps > 1 else t_1
-    # the roofline's launch duration: HIP events over the timed region, per launch
-    kern_ms = graph_ms
+    nan_fill()
+    graph_first = timed_seq(graph.replay)
+    graph_ok = same_as_setup()
+    if not graph_ok:
+        raise RuntimeError("the graph replay differs from the set-up pass")
+    later = sorted(timed_seq(graph.replay) for _ in range(3))
+    graph_later = later[1]
 
     # each kernel's own start and stop (hipExtLaunchKernel events) over the
     # timed region's launches queued behind a sleep kernel; these
@@ -495,21 +501,35 @@ def main():
     torch.cuda.synchronize()
     kernel_durations = [a.elapsed_time(b) for a, b in evs]
     ext_ms = sum(kernel_durations) / len(kernel_durations)
+
+    # not the metric: the same launches through the tensor-level Python
+    # launcher (device pointers looked up per call)
+    eager_wall, eager_ms = timed_seq(lambda: [step(i) for i in range(args.steps)])
+
+    # The launch-mode diagnostics above ran first, so the GPU and the host
+    # are past their first-use costs when the timed region starts.
+    # warm-up: `warmup` launches over the sets counted down from the last one
+    # (with few steps, not the timed sets)
+    run_calls([(handle, B) + tuple(sets[NSETS - 1 - (i % NSETS)][k].data_ptr()
+                                   for k in ("q", "qd", "qdd", "P", "J", "T")) + (stream.cuda_stream,)
+               for i in range(args.warmup)])
+    nan_fill()
+    flush.sum()
+    torch.cuda.synchronize()
+    # The timed region: exactly `steps` launches, step i over buffer set i % 64
+    t0 = time.perf_counter()
+    e0.record(stream)
+    run_calls(calls)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    # the roofline's launch duration: HIP events over the timed region, per launch
+    kern_ms = e0.elapsed_time(e1) / args.steps
+    if not same_as_setup():
+        raise RuntimeError("the timed launches differ from the set-up pass")
+    value = B * args.steps / elapsed
     achieved = BYTES_PER_EVAL * B / (kern_ms * 1e-3) / 1e9
 
-    # not the metric: `steps` eager launches (one C-ABI call per step from
-    # Python) after the same cache flush, HIP events around the sequence
-    def timed_seq(fn):
-        flush.sum()
-        torch.cuda.synchronize()
-        s0 = time.perf_counter()
-        e0.record(stream)
-        fn()
-        e1.record(stream)
-        torch.cuda.synchronize()
-        return (time.perf_counter() - s0) / args.steps, e0.elapsed_time(e1) / args.steps
-
-    eager_wall, eager_ms = timed_seq(lambda: [step(i) for i in range(args.steps)])
     del flush
     traffic = traffic_of(all_traffic, tuned[B][:2] if tuned else None, sa.kernel_variant(m))
 
@@ -529,20 +549,23 @@ def main():
     del csrc, cdst
 
     launch_modes = {
-        "graph": {"ms_per_step": elapsed / args.steps * 1e3, "events_ms_per_launch": graph_ms,
-                  "replay_matches_eager": graph_ok,
-                  "what": f"the timed region: one replay of a HIP graph of {args.steps} launches; its events "
-                          f"(roofline.kernel_ms) also span the graph's launch latency"},
-        "per_launch_without_graph_latency": {
-            "ms": per_launch_ms, "graph_ms_steps": t_n, "graph_ms_one_launch": t_1,
-            "what": f"(median replay of the {args.steps}-launch graph - median replay of a one-launch graph) / "
-                    f"{args.steps - 1}, each replay after the cache flush"},
+        "c_abi": {"ms_per_step": elapsed / args.steps * 1e3, "events_ms_per_launch": kern_ms,
+                  "matches_setup_pass": True,
+                  "what": f"the timed region: {args.steps} calls of sapamd_fk_jac_rnea_batched through ctypes with "
+                          f"device pointers resolved beforehand, one launch each"},
+        "graph": {"first_replay": {"ms_per_step": graph_first[0] * 1e3, "events_ms_per_launch": graph_first[1]},
+                  "later_replays_median": {"ms_per_step": graph_later[0] * 1e3,
+                                           "events_ms_per_launch": graph_later[1]},
+                  "replay_matches_setup_pass": graph_ok,
+                  "what": f"the same {args.steps} launches captured once in a HIP graph, replayed after the same "
+                          f"cache flush"},
         "kernel_events": {"ms_per_kernel": ext_ms, "min_ms": min(kernel_durations), "max_ms": max(kernel_durations),
                           "what": f"the timed region's {args.steps} launches queued behind a sleep kernel, each kernel's "
                                   f"own start and stop (hipExtLaunchKernel events; they end before the "
                                   f"end-of-kernel cache release)"},
-        "eager": {"ms_per_step": eager_wall * 1e3, "events_ms_per_launch": eager_ms,
-                  "what": "one C-ABI call per step from Python"},
+        "tensor_launcher": {"ms_per_step": eager_wall * 1e3, "events_ms_per_launch": eager_ms,
+                            "what": "one call per step of the tensor-level Python launcher "
+                                    "(sapien_amd.native_fk_jac_rnea: device pointers looked up per call)"},
     }
     out = {
         "metric": METRIC,
@@ -613,7 +636,7 @@ def main():
         for i in range(2 * nset):
             launch(Bs, *bsets[i % nset], stream.cuda_stream)
         reps_t = max(50, nset)
-        # each kernel's own duration, the launches queued back to back
+        # each kernel's own duration, the launches queued behind a sleep kernel
         kus = tune.kernel_us(lambda i, e0, e1: timed_launch(Bs, *bsets[i % nset], e0, e1, stream.cuda_stream), reps_t)
         gbs = BYTES_PER_EVAL * Bs / (kus * 1e-6) / 1e9
         sweep.append({"envs": Bs, "kernel_us": kus, "evals_s": Bs / (kus * 1e-6), "achieved_gbs": gbs,
