@@ -49,25 +49,6 @@ struct Layout {
   int32_t staged, per_wave;
 };
 
-// The wavefront's lanes copy n floats from its LDS run to global memory: with
-// a 16-byte aligned destination (vec4; the LDS runs always are) 64 float4 per
-// store instruction, else 64 consecutive floats per instruction.
-template <bool NT> __device__ __forceinline__ void copy_run(const float *src, float *dst, int n, int lane, bool vec4) {
-  int i0 = 0;
-  if (vec4) {
-    const int n4 = n >> 2;
-    typedef float v4f __attribute__((ext_vector_type(4)));
-    const v4f *s4 = reinterpret_cast<const v4f *>(src);
-    v4f *d4 = reinterpret_cast<v4f *>(dst);
-    for (int i = lane; i < n4; i += 64) {
-      if constexpr (NT) __builtin_nontemporal_store(s4[i], d4 + i);
-      else d4[i] = s4[i];
-    }
-    i0 = n4 << 2;
-  }
-  for (int i = i0 + lane; i < n; i += 64) st<NT>(src[i], dst + i);
-}
-
 template <int G, int C>
 __global__ __launch_bounds__(256) void k_chain_fused(const ModelTable<float> *__restrict__ Mg, int64_t B, int flags,
                                                      const float *__restrict__ q, const float *__restrict__ qd,

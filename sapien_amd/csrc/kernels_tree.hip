@@ -44,10 +44,11 @@ struct TreeIO {
 __host__ __device__ constexpr int crba_lds_bytes(int) { return 256 * 12 * 4; }
 
 // One environment per G-lane group; M is the LDS copy of the model table;
-// crba is the block's dynamic LDS (used when flags has KF_CRBA).
+// crba is the block's dynamic LDS (used when flags has KF_CRBA): 3 float4 per
+// lane.
 template <int G>
 __device__ __forceinline__ void tree_body(const ModelTable<float> *M, const ModelTable<float> *Mg, int64_t B,
-                                          int64_t env, int flags, TreeIO io, float *crba) {
+                                          int64_t env, int flags, TreeIO io, float4 *crba) {
   const int g = threadIdx.x & (G - 1);
   const bool live = env < B;
   const int64_t e = live ? env : B - 1;
@@ -121,6 +122,16 @@ __device__ __forceinline__ void tree_body(const ModelTable<float> *M, const Mode
     }
   }
 
+  // composite inertia of the subtree (additive form about the world origin),
+  // summed with the RNEA forces when both are computed
+  const bool do_crba = flags & KF_CRBA;
+  float y[10];
+  if (do_crba) {
+    const AInertia<float> Ya = additive(world_inertia_fast(J, X.R, X.p));
+    const float yv[10] = {Ya.m, Ya.h.x, Ya.h.y, Ya.h.z, Ya.I[0], Ya.I[1], Ya.I[2], Ya.I[3], Ya.I[4], Ya.I[5]};
+#pragma unroll
+    for (int i = 0; i < 10; ++i) y[i] = valid ? yv[i] : 0.f;
+  }
   if (dyn) {
     S6<float> v = qdj * col;
     T.prefix(v);
@@ -129,17 +140,22 @@ __device__ __forceinline__ void tree_body(const ModelTable<float> *M, const Mode
     if (!(flags & KF_NOGRAV)) a.l = a.l - V3<float>{M->gravity[0], M->gravity[1], M->gravity[2]};
     const WInertia<float> Y = world_inertia_fast(J, X.R, X.p);
     S6<float> f = valid ? inertia_apply(Y, a) + crossf(v, inertia_apply(Y, v)) : zero6<float>();
-    T.subtree(f);
+    if (do_crba) {
+      // one subtree pass for the forces and the composite inertias
+      float x[16] = {f.l.x, f.l.y, f.l.z, f.a.x, f.a.y, f.a.z};
+#pragma unroll
+      for (int i = 0; i < 10; ++i) x[6 + i] = y[i];
+      T.template subtree<16>(x);
+      f = {{x[0], x[1], x[2]}, {x[3], x[4], x[5]}};
+#pragma unroll
+      for (int i = 0; i < 10; ++i) y[i] = x[6 + i];
+    } else {
+      T.subtree(f);
+    }
     if (live && valid) st<NT>(dot6(col, f), io.tau + e * (io.ld_tau ? io.ld_tau : nv) + u);
   }
-  if (flags & KF_CRBA) {
-    // composite inertia of the subtree (additive form about the world origin)
-    AInertia<float> Ya = additive(world_inertia_fast(J, X.R, X.p));
-    float y[10] = {Ya.m, Ya.h.x, Ya.h.y, Ya.h.z, Ya.I[0], Ya.I[1], Ya.I[2], Ya.I[3], Ya.I[4], Ya.I[5]};
-    if (!valid)
-#pragma unroll
-      for (int i = 0; i < 10; ++i) y[i] = 0.f;
-    T.template subtree<10>(y);
+  if (do_crba) {
+    if (!dyn) T.template subtree<10>(y);
     AInertia<float> Yc;
     Yc.m = y[0];
     Yc.h = {y[1], y[2], y[3]};
@@ -149,11 +165,10 @@ __device__ __forceinline__ void tree_body(const ModelTable<float> *M, const Mode
     // every lane publishes its column and force; lane j then reads those of
     // the other joints (one address across the group at a time) and stores
     // row u of M
-    float *cf = crba + (threadIdx.x & ~(G - 1) & 255) * 12;
-    cf[g * 12 + 0] = col.l.x; cf[g * 12 + 1] = col.l.y; cf[g * 12 + 2] = col.l.z;
-    cf[g * 12 + 3] = col.a.x; cf[g * 12 + 4] = col.a.y; cf[g * 12 + 5] = col.a.z;
-    cf[g * 12 + 6] = Fj.l.x; cf[g * 12 + 7] = Fj.l.y; cf[g * 12 + 8] = Fj.l.z;
-    cf[g * 12 + 9] = Fj.a.x; cf[g * 12 + 10] = Fj.a.y; cf[g * 12 + 11] = Fj.a.z;
+    float4 *cf = crba + (threadIdx.x & ~(G - 1) & 255) * 3;
+    cf[g * 3 + 0] = make_float4(col.l.x, col.l.y, col.l.z, col.a.x);
+    cf[g * 3 + 1] = make_float4(col.a.y, col.a.z, Fj.l.x, Fj.l.y);
+    cf[g * 3 + 2] = make_float4(Fj.l.z, Fj.a.x, Fj.a.y, Fj.a.z);
     wave_sync();
     // branch-free: every load is issued up front, both products computed and
     // selected. M is symmetric (both triangles come from the same dot6(S_a,
@@ -161,13 +176,14 @@ __device__ __forceinline__ void tree_body(const ModelTable<float> *M, const Mode
     // instead of (u, i): at each i the group writes one contiguous row.
     float *Mcol = io.M + e * nv * nv + u;
     const bool out = live && valid;
+    // joint i's subtree size and user dof sit on lane i of every group (one
+    // model per wavefront): read into scalars
 #pragma unroll 4
     for (int i = 0; i < nj; ++i) {
-      const float *ri = cf + i * 12;
-      const S6<float> ci = {{ri[0], ri[1], ri[2]}, {ri[3], ri[4], ri[5]}};
-      const S6<float> Fi = {{ri[6], ri[7], ri[8]}, {ri[9], ri[10], ri[11]}};
-      const JointRec<float> &Ji = joints_of(M)[i];
-      const int sub_i = Ji.subtree, vu_i = Ji.vuser;
+      const float4 r0 = cf[i * 3 + 0], r1 = cf[i * 3 + 1], r2 = cf[i * 3 + 2];
+      const S6<float> ci = {{r0.x, r0.y, r0.z}, {r0.w, r1.x, r1.y}};
+      const S6<float> Fi = {{r1.z, r1.w, r2.x}, {r2.y, r2.z, r2.w}};
+      const int sub_i = __builtin_amdgcn_readlane(T.size, i), vu_i = __builtin_amdgcn_readlane(u, i);
       const float dij = dot6(ci, Fj), dji = dot6(col, Fi);
       const bool i_anc_j = i <= g && g < i + sub_i;
       const bool j_anc_i = g < i && i < g + T.size;
@@ -294,7 +310,7 @@ template <int G>
 __global__ __launch_bounds__(256) void k_tree_fused(const ModelTable<float> *__restrict__ Mg, int64_t B, int flags,
                                                     TreeIO io) {
   __shared__ float4 s_tab[kTreeTableWords4];
-  extern __shared__ float s_crba[];  // crba_lds_bytes(G) when KF_CRBA, else empty
+  extern __shared__ float4 s_crba[];  // crba_lds_bytes(G) when KF_CRBA, else empty
   stage_table(s_tab, Mg);
   __syncthreads();
   const int64_t env = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
@@ -447,7 +463,7 @@ __global__ __launch_bounds__(256) void k_link_state(const ModelTable<float> *__r
 // Heterogeneous launch: block ranges map to segments (one tree type each).
 __global__ __launch_bounds__(256) void k_hetero(HeteroArgs args, int flags) {
   __shared__ float4 s_tab[kTreeTableWords4];
-  extern __shared__ float s_crba[];  // crba_lds_bytes(largest G) when KF_CRBA, else empty
+  extern __shared__ float4 s_crba[];  // crba_lds_bytes(largest G) when KF_CRBA, else empty
   int s = 0;
   while (s + 1 < args.nseg && (int64_t)blockIdx.x >= args.seg[s + 1].block0) ++s;
   const HeteroSegment &S = args.seg[s];
