@@ -125,6 +125,8 @@ struct sapamd_model {
   mutable ClikCont *d_cont = nullptr;  // CLIK hand-over records (grown on demand)
   mutable int64_t cont_cap = 0;
   hipStream_t stream = nullptr;  // the device's shared per-call stream (not owned)
+  // per-call buffer: coherent pinned host memory (h_buf) that the kernels read
+  // and write directly through its device address (d_buf): no staging copies
   double *h_buf = nullptr, *d_buf = nullptr;
   size_t buf_doubles = 0;
   bool tables_ready = false;
@@ -165,7 +167,6 @@ struct sapamd_model {
     if (d_chain) (void)hipFree(d_chain);
     if (d_counter) (void)hipFree(d_counter);
     if (d_cont) (void)hipFree(d_cont);
-    if (d_buf) (void)hipFree(d_buf);
     if (h_buf) (void)hipHostFree(h_buf);
   }
 
@@ -349,12 +350,11 @@ struct sapamd_model {
     size_t n = nv(), l = L();
     size_t need = 8 * n + 7 * l + 12 * n * l + 2 * n * n + 64;
     if (need > buf_doubles) {
-      if (d_buf) HIP_OK(hipFree(d_buf));
       if (h_buf) HIP_OK(hipHostFree(h_buf));
       d_buf = nullptr;
       h_buf = nullptr;
-      HIP_OK(hipMalloc(&d_buf, need * sizeof(double)));
-      HIP_OK(hipHostMalloc(&h_buf, need * sizeof(double), hipHostMallocDefault));
+      HIP_OK(hipHostMalloc(&h_buf, need * sizeof(double), hipHostMallocCoherent));
+      HIP_OK(hipHostGetDevicePointer(reinterpret_cast<void **>(&d_buf), h_buf, 0));
       buf_doubles = need;
     }
     tables_ready = true;
@@ -522,23 +522,26 @@ struct sapamd_model {
     if (len != nv()) throw std::runtime_error(std::string(what) + " must have length nv");
   }
 
-  // stage arrays of n doubles, back to back from `off`, with one copy
+  // arrays of n doubles into the per-call buffer, back to back from `off` (in
+  // doubles); returns their device address. Every per-call method ends in
+  // fetch(), so no kernel still reads the buffer when the next call writes it.
   double *stage_seq(std::initializer_list<const double *> srcs, size_t n, size_t off) {
     size_t k = 0;
     for (const double *src : srcs) std::memcpy(h_buf + off + n * k++, src, n * sizeof(double));
-    HIP_OK(hipMemcpyAsync(d_buf + off, h_buf + off, k * n * sizeof(double), hipMemcpyHostToDevice, stream));
     return d_buf + off;
   }
-  // stage host doubles to the device buffer at `off` (in doubles)
   double *stage(const double *src, size_t n, size_t off) {
     std::memcpy(h_buf + off, src, n * sizeof(double));
-    HIP_OK(hipMemcpyAsync(d_buf + off, h_buf + off, n * sizeof(double), hipMemcpyHostToDevice, stream));
     return d_buf + off;
   }
-  // device buffer at `off` -> the pinned host buffer at the same offset, then
-  // to dst (dst = nullptr: the caller reads h_buf)
+  // host view of a device address inside the per-call buffer
+  template <class T> T *host_of(T *dev) const {
+    return reinterpret_cast<T *>(reinterpret_cast<char *>(h_buf) +
+                                 (reinterpret_cast<char *>(dev) - reinterpret_cast<char *>(d_buf)));
+  }
+  // wait for the per-call kernel, then copy n doubles at `off` to dst (dst =
+  // nullptr: the caller reads h_buf)
   void fetch(double *dst, size_t n, size_t off) {
-    HIP_OK(hipMemcpyAsync(h_buf + off, d_buf + off, n * sizeof(double), hipMemcpyDeviceToHost, stream));
     HIP_OK(hipStreamSynchronize(stream));
     if (dst) std::memcpy(dst, h_buf + off, n * sizeof(double));
   }
@@ -802,8 +805,7 @@ int sapamd_compute_inverse_kinematics(sapamd_model *m, int32_t link, const float
     double *derr = dqo + nv;
     int32_t *dmask = reinterpret_cast<int32_t *>(derr + 8);
     int32_t *dints = dmask + nv + 2;
-    if (nm > 0)
-      HIP_OK(hipMemcpyAsync(dmask, active_qmask, nv * sizeof(int32_t), hipMemcpyHostToDevice, m->stream));
+    if (nm > 0) std::memcpy(m->host_of(dmask), active_qmask, nv * sizeof(int32_t));
     ClikParams P;
     P.link = link;
     P.chain = m->d_chain + m->chain_off[link];
@@ -820,10 +822,7 @@ int sapamd_compute_inverse_kinematics(sapamd_model *m, int32_t link, const float
                                m->stream));
     m->fetch(q_out, nv, 8 + nv);
     m->fetch(err6, 6, 8 + 2 * nv);
-    int32_t ints[2];
-    HIP_OK(hipMemcpyAsync(ints, dints, sizeof(ints), hipMemcpyDeviceToHost, m->stream));
-    HIP_OK(hipStreamSynchronize(m->stream));
-    *success = ints[0];
+    *success = *m->host_of(dints);
   });
 }
 
@@ -1057,7 +1056,6 @@ int sapamd_get_random_qpos(sapamd_model *m, uint64_t seed, double *out) {
     if (sapamd_joint_limits(m, lim.lo, lim.hi) != SAPAMD_OK) throw std::runtime_error(sapamd_last_error());
     DeviceGuard g(m->device);
     HIP_OK(launch_random_qpos<double>(1, nv, seed, lim, m->d_buf, m->stream));
-    HIP_OK(hipMemcpyAsync(m->h_buf, m->d_buf, sizeof(double) * nv, hipMemcpyDeviceToHost, m->stream));
     HIP_OK(hipStreamSynchronize(m->stream));
     for (int u = 0; u < nv; ++u) out[u] = m->h_buf[u];
   });
