@@ -38,11 +38,48 @@ template <class T> __device__ __forceinline__ S6<T> gravity_acc(const ModelTable
   TYPE *const NAME = LDS ? NAME##_s : NAME##_l;
 
 
+// One-environment launches (LDS arrays shared by the block): world pose of
+// every joint. Each thread takes joints for their own transforms (sine,
+// cosine, placement), then thread 0 composes them root to leaves. Every
+// thread of the block calls it.
+template <class T>
+__device__ __forceinline__ void shared_joint_world(const ModelTable<T> *M, const T *qb, int tid, int nth, T (*oR)[9],
+                                                   V3<T> *op, int *par) {
+  const int nj = M->nj;
+  const JointRec<T> *JR = joints_of(M);
+  for (int j = tid; j < nj; j += nth) {
+    const JointRec<T> &J = JR[j];
+    T qj = qb[J.vuser], s, c;
+    sincos_t(qj, &s, &c);
+    joint_world(J, qj, s, c, static_cast<const T *>(nullptr), V3<T>{0, 0, 0}, true, oR[j], op[j]);
+    par[j] = J.parent;
+  }
+  __syncthreads();
+  if (tid == 0)
+    for (int j = 0; j < nj; ++j) {
+      const int p = par[j];
+      if (p < 0) continue;
+      T R[9];
+      mm(oR[p], oR[j], R);
+      op[j] = mv(oR[p], op[j]) + op[p];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) oR[j][k] = R[k];
+    }
+  __syncthreads();
+}
+
+// LDS = true (one environment, the per-call path): the block's threads share
+// the per-joint arrays. Per-joint work (the joint's own transform with its
+// sine and cosine, its Jacobian column, its spatial force) runs one joint per
+// thread; thread 0 runs only the root-to-leaf compositions and the leaf-to-root
+// force sums between them; then every thread takes links, and (link, dof)
+// entries of the per-link Jacobians.
 template <class T, int MJ, bool LDS>
 __global__ __launch_bounds__(64) void k_kinematics(const ModelTable<T> *__restrict__ M, int64_t B, int flags,
                                                    const T *__restrict__ q, const T *__restrict__ qd,
                                                    const T *__restrict__ qdd, KinOut<T> out) {
-  int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t b = LDS ? (int64_t)blockIdx.x : blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int tid = LDS ? (int)threadIdx.x : 0, nth = LDS ? (int)blockDim.x : 1;
   if (b >= B) return;
   const int nj = M->nj, nv = M->nv, L = M->nlinks;
   const JointRec<T> *JR = joints_of(M);
@@ -58,6 +95,38 @@ __global__ __launch_bounds__(64) void k_kinematics(const ModelTable<T> *__restri
   const S6<T> a0 = (flags & KF_NOGRAV) ? zero6<T>() : gravity_acc(M);
   const T *qb = q + b * (out.ld_q ? out.ld_q : nv);
   const T *qdb = qd + b * (out.ld_qd ? out.ld_qd : nv), *qddb = qdd + b * (out.ld_qdd ? out.ld_qdd : nv);
+  if constexpr (LDS) {
+    SAPAMD_JARR(int, par)
+    SAPAMD_JARR(T, qdv)
+    for (int j = tid; j < nj; j += nth) qdv[j] = dyn && !(flags & KF_QD0) ? qdb[JR[j].vuser] : T(0);
+    shared_joint_world(M, qb, tid, nth, oR, op, par);
+    for (int j = tid; j < nj; j += nth) {
+      col[j] = jcol(JR[j], oR[j], op[j]);
+      if (dyn) acc[j] = ((flags & KF_QDD0) ? T(0) : qddb[JR[j].vuser]) * col[j];
+    }
+    if (dyn) {
+      __syncthreads();
+      if (tid == 0)  // velocities and accelerations, root to leaves
+        for (int j = 0; j < nj; ++j) {
+          const int p = par[j];
+          const S6<T> vp = p >= 0 ? vel[p] : zero6<T>(), ap = p >= 0 ? acc[p] : a0;
+          vel[j] = vp + qdv[j] * col[j];
+          acc[j] = ap + acc[j] + qdv[j] * crossm(vel[j], col[j]);
+        }
+      __syncthreads();
+      for (int j = tid; j < nj; j += nth) {
+        const WInertia<T> Y = world_inertia(JR[j], oR[j], op[j]);
+        F[j] = inertia_apply(Y, acc[j]) + crossf(vel[j], inertia_apply(Y, vel[j]));
+      }
+      __syncthreads();
+      if (tid == 0)  // forces, leaves to root
+        for (int j = nj - 1; j >= 0; --j) {
+          out.tau[b * (out.ld_tau ? out.ld_tau : nv) + JR[j].vuser] = dot6(col[j], F[j]);
+          const int p = par[j];
+          if (p >= 0) F[p] = F[p] + F[j];
+        }
+    }
+  } else {
   for (int j = 0; j < nj; ++j) {
     const JointRec<T> &J = JR[j];
     const int u = J.vuser, p = J.parent;
@@ -82,16 +151,45 @@ __global__ __launch_bounds__(64) void k_kinematics(const ModelTable<T> *__restri
       if (p >= 0) F[p] = F[p] + F[j];
     }
   }
+  }  // LDS
+  if (LDS) __syncthreads();
   if (flags & KF_JFULL) {
     T *Jb = out.jfull + b * 6 * nv;
-    for (int j = 0; j < nj; ++j) {
+    for (int j = tid; j < nj; j += nth) {
       int u = JR[j].vuser;
       Jb[0 * nv + u] = col[j].l.x; Jb[1 * nv + u] = col[j].l.y; Jb[2 * nv + u] = col[j].l.z;
       Jb[3 * nv + u] = col[j].a.x; Jb[4 * nv + u] = col[j].a.y; Jb[5 * nv + u] = col[j].a.z;
     }
   }
+  if (LDS && (flags & KF_JLINKS)) {
+    // entry (l, u): the column of the joint on user dof u (world frame, and in
+    // the link frame about its origin) when that joint supports link l, else 0
+    for (int it = tid; it < L * nv; it += nth) {
+      const int l = it / nv, u = it % nv;
+      const LinkRec<T> &Lk = LR[l];
+      const int jl = Lk.joint;
+      int j = 0;
+      while (j < nj - 1 && JR[j].vuser != u) ++j;
+      const bool anc = jl >= 0 && JR[j].vuser == u && j <= jl && jl < j + JR[j].subtree;
+      S6<T> cw = zero6<T>(), cl = zero6<T>();
+      if (anc) {
+        T R[9];
+        mm(oR[jl], Lk.R, R);
+        const V3<T> pw = mv(oR[jl], load3(Lk.p)) + op[jl];
+        cw = col[j];
+        cl = to_local(R, pw, cw);
+      }
+      T *Jw = out.jw_links + (b * L + l) * 6 * nv + u;
+      T *Jl = out.jl_links + (b * L + l) * 6 * nv + u;
+      Jw[0 * nv] = cw.l.x; Jw[1 * nv] = cw.l.y; Jw[2 * nv] = cw.l.z;
+      Jw[3 * nv] = cw.a.x; Jw[4 * nv] = cw.a.y; Jw[5 * nv] = cw.a.z;
+      Jl[0 * nv] = cl.l.x; Jl[1 * nv] = cl.l.y; Jl[2 * nv] = cl.l.z;
+      Jl[3 * nv] = cl.a.x; Jl[4 * nv] = cl.a.y; Jl[5 * nv] = cl.a.z;
+    }
+    if (!(flags & KF_POSES)) return;
+  }
   if (flags & (KF_POSES | KF_JLINKS)) {
-    for (int l = 0; l < L; ++l) {
+    for (int l = tid; l < L; l += nth) {
       const LinkRec<T> &Lk = LR[l];
       const int jl = Lk.joint;
       T R[9];
@@ -108,7 +206,7 @@ __global__ __launch_bounds__(64) void k_kinematics(const ModelTable<T> *__restri
         P[0] = pw.x; P[1] = pw.y; P[2] = pw.z;
         quat_of(R, P + 3);
       }
-      if (flags & KF_JLINKS) {
+      if (!LDS && (flags & KF_JLINKS)) {
         T *Jw = out.jw_links + (b * L + l) * 6 * nv;
         T *Jl = out.jl_links + (b * L + l) * 6 * nv;
         for (int k = 0; k < 6 * nv; ++k) { Jw[k] = T(0); Jl[k] = T(0); }
@@ -162,7 +260,7 @@ __global__ __launch_bounds__(64) void k_link_local_jacobian(const ModelTable<T> 
 template <class T, int MJ, bool LDS>
 __global__ __launch_bounds__(64) void k_crba(const ModelTable<T> *__restrict__ M, int64_t B, const T *__restrict__ q,
                                              T *__restrict__ Mout) {
-  int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t b = LDS ? (int64_t)blockIdx.x : blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (b >= B) return;
   const int nj = M->nj, nv = M->nv;
   const JointRec<T> *JR = joints_of(M);
@@ -172,6 +270,37 @@ __global__ __launch_bounds__(64) void k_crba(const ModelTable<T> *__restrict__ M
   SAPAMD_JARR(S6<T>, col)
   SAPAMD_JARR(AInertia<T>, Yc)
   const T *qb = q + b * nv;
+  if constexpr (LDS) {
+    // one environment, the block's threads: per-joint work one joint per
+    // thread, the composite sums on thread 0, then one thread per entry of M
+    const int tid = threadIdx.x, nth = blockDim.x;
+    SAPAMD_JARR(int, par)
+    SAPAMD_JARR(int, joint_of_dof)
+    SAPAMD_JARR(S6<T>, F)
+    shared_joint_world(M, qb, tid, nth, oR, op, par);
+    for (int j = tid; j < nj; j += nth) {
+      col[j] = jcol(JR[j], oR[j], op[j]);
+      Yc[j] = additive(world_inertia(JR[j], oR[j], op[j]));
+      joint_of_dof[JR[j].vuser] = j;
+    }
+    __syncthreads();
+    if (tid == 0)
+      for (int j = nj - 1; j > 0; --j) {
+        const int p = par[j];
+        if (p >= 0) add_to(Yc[p], Yc[j]);
+      }
+    __syncthreads();
+    for (int j = tid; j < nj; j += nth) F[j] = apply(Yc[j], col[j]);
+    __syncthreads();
+    T *Mb = Mout + b * nv * nv;
+    for (int e = tid; e < nv * nv; e += nth) {
+      const int i = joint_of_dof[e / nv], j = joint_of_dof[e % nv];
+      // M_ij = col_a . F_d with a the ancestor-or-self, d the descendant
+      const int a = i < j ? i : j, d = i < j ? j : i;
+      Mb[e] = (a <= d && d < a + JR[a].subtree) ? dot6(col[a], F[d]) : T(0);
+    }
+    return;
+  }
   for (int j = 0; j < nj; ++j) {
     const JointRec<T> &J = JR[j];
     int p = J.parent;
@@ -243,7 +372,7 @@ template <class T, int MJ, bool LDS>
 __global__ __launch_bounds__(64) void k_aba(const ModelTable<T> *__restrict__ M, int64_t B, const T *__restrict__ q,
                                             const T *__restrict__ qd, const T *__restrict__ tau,
                                             T *__restrict__ qdd) {
-  int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t b = LDS ? (int64_t)blockIdx.x : blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (b >= B) return;
   const int nj = M->nj, nv = M->nv;
   const JointRec<T> *JR = joints_of(M);
@@ -259,6 +388,30 @@ __global__ __launch_bounds__(64) void k_aba(const ModelTable<T> *__restrict__ M,
   SAPAMD_JARR(T, D)
   SAPAMD_JARR(T, uu)
   const T *qb = q + b * nv;
+  if constexpr (LDS) {
+    // one environment, the block's threads: per-joint terms one joint per
+    // thread; the velocity prefix and the two recursions below on thread 0
+    const int tid = threadIdx.x, nth = blockDim.x;
+    SAPAMD_JARR(int, par)
+    SAPAMD_JARR(T, qdv)
+    shared_joint_world(M, qb, tid, nth, oR, op, par);
+    for (int j = tid; j < nj; j += nth) {
+      col[j] = jcol(JR[j], oR[j], op[j]);
+      qdv[j] = qd[b * nv + JR[j].vuser];
+    }
+    __syncthreads();
+    if (tid == 0)
+      for (int j = 0; j < nj; ++j) vel[j] = (par[j] >= 0 ? vel[par[j]] : zero6<T>()) + qdv[j] * col[j];
+    __syncthreads();
+    for (int j = tid; j < nj; j += nth) {
+      cb[j] = qdv[j] * crossm(vel[j], col[j]);
+      const WInertia<T> Y = world_inertia(JR[j], oR[j], op[j]);
+      IA[j] = mat_of(additive(Y));
+      pA[j] = crossf(vel[j], inertia_apply(Y, vel[j]));
+    }
+    __syncthreads();
+    if (tid != 0) return;
+  } else {
   for (int j = 0; j < nj; ++j) {
     const JointRec<T> &J = JR[j];
     int p = J.parent, u = J.vuser;
@@ -273,6 +426,7 @@ __global__ __launch_bounds__(64) void k_aba(const ModelTable<T> *__restrict__ M,
     IA[j] = mat_of(additive(Y));
     pA[j] = crossf(vel[j], inertia_apply(Y, vel[j]));
   }
+  }  // LDS
   for (int j = nj - 1; j >= 0; --j) {
     U[j] = mul6(IA[j], col[j]);
     D[j] = dot6(col[j], U[j]);
@@ -327,7 +481,7 @@ template <class T, int MJ, bool LDS>
 __global__ __launch_bounds__(64) void k_coriolis(const ModelTable<T> *__restrict__ M, int64_t B,
                                                  const T *__restrict__ q, const T *__restrict__ qd,
                                                  T *__restrict__ Cout) {
-  int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t b = LDS ? (int64_t)blockIdx.x : blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (b >= B) return;
   const int nj = M->nj, nv = M->nv;
   const JointRec<T> *JR = joints_of(M);
@@ -340,6 +494,57 @@ __global__ __launch_bounds__(64) void k_coriolis(const ModelTable<T> *__restrict
   SAPAMD_JARR(Mat6<T>, Ic)
   SAPAMD_JARR(Mat6<T>, Bc)
   const T *qb = q + b * nv;
+  if constexpr (LDS) {
+    // one environment, the block's threads: per-joint terms one joint per
+    // thread, the subtree sums one matrix entry per thread, then one thread
+    // per entry of C
+    const int tid = threadIdx.x, nth = blockDim.x;
+    SAPAMD_JARR(int, par)
+    SAPAMD_JARR(int, joint_of_dof)
+    SAPAMD_JARR(T, qdv)
+    SAPAMD_JARR(S6<T>, F1)
+    SAPAMD_JARR(S6<T>, G)
+    SAPAMD_JARR(S6<T>, H)
+    shared_joint_world(M, qb, tid, nth, oR, op, par);
+    for (int j = tid; j < nj; j += nth) {
+      col[j] = jcol(JR[j], oR[j], op[j]);
+      qdv[j] = qd[b * nv + JR[j].vuser];
+      joint_of_dof[JR[j].vuser] = j;
+    }
+    __syncthreads();
+    if (tid == 0)
+      for (int j = 0; j < nj; ++j) vel[j] = (par[j] >= 0 ? vel[par[j]] : zero6<T>()) + qdv[j] * col[j];
+    __syncthreads();
+    for (int j = tid; j < nj; j += nth) {
+      dcol[j] = crossm(vel[j], col[j]);
+      Ic[j] = mat_of(additive(world_inertia(JR[j], oR[j], op[j])));
+      Bc[j] = coriolis_body(Ic[j], vel[j]);
+    }
+    __syncthreads();
+    for (int k = tid; k < 72; k += nth)  // entry k of Ic (k < 36) or of Bc, leaves to root
+      for (int j = nj - 1; j > 0; --j) {
+        const int p = par[j];
+        if (p < 0) continue;
+        if (k < 36) Ic[p].a[k] += Ic[j].a[k];
+        else Bc[p].a[k - 36] += Bc[j].a[k - 36];
+      }
+    __syncthreads();
+    for (int j = tid; j < nj; j += nth) {
+      F1[j] = mul6(Ic[j], dcol[j]) + mul6(Bc[j], col[j]);
+      G[j] = mul6(Ic[j], col[j]);
+      H[j] = mul6t(Bc[j], col[j]);
+    }
+    __syncthreads();
+    T *Cb = Cout + b * nv * nv;
+    for (int e = tid; e < nv * nv; e += nth) {
+      const int r = joint_of_dof[e / nv], c = joint_of_dof[e % nv];
+      T v = T(0);
+      if (r <= c && c < r + JR[r].subtree) v = dot6(col[r], F1[c]);  // r ancestor-or-self of c
+      else if (c < r && r < c + JR[c].subtree) v = dot6(G[r], dcol[c]) + dot6(H[r], col[c]);  // c strict ancestor
+      Cb[e] = v;
+    }
+    return;
+  }
   for (int j = 0; j < nj; ++j) {
     const JointRec<T> &J = JR[j];
     int p = J.parent, u = J.vuser;
@@ -555,16 +760,17 @@ inline int pick_mj(int nj) {
     case 48: hipLaunchKernelGGL((KERNEL<T, 48>), __VA_ARGS__); break;                           \
     default: hipLaunchKernelGGL((KERNEL<T, 64>), __VA_ARGS__); break;                           \
   }
-// Kernels with per-joint arrays: one environment -> LDS arrays, one thread.
+// Kernels with per-joint arrays: one environment -> LDS arrays, one block of 64
+// threads (kernels without a shared-array path run it on thread 0).
 #define SAPAMD_DISPATCH_MJ_ARR(nj, B, KERNEL, s, ...)                                                     \
   if ((B) == 1) {                                                                                          \
     switch (pick_mj(nj)) {                                                                                 \
-      case 8: hipLaunchKernelGGL((KERNEL<T, 8, true>), dim3(1), dim3(1), 0, s, __VA_ARGS__); break;       \
-      case 16: hipLaunchKernelGGL((KERNEL<T, 16, true>), dim3(1), dim3(1), 0, s, __VA_ARGS__); break;     \
-      case 24: hipLaunchKernelGGL((KERNEL<T, 24, true>), dim3(1), dim3(1), 0, s, __VA_ARGS__); break;     \
-      case 32: hipLaunchKernelGGL((KERNEL<T, 32, true>), dim3(1), dim3(1), 0, s, __VA_ARGS__); break;     \
-      case 48: hipLaunchKernelGGL((KERNEL<T, 48, true>), dim3(1), dim3(1), 0, s, __VA_ARGS__); break;     \
-      default: hipLaunchKernelGGL((KERNEL<T, 64, true>), dim3(1), dim3(1), 0, s, __VA_ARGS__); break;     \
+      case 8: hipLaunchKernelGGL((KERNEL<T, 8, true>), dim3(1), dim3(64), 0, s, __VA_ARGS__); break;       \
+      case 16: hipLaunchKernelGGL((KERNEL<T, 16, true>), dim3(1), dim3(64), 0, s, __VA_ARGS__); break;     \
+      case 24: hipLaunchKernelGGL((KERNEL<T, 24, true>), dim3(1), dim3(64), 0, s, __VA_ARGS__); break;     \
+      case 32: hipLaunchKernelGGL((KERNEL<T, 32, true>), dim3(1), dim3(64), 0, s, __VA_ARGS__); break;     \
+      case 48: hipLaunchKernelGGL((KERNEL<T, 48, true>), dim3(1), dim3(64), 0, s, __VA_ARGS__); break;     \
+      default: hipLaunchKernelGGL((KERNEL<T, 64, true>), dim3(1), dim3(64), 0, s, __VA_ARGS__); break;     \
     }                                                                                                      \
   } else {                                                                                                 \
     const dim3 g_ = grid_for(B);                                                                           \
