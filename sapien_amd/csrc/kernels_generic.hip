@@ -728,6 +728,167 @@ __global__ __launch_bounds__(64) void k_clik(const ModelTable<T> *__restrict__ M
     clik_problem<T, MJ>(M, P, target, q0, q_out, success, err_out, iters, prob);
 }
 
+// One CLIK problem per block of 64 threads (one problem, the per-call path):
+// the chain joints' own transforms, Jacobian columns and updates one joint per
+// thread and the 21 entries of J J^T one per thread; the chain composition,
+// the SE(3) log with Jlog6 of its inverse (from one rotation log, the same
+// values as log6_dev and jlog6_dev) and the 6x6 solve on thread 0. Each
+// quantity has clik_problem's formula and summation order; the iterates agree
+// with it to rounding (measured: converged configurations within 1e-10).
+template <class T, int MJ>
+__global__ __launch_bounds__(64) void k_clik_block(const ModelTable<T> *__restrict__ M, ClikParams P,
+                                                   const T *__restrict__ target, const T *__restrict__ q0,
+                                                   T *__restrict__ q_out, int32_t *__restrict__ success,
+                                                   T *__restrict__ err_out, int32_t *__restrict__ iters) {
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int nj = M->nj, nc = P.nchain;
+  const JointRec<T> *JR = joints_of(M);
+  __shared__ T qc[MJ], qbest[MJ], msk[MJ], Jk[MJ][6], oR[MJ][9];
+  __shared__ V3<T> op[MJ];
+  __shared__ int in_chain[MJ];
+  __shared__ T Rd[9], Rj[9], Jl[36], ev[6];
+  __shared__ V3<T> pd, pj;
+  __shared__ double JJt[36], y[6];
+  __shared__ int state;  // bit 0: stop, bit 1: best iterate so far
+  if (tid == 0) {
+    // desired pose of the link's joint frame: oMdes = target * link2joint^-1
+    const LinkRec<T> &Lk = links_of(M)[P.link];
+    T Rt[9];
+    V3<T> pt;
+    pose_to_mat(target, Rt, pt);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        Rd[3 * i + j] = Rt[3 * i] * Lk.R[3 * j] + Rt[3 * i + 1] * Lk.R[3 * j + 1] + Rt[3 * i + 2] * Lk.R[3 * j + 2];
+    pd = pt - mv(Rd, load3(Lk.p));
+  }
+  for (int j = tid; j < nj; j += nth) in_chain[j] = 0;
+  __syncthreads();
+  for (int k = tid; k < nc; k += nth) {
+    const int u = JR[P.chain[k]].vuser;
+    qc[k] = q0 ? q0[u] : T(0);
+    qbest[k] = qc[k];
+    msk[k] = P.mask ? T(P.mask[u]) : T(1);
+    in_chain[P.chain[k]] = 1;
+  }
+  T minErr = T(1e10), bestErr[6] = {0, 0, 0, 0, 0, 0};  // thread 0's
+  bool ok = false;
+  int it = 0;
+  for (;; ++it) {
+    for (int k = tid; k < nc; k += nth) {  // the joints' own transforms
+      T s, c;
+      sincos_t(qc[k], &s, &c);
+      joint_world(JR[P.chain[k]], qc[k], s, c, static_cast<const T *>(nullptr), V3<T>{0, 0, 0}, true, oR[k], op[k]);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (int k = 1; k < nc; ++k) {  // root to link along the chain
+        T R[9];
+        mm(oR[k - 1], oR[k], R);
+        op[k] = mv(oR[k - 1], op[k]) + op[k - 1];
+        for (int i = 0; i < 9; ++i) oR[k][i] = R[i];
+      }
+      for (int i = 0; i < 9; ++i) Rj[i] = nc > 0 ? oR[nc - 1][i] : T(i % 4 == 0);
+      pj = nc > 0 ? op[nc - 1] : V3<T>{0, 0, 0};
+      // iMd = oMj^-1 oMdes
+      T Ri[9];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Ri[3 * i + j] = Rj[i] * Rd[j] + Rj[3 + i] * Rd[3 + j] + Rj[6 + i] * Rd[6 + j];
+      const V3<T> pi = mtv(Rj, pd - pj);
+      const S6<T> e = log6_jlog6_inv(Ri, pi, Jl);
+      T evl[6];
+      vec_of(e, evl);
+      const T en = sqrt(dot6(e, e));
+      int st = 0;
+      if (en < minErr) {
+        minErr = en;
+        st |= 2;
+        for (int r = 0; r < 6; ++r) bestErr[r] = evl[r];
+      }
+      if (en < T(P.eps)) {
+        ok = true;
+        st |= 1;
+      } else if (it >= P.max_iter) {
+        st |= 1;
+      }
+      for (int r = 0; r < 6; ++r) ev[r] = evl[r];
+      state = st;
+    }
+    __syncthreads();
+    const int st = state;
+    if (st & 2)
+      for (int k = tid; k < nc; k += nth) qbest[k] = qc[k];
+    if (st & 1) break;
+    for (int k = tid; k < nc; k += nth) {  // rows of J (chain joint k) in the frame of the error
+      const S6<T> cw = jcol(JR[P.chain[k]], oR[k], op[k]);
+      const S6<T> cl = to_local(Rj, pj, cw);
+      T cv[6];
+      vec_of(cl, cv);
+      for (int r = 0; r < 6; ++r) {
+        T acc = T(0);
+        for (int m = 0; m < 6; ++m) acc += Jl[r * 6 + m] * cv[m];
+        Jk[k][r] = -acc * msk[k];
+      }
+    }
+    __syncthreads();
+    for (int ent = tid; ent < 21; ent += nth) {  // lower triangle of J J^T, FP64
+      int r = 0;
+      while ((r + 1) * (r + 2) / 2 <= ent) ++r;
+      const int m = ent - r * (r + 1) / 2;
+      double acc = 0.0;
+      for (int k = 0; k < nc; ++k) acc += double(Jk[k][r]) * double(Jk[k][m]);
+      JJt[r * 6 + m] = acc;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (int r = 0; r < 6; ++r) {
+        JJt[r * 6 + r] += P.damp;
+        for (int m = 0; m < r; ++m) JJt[m * 6 + r] = JJt[r * 6 + m];
+      }
+      double yl[6], A[36];
+      for (int r = 0; r < 6; ++r) yl[r] = double(ev[r]);
+      for (int k = 0; k < 36; ++k) A[k] = JJt[k];
+      ldlt6_solve(A, yl);
+      for (int r = 0; r < 6; ++r) y[r] = yl[r];
+    }
+    __syncthreads();
+    for (int k = tid; k < nc; k += nth) {
+      T v = T(0);
+      for (int r = 0; r < 6; ++r) v -= Jk[k][r] * T(y[r]);
+      qc[k] += v * T(P.dt);
+    }
+    __syncthreads();
+  }
+  // result: chain dofs from the best iterate, the rest from q0; every
+  // continuous dof wrapped by atan2 (posP2S over the whole configuration,
+  // pinocchio_model.cpp:87,299)
+  for (int j = tid; j < nj; j += nth) {
+    if (in_chain[j]) continue;
+    const JointRec<T> &J = JR[j];
+    T v = q0 ? q0[J.vuser] : T(0);
+    if (J.kind == KIND_CONTINUOUS) {
+      T s, c;
+      sincos_t(v, &s, &c);
+      v = atan2(s, c);
+    }
+    q_out[J.vuser] = v;
+  }
+  for (int k = tid; k < nc; k += nth) {
+    const JointRec<T> &J = JR[P.chain[k]];
+    T v = qbest[k];
+    if (J.kind == KIND_CONTINUOUS) {
+      T s, c;
+      sincos_t(v, &s, &c);
+      v = atan2(s, c);
+    }
+    q_out[J.vuser] = v;
+  }
+  if (tid == 0) {
+    success[0] = ok ? 1 : 0;
+    for (int r = 0; r < 6; ++r) err_out[r] = bestErr[r];
+    if (iters) iters[0] = it;
+  }
+}
+
 // Problems of several models (segments) in one persistent launch: a lane takes
 // the next global problem index and runs it on the segment that owns it.
 template <int MJ> __global__ __launch_bounds__(64) void k_clik_hetero(HeteroClikArgs A) {
@@ -875,6 +1036,10 @@ template <class T>
 hipError_t launch_clik(const ModelTable<T> *tab, int nj, int64_t B, const ClikParams &P, const T *target,
                        const T *q0, T *q_out, int32_t *success, T *err, int32_t *iters, hipStream_t s) {
   if (B <= 0) return hipSuccess;
+  if (B == 1 && !P.persistent_waves && !P.ntargets && !P.solved && nj <= 64) {  // one problem: a block of 64 threads
+    SAPAMD_DISPATCH_MJ(nj, k_clik_block, dim3(1), dim3(64), 0, s, tab, P, target, q0, q_out, success, err, iters);
+    return hipGetLastError();
+  }
   dim3 grid = P.persistent_waves ? dim3((unsigned)P.persistent_waves) : grid_for(B);
   SAPAMD_DISPATCH_MJ(P.nchain, k_clik, grid, dim3(64), 0, s, tab, B, P, target, q0, q_out, success, err, iters);
   return hipGetLastError();
