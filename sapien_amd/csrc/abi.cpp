@@ -1158,7 +1158,6 @@ int sapamd_hetero_batched(const sapamd_model *const *models, int32_t nmodels, co
       s.G = G;
       s.nj = nj;
       s.B = B[i];
-      s.block0 = blocks;
       s.q = q[i];
       s.qd = qd ? qd[i] : nullptr;
       s.qdd = qdd ? qdd[i] : nullptr;
@@ -1166,7 +1165,15 @@ int sapamd_hetero_batched(const sapamd_model *const *models, int32_t nmodels, co
       s.J = J ? J[i] : nullptr;
       s.tau = tau ? tau[i] : nullptr;
       s.M = M ? M[i] : nullptr;
-      blocks += (B[i] * G + 255) / 256;
+    }
+    // blocks of the costliest trees first (most lanes per environment, then
+    // most joints): their wavefronts bound the launch, so they start first
+    std::stable_sort(args.seg, args.seg + args.nseg, [](const HeteroSegment &x, const HeteroSegment &y) {
+      return x.G != y.G ? x.G > y.G : x.nj > y.nj;
+    });
+    for (int i = 0; i < args.nseg; ++i) {
+      args.seg[i].block0 = blocks;
+      blocks += (args.seg[i].B * args.seg[i].G + 255) / 256;
     }
     HIP_OK(launch_hetero(args, blocks, flags, S(stream)));
   });
