@@ -7,8 +7,12 @@ inputs already resident in HBM. Step i uses input/output buffer set i % 64
 launches and a 512 MiB read that evicts the Infinity Cache, the timed region
 is `steps` calls of the C-ABI entry point sapamd_fk_jac_rnea_batched (ctypes,
 device pointers resolved beforehand), bracketed by HIP events on the launch
-stream and by the host clock. `value` = 16384 x steps / host time. Prints ONE
-JSON line.
+stream and by the host clock. `value` = 16384 x steps / host time. Set-up
+launches the same kernel before the warm-up: the shape autotuner, one pass
+over every buffer set (the reference outputs the timed launches are checked
+against) and the launch-mode diagnostics (the first timed sequence after
+set-up ran 20-60 % slower than later ones, whatever the launch mode). Prints
+ONE JSON line.
 
 Beside the headline number it reports:
   roofline      algorithmic bytes per launch / mean launch duration (HIP events
