@@ -95,10 +95,14 @@ __global__ __launch_bounds__(64) void k_kinematics(const ModelTable<T> *__restri
   const S6<T> a0 = (flags & KF_NOGRAV) ? zero6<T>() : gravity_acc(M);
   const T *qb = q + b * (out.ld_q ? out.ld_q : nv);
   const T *qdb = qd + b * (out.ld_qd ? out.ld_qd : nv), *qddb = qdd + b * (out.ld_qdd ? out.ld_qdd : nv);
+  SAPAMD_JARR(int, joint_of_dof)
   if constexpr (LDS) {
     SAPAMD_JARR(int, par)
     SAPAMD_JARR(T, qdv)
-    for (int j = tid; j < nj; j += nth) qdv[j] = dyn && !(flags & KF_QD0) ? qdb[JR[j].vuser] : T(0);
+    for (int j = tid; j < nj; j += nth) {
+      qdv[j] = dyn && !(flags & KF_QD0) ? qdb[JR[j].vuser] : T(0);
+      joint_of_dof[JR[j].vuser] = j;
+    }
     shared_joint_world(M, qb, tid, nth, oR, op, par);
     for (int j = tid; j < nj; j += nth) {
       col[j] = jcol(JR[j], oR[j], op[j]);
@@ -168,9 +172,8 @@ __global__ __launch_bounds__(64) void k_kinematics(const ModelTable<T> *__restri
       const int l = it / nv, u = it % nv;
       const LinkRec<T> &Lk = LR[l];
       const int jl = Lk.joint;
-      int j = 0;
-      while (j < nj - 1 && JR[j].vuser != u) ++j;
-      const bool anc = jl >= 0 && JR[j].vuser == u && j <= jl && jl < j + JR[j].subtree;
+      const int j = joint_of_dof[u];
+      const bool anc = jl >= 0 && j <= jl && jl < j + JR[j].subtree;
       S6<T> cw = zero6<T>(), cl = zero6<T>();
       if (anc) {
         T R[9];
