@@ -99,14 +99,17 @@ __global__ __launch_bounds__(64) void k_kinematics(const ModelTable<T> *__restri
   if constexpr (LDS) {
     SAPAMD_JARR(int, par)
     SAPAMD_JARR(T, qdv)
+    SAPAMD_JARR(T, qddv)
+    // every input read in one pass (q with the joints' own transforms)
     for (int j = tid; j < nj; j += nth) {
       qdv[j] = dyn && !(flags & KF_QD0) ? qdb[JR[j].vuser] : T(0);
+      qddv[j] = dyn && !(flags & KF_QDD0) ? qddb[JR[j].vuser] : T(0);
       joint_of_dof[JR[j].vuser] = j;
     }
     shared_joint_world(M, qb, tid, nth, oR, op, par);
     for (int j = tid; j < nj; j += nth) {
       col[j] = jcol(JR[j], oR[j], op[j]);
-      if (dyn) acc[j] = ((flags & KF_QDD0) ? T(0) : qddb[JR[j].vuser]) * col[j];
+      if (dyn) acc[j] = qddv[j] * col[j];
     }
     if (dyn) {
       __syncthreads();
@@ -390,6 +393,7 @@ __global__ __launch_bounds__(64) void k_aba(const ModelTable<T> *__restrict__ M,
   SAPAMD_JARR(Mat6<T>, IA)
   SAPAMD_JARR(T, D)
   SAPAMD_JARR(T, uu)
+  SAPAMD_JARR(T, tauv)  // the per-call path's torques, read once up front
   const T *qb = q + b * nv;
   if constexpr (LDS) {
     // one environment, the block's threads: per-joint terms one joint per
@@ -397,11 +401,13 @@ __global__ __launch_bounds__(64) void k_aba(const ModelTable<T> *__restrict__ M,
     const int tid = threadIdx.x, nth = blockDim.x;
     SAPAMD_JARR(int, par)
     SAPAMD_JARR(T, qdv)
-    shared_joint_world(M, qb, tid, nth, oR, op, par);
+    // every input read in one pass (q with the joints' own transforms)
     for (int j = tid; j < nj; j += nth) {
-      col[j] = jcol(JR[j], oR[j], op[j]);
       qdv[j] = qd[b * nv + JR[j].vuser];
+      tauv[j] = tau[b * nv + JR[j].vuser];
     }
+    shared_joint_world(M, qb, tid, nth, oR, op, par);
+    for (int j = tid; j < nj; j += nth) col[j] = jcol(JR[j], oR[j], op[j]);
     __syncthreads();
     if (tid == 0)
       for (int j = 0; j < nj; ++j) vel[j] = (par[j] >= 0 ? vel[par[j]] : zero6<T>()) + qdv[j] * col[j];
@@ -433,7 +439,7 @@ __global__ __launch_bounds__(64) void k_aba(const ModelTable<T> *__restrict__ M,
   for (int j = nj - 1; j >= 0; --j) {
     U[j] = mul6(IA[j], col[j]);
     D[j] = dot6(col[j], U[j]);
-    uu[j] = tau[b * nv + JR[j].vuser] - dot6(col[j], pA[j]);
+    uu[j] = (LDS ? tauv[j] : tau[b * nv + JR[j].vuser]) - dot6(col[j], pA[j]);
     int p = JR[j].parent;
     if (p >= 0) {
       T Uv[6];
@@ -508,12 +514,12 @@ __global__ __launch_bounds__(64) void k_coriolis(const ModelTable<T> *__restrict
     SAPAMD_JARR(S6<T>, F1)
     SAPAMD_JARR(S6<T>, G)
     SAPAMD_JARR(S6<T>, H)
-    shared_joint_world(M, qb, tid, nth, oR, op, par);
-    for (int j = tid; j < nj; j += nth) {
-      col[j] = jcol(JR[j], oR[j], op[j]);
+    for (int j = tid; j < nj; j += nth) {  // q' read with q
       qdv[j] = qd[b * nv + JR[j].vuser];
       joint_of_dof[JR[j].vuser] = j;
     }
+    shared_joint_world(M, qb, tid, nth, oR, op, par);
+    for (int j = tid; j < nj; j += nth) col[j] = jcol(JR[j], oR[j], op[j]);
     __syncthreads();
     if (tid == 0)
       for (int j = 0; j < nj; ++j) vel[j] = (par[j] >= 0 ? vel[par[j]] : zero6<T>()) + qdv[j] * col[j];
