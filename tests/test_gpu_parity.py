@@ -72,6 +72,42 @@ def test_dropin_fp64_matches_oracle(tree):
         assert np.allclose(m.compute_coriolis_matrix(q[b], qd[b]), o.coriolis(q[b], qd[b])[0], atol=1e-9)
 
 
+@pytest.mark.parametrize("n", [40, 60])
+def test_dropin_fp64_long_chains_match_oracle(n):
+    """Per-call methods on chains past 32 joints (the 48- and 64-joint
+    one-environment kernels: per-joint work and output entries spread over
+    the block's threads, per-joint arrays in LDS) against the FP64 oracle."""
+    rob = S.chain(n, seed=60 + n)
+    m, o = _pair(rob)
+    q, qd, qdd, tau = S.samples(rob, 2, 2101 + n)
+    for b in range(2):
+        m.compute_forward_kinematics(q[b])
+        Pr = o.link_poses(q[b])[0]
+        for l in range(rob.nlinks):
+            pose = m.get_link_pose(l)
+            assert np.allclose(pose.p, Pr[l, :3], atol=2e-6)
+            assert np.allclose(pose.q, Pr[l, 3:], atol=2e-7) or np.allclose(pose.q, -Pr[l, 3:], atol=2e-7)
+        m.compute_full_jacobian(q[b])
+        for l in (0, rob.nlinks // 2, rob.nlinks - 1):
+            for local in (False, True):
+                assert rel_err(m.get_link_jacobian(l, local), o.link_jacobian(q[b], l, local)[0]) <= 1e-12
+        assert rel_err(m.compute_generalized_mass_matrix(q[b]), o.crba(q[b])[0]) <= 1e-12
+        assert rel_err(m.compute_inverse_dynamics(q[b], qd[b], qdd[b]), o.rnea(q[b], qd[b], qdd[b])[0]) <= 1e-12
+        assert rel_err(m.compute_coriolis_matrix(q[b], qd[b]), o.coriolis(q[b], qd[b])[0]) <= 1e-12
+        assert rel_err(m.compute_forward_dynamics(q[b], qd[b], tau[b]), o.aba(q[b], qd[b], tau[b])[0]) <= 1e-8
+    # IK from 0.05 rad of a solution: both converge, to the same configuration
+    import sapien_amd as sa
+
+    ee = rob.nlinks - 1
+    qs = S.samples(rob, 1, 2201 + n)[0][0]
+    q0 = qs + np.random.default_rng(2202 + n).normal(scale=0.05, size=qs.shape)
+    t7 = o.link_poses(qs)[0, ee].astype(np.float32).astype(np.float64)
+    qo, ok, err = m.compute_inverse_kinematics(ee, sa.Pose(t7[:3], t7[3:]), initial_qpos=q0)
+    qr, okr, errr, _ = o.clik(ee, t7, q0=q0)
+    assert ok and okr
+    assert np.allclose(qo, qr, atol=1e-6)
+
+
 def test_dropin_movo_fixture_reference_semantics(golden_dir):
     """unittest/test_physx/test_articulation.py:75-97 semantics on the movo fixture."""
     import xml.etree.ElementTree as ET
