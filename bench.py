@@ -366,7 +366,7 @@ def main():
         probe_traffic(args.kernel)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build runs on a single MI355X "
+        print("bench.py measures one GPU only: sapien_amd runs on a single MI355X "
               "(the environments shard trivially across GPUs; no multi-GPU run is made).", file=sys.stderr)
         sys.exit(2)
     # the counter passes (child processes) before this process touches the GPU
