@@ -108,6 +108,27 @@ def test_dropin_fp64_long_chains_match_oracle(n):
     assert np.allclose(qo, qr, atol=1e-6)
 
 
+def test_dropin_models_interleaved():
+    """Per-call methods of two models called alternately: each model reads and
+    writes its own pinned per-call buffer, so results and caches stay apart."""
+    ra, rb = S.robot("chain7"), S.robot("torso30")
+    (ma, oa), (mb, ob) = _pair(ra), _pair(rb)
+    qa, qda, qdda, _ = S.samples(ra, 4, 2301)
+    qb, qdb, qddb, _ = S.samples(rb, 4, 2302)
+    for k in range(4):
+        ma.compute_forward_kinematics(qa[k])
+        tb = mb.compute_inverse_dynamics(qb[k], qdb[k], qddb[k])
+        mb.compute_forward_kinematics(qb[k])
+        ta = ma.compute_inverse_dynamics(qa[k], qda[k], qdda[k])
+        Pa, Pb = oa.link_poses(qa[k])[0], ob.link_poses(qb[k])[0]
+        for m, P, n in ((ma, Pa, ra.nlinks), (mb, Pb, rb.nlinks)):
+            for l in range(n):
+                pose = m.get_link_pose(l)
+                assert np.allclose(pose.p, P[l, :3], atol=2e-7)
+        assert np.allclose(ta, oa.rnea(qa[k], qda[k], qdda[k])[0], atol=1e-9)
+        assert np.allclose(tb, ob.rnea(qb[k], qdb[k], qddb[k])[0], atol=1e-9)
+
+
 def test_dropin_movo_fixture_reference_semantics(golden_dir):
     """unittest/test_physx/test_articulation.py:75-97 semantics on the movo fixture."""
     import xml.etree.ElementTree as ET
